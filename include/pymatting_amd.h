@@ -1,0 +1,198 @@
+/* pymatting_amd: C ABI of the MI355X (gfx950) closed-form matting library.
+ *
+ * Plain C types only: device pointers are `void*`/typed pointers obtained
+ * from the caller's allocator (e.g. torch.Tensor.data_ptr()), streams are
+ * `hipStream_t` passed as `void*`. Every function returns an int status
+ * (HM_OK or a negative HM_ERR_*); hm_last_error() returns the message of the
+ * last failure on the calling thread.
+ *
+ * The Python package pymatting_amd binds these with ctypes and mirrors the
+ * reference's Python API (pymatting 1.1.15) on top of them.
+ */
+#ifndef PYMATTING_AMD_H
+#define PYMATTING_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HM_OK 0
+#define HM_ERR_ARG (-1)         /* bad argument (maps to ValueError / TypeError) */
+#define HM_ERR_HIP (-2)         /* HIP runtime failure (RuntimeError) */
+#define HM_ERR_NOCONV (-3)      /* CG did not converge within maxiter (ValueError) */
+#define HM_ERR_UNSUPPORTED (-4) /* valid request this build does not implement */
+#define HM_ERR_VALUE (-5)       /* invalid data, e.g. singular coarse matrix */
+#define HM_ERR_CALLBACK (-6)    /* a host callback reported failure */
+
+/* Precision of vectors and operator arithmetic. */
+#define HM_F64 0
+#define HM_F32 1
+
+/* Operator forms of the level-0 closed-form operator. */
+#define HM_OP_LAPLACIAN 0 /* L (windows with all pixels known skipped if a class map is given) */
+#define HM_OP_LAMBDA 1    /* A = L + lambda * diag(known)  (make_linear_system) */
+#define HM_OP_MASKED 2    /* [L_UU 0; 0 I]: hard constraints of estimate_alpha_cf */
+
+/* Preconditioners. */
+#define HM_PC_NONE 0
+#define HM_PC_JACOBI 1
+#define HM_PC_VCYCLE 2
+#define HM_PC_CSR 3  /* z = M r with a matrix M (cg.py:59-62) */
+#define HM_PC_HOST 4 /* z = M(r) through a host callback (callable M, cg.py:57-58) */
+
+typedef struct hm_cf hm_cf;   /* device image + closed-form window operator */
+typedef struct hm_dia hm_dia; /* stored 25-point grid stencil (coarse levels, ELL form) */
+typedef struct hm_csr hm_csr; /* general CSR matrix (cg on arbitrary matrices) */
+typedef struct hm_vc hm_vc;   /* V-cycle hierarchy */
+
+typedef struct hm_cg_stats {
+    int iterations;     /* completed CG iterations */
+    int converged;      /* 1 if the residual test passed */
+    double norm_b;      /* ||b|| */
+    double norm_r;      /* final recursive residual norm */
+} hm_cg_stats;
+
+const char* hm_last_error(void);
+int hm_version(void);
+
+/* Copies between host and device memory on a stream (kind: 0 = device to
+ * host, 1 = host to device, 2 = device to device); synchronises for kind 0. */
+int hm_copy(void* dst, const void* src, int64_t bytes, int kind, void* stream);
+
+/* ---- trimap split: pymatting/util/util.py:439-525 (trimap_split) ----
+ * cls[i] = 1 if trimap[i] >= fg_threshold, 2 if <= bg_threshold, else 0.
+ * counts[0] = #fg, counts[1] = #bg (device int64[2]). */
+int hm_trimap_classes(const double* trimap, int64_t n, double fg_threshold, double bg_threshold,
+                      uint8_t* cls, int64_t* counts, void* stream);
+
+/* ---- cf_laplacian: pymatting/laplacian/cf_laplacian.py:5-177 ----
+ * image_hwc: device float64 (h, w, 3). cls: nullable device class map.
+ * mode: HM_OP_*. radius must be 1. */
+int hm_cf_create(const double* image_hwc, int h, int w, double epsilon, int radius,
+                 const uint8_t* cls, double lambda_value, int mode, void* stream, hm_cf** out);
+int hm_cf_destroy(hm_cf* op);
+/* Reference CSR layout of cf_laplacian (values (n,5,5), indices, indptr),
+ * entries bit-identical to the reference's serial assembly. */
+int hm_cf_assemble_csr(hm_cf* op, double* values, int64_t* indices, int64_t* indptr,
+                       void* stream);
+/* y = A x with the matrix-free stencil; prec selects float64/float32 vectors. */
+int hm_cf_apply(hm_cf* op, int prec, const void* x, void* y, void* stream);
+/* d = diag(A) (float64). */
+int hm_cf_diagonal(hm_cf* op, double* d, void* stream);
+/* Right-hand side: lambda*is_fg (HM_OP_LAMBDA, make_linear_system,
+ * laplacian.py:46-47) or -(L m)_U with m = is_fg on known pixels (HM_OP_MASKED,
+ * estimate_alpha_cf.py:66-74). */
+int hm_cf_rhs(hm_cf* op, double* b, void* stream);
+
+/* Canonical CSR of the operator's matrix, as scipy leaves (L + diags(c)) after
+ * tocsr() and sum_duplicates() (laplacian.py:49-55): in-image couplings with
+ * a non-zero value, columns ascending. hm_cf_system_indptr fills indptr
+ * (device int64[n+1]) and returns nnz through *nnz; hm_cf_system_fill then
+ * writes indices and values. */
+int hm_cf_system_indptr(hm_cf* op, int64_t* indptr, int64_t* nnz, void* stream);
+int hm_cf_system_fill(hm_cf* op, const int64_t* indptr, int64_t* indices, double* values,
+                      void* stream);
+
+/* ---- stored 25-point operators (ELL/DIA, slot s = (dy+2)*5 + (dx+2)) ----
+ * coef: device float64 [25][h*w] slot-major. */
+int hm_dia_create(const double* coef, int h, int w, void* stream, hm_dia** out);
+int hm_dia_from_cf(hm_cf* op, void* stream, hm_dia** out);
+int hm_dia_destroy(hm_dia* A);
+int hm_dia_apply(hm_dia* A, int prec, const void* x, void* y, void* stream);
+int hm_dia_coefficients(hm_dia* A, double* coef, void* stream);
+
+/* ---- general CSR (cg.py on arbitrary sparse/dense A) ---- */
+int hm_csr_create(int64_t n, int64_t nnz, const int64_t* indptr, const int64_t* indices,
+                  const double* values, hm_csr** out);
+int hm_csr_destroy(hm_csr* A);
+int hm_csr_apply(hm_csr* A, const double* x, double* y, void* stream);
+
+/* ---- vcycle: pymatting/preconditioner/vcycle.py:6-153 ----
+ * Geometric multigrid on the (h, w) grid of the operator: full-weighting
+ * restriction, Galerkin coarse operators R A P, damped Jacobi smoothing and a
+ * dense direct solve once h*w <= direct_solve_size. Exactly one of cf / dia is
+ * non-null. prec: HM_F64, or HM_F32 for float32 coefficients and smoothing. */
+int hm_vcycle_create(hm_cf* cf, hm_dia* dia, int h, int w, int num_pre_iter,
+                     int num_post_iter, double omega, int direct_solve_size, int prec,
+                     void* stream, hm_vc** out);
+/* Direct-solve-only "V-cycle" for a dense n x n device matrix (n <= 2048):
+ * the case h*w <= direct_solve_size of vcycle.py:59-60 for operators that are
+ * not grid stencils. */
+int hm_vcycle_create_dense(const double* A_dense, int n, int prec, void* stream, hm_vc** out);
+int hm_vcycle_destroy(hm_vc* vc);
+int hm_vcycle_levels(hm_vc* vc); /* number of grids including the direct level */
+/* z = V-cycle(r) (vectors in the hierarchy's precision). */
+int hm_vcycle_apply(hm_vc* vc, const void* r, void* z, void* stream);
+
+/* ---- cg: pymatting/solver/cg.py:4-117 ----
+ * Host callbacks receive device pointers to float64 vectors of length n and
+ * return 0 on success. hm_apply_cb computes y = f(x); hm_iter_cb is called
+ * after every iteration's update with the current x and r, before the
+ * convergence test, like cg.py's callback. */
+typedef int (*hm_apply_cb)(void* ctx, const double* x, double* y);
+typedef int (*hm_iter_cb)(void* ctx, const double* x, const double* r, double norm_b,
+                          double norm_r);
+
+typedef struct hm_cg_args {
+    /* operator: exactly one of cf, dia, csr, a_cb (then n gives the size) */
+    hm_cf* cf;
+    hm_dia* dia;
+    hm_csr* csr;
+    hm_apply_cb a_cb;
+    void* a_ctx;
+    int64_t n;
+    /* preconditioner */
+    int pc_kind;              /* HM_PC_* */
+    const double* inv_diag;   /* HM_PC_JACOBI: 1 / diag(A) */
+    hm_vc* vc;                /* HM_PC_VCYCLE (float64 hierarchy) */
+    hm_csr* pc_csr;           /* HM_PC_CSR */
+    hm_apply_cb m_cb;         /* HM_PC_HOST */
+    void* m_ctx;
+    /* vectors (device float64) */
+    const double* b;
+    const double* x0;         /* nullable: start from zeros */
+    double* x;
+    double atol;
+    double rtol;
+    int maxiter;
+    int reorthogonalize;
+    int check_every;          /* iterations enqueued between host checks */
+    hm_iter_cb callback;      /* nullable */
+    void* cb_ctx;
+} hm_cg_args;
+
+/* Preconditioned CG. Returns HM_ERR_NOCONV after maxiter iterations without
+ * convergence; stats are filled in either case. */
+int hm_cg_solve(const hm_cg_args* args, hm_cg_stats* stats, void* stream);
+
+/* Mixed precision: float32 stencil, SpMV and smoothing inside CG with float64
+ * dot products, wrapped in float64 iterative refinement on the residual
+ * b - A x (recomputed with the float64 operator). vc must be HM_F32. */
+int hm_cg_solve_mixed(hm_cf* cf64, hm_vc* vc32, const double* b, double* x, double rtol,
+                      double inner_rtol, int max_outer, int maxiter, hm_cg_stats* stats,
+                      void* stream);
+
+/* z = r * inv_diag (jacobi.py:24-31 applied to a vector). */
+int hm_diag_apply(const double* inv_diag, const double* r, double* z, int64_t n, void* stream);
+
+/* alpha = clip(x, 0, 1) with known pixels (cls != 0) replaced by their trimap
+ * value first (estimate_alpha_cf.py:72-76). cls and trimap may be null. */
+int hm_alpha_finalize(const double* x, const double* trimap, const uint8_t* cls, double* alpha,
+                      int64_t n, void* stream);
+
+/* ---- estimate_foreground_ml: pymatting/foreground/estimate_foreground_ml.py:61-253 ----
+ * image (h0,w0,3) and alpha (h0,w0) device float32 (HM_F32) or float64
+ * (HM_F64); F, B outputs of the same type. level_h/level_w: n_levels+1 host
+ * ints. */
+int hm_fb_ml(const void* image, const void* alpha, int h0, int w0, int n_levels,
+             const int* level_h, const int* level_w, double regularization, int n_small,
+             int n_big, int small_size, double gradient_weight, int prec, void* F, void* B,
+             void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYMATTING_AMD_H */

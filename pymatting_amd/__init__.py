@@ -1,0 +1,27 @@
+"""pymatting_amd: closed-form alpha matting and multilevel foreground estimation
+on AMD Instinct MI355X (gfx950), with pymatting's Python API as a drop-in.
+
+Drop-ins (same names and arguments as pymatting 1.1.15):
+  cf_laplacian, make_linear_system, cg, CounterCallback, ProgressCallback,
+  jacobi, vcycle, estimate_alpha_cf (estimate_alpha), estimate_foreground_ml,
+  trimap_split, sanity_check_image.
+Device-resident API: CfOperator, DiaOperator, LambdaSolver, solve_alpha_lambda.
+
+All numerical work runs in hand-written HIP kernels of libpymatting_amd.so
+(C ABI: include/pymatting_amd.h). There is no CPU fallback.
+"""
+from pymatting_amd.alpha import (LambdaSolver, estimate_alpha, estimate_alpha_cf,
+                                 solve_alpha_lambda)
+from pymatting_amd.foreground import estimate_foreground_ml
+from pymatting_amd.laplacian import (CfLaplacianMatrix, CfOperator, DiaOperator, cf_laplacian,
+                                     make_linear_system)
+from pymatting_amd.preconditioner import ichol, jacobi, vcycle
+from pymatting_amd.solver import CounterCallback, ProgressCallback, cg
+from pymatting_amd.util import sanity_check_image, trimap_split
+
+__all__ = [
+    "cf_laplacian", "make_linear_system", "cg", "CounterCallback", "ProgressCallback", "jacobi",
+    "vcycle", "ichol", "estimate_alpha_cf", "estimate_alpha", "estimate_foreground_ml",
+    "trimap_split", "sanity_check_image", "CfOperator", "DiaOperator", "CfLaplacianMatrix",
+    "LambdaSolver", "solve_alpha_lambda",
+]

@@ -1,0 +1,109 @@
+"""Closed-form alpha estimation on the GPU.
+
+``estimate_alpha_cf`` is the drop-in for pymatting/alpha/estimate_alpha_cf.py:8-78
+(hard constraints: L_U x_U = -R m over the unknown pixels). On the device the
+reduced system is the masked full-grid operator [L_UU 0; 0 I] with right-hand
+side -(R m) on unknown pixels and 0 on known ones, whose solution restricted
+to the unknown pixels is the reference's x_U. Its default preconditioner is
+the V-cycle on that operator (the reference's default, ichol, is a sequential
+factorisation).
+
+``solve_alpha_lambda`` is the lambda-penalised solve the benchmark workloads
+use: A = L + lambda diag(known), b = lambda is_fg, PCG with the V-cycle, clip
+(tests/test_preconditioners.py:24-45, examples/expert_example.py:14-41), in
+float64 or in the mixed float32/float64 form.
+"""
+import time
+
+import numpy as np
+import torch
+
+from pymatting_amd import _ffi
+from pymatting_amd._device import as_device, empty, is_torch
+from pymatting_amd.laplacian import CfOperator
+from pymatting_amd.preconditioner import VCycle
+from pymatting_amd.solver import cg
+from pymatting_amd.util import device_classes, sanity_check_image, trimap_split
+
+
+def _finalize(x, trimap_dev, classes, shape):
+    alpha = empty(x.numel())
+    _ffi.check(_ffi.lib().hm_alpha_finalize(_ffi.ptr(x), _ffi.ptr(trimap_dev), _ffi.ptr(classes),
+                                            _ffi.ptr(alpha), x.numel(), _ffi.stream()))
+    return alpha.reshape(shape)
+
+
+def estimate_alpha_cf(image, trimap, preconditioner=None, laplacian_kwargs={}, cg_kwargs={}):
+    """estimate_alpha_cf.py:8-78: closed-form alpha matting (Levin et al. 2007).
+
+    preconditioner: a factory A -> M (e.g. ``jacobi``, or
+    ``lambda A: vcycle(A, A.grid_shape)``); None selects the V-cycle.
+    Returns alpha in the type of `image` (numpy or torch), shape trimap.shape.
+    """
+    sanity_check_image(image)
+    trimap_split(trimap)  # warnings and ValueErrors of the reference
+    if "is_known" in laplacian_kwargs:
+        raise TypeError("cf_laplacian() got multiple values for keyword argument 'is_known'")
+    classes, tdev = device_classes(trimap)
+    op = CfOperator(image, mode="masked", classes=classes, trimap=tdev, **laplacian_kwargs)
+    b = op.rhs()
+    if preconditioner is None:
+        M = VCycle(op, op.grid_shape)
+    else:
+        M = preconditioner(op)
+    x = cg(op, b, M=M, **cg_kwargs)
+    alpha = _finalize(x, tdev, classes, tuple(trimap.shape))
+    return alpha if is_torch(image) else alpha.cpu().numpy()
+
+
+estimate_alpha = estimate_alpha_cf
+
+
+class LambdaSolver:
+    """Device-resident lambda-penalised alpha solve for one image.
+
+    Set-up (operator, right-hand side, V-cycle hierarchy) happens in the
+    constructor; ``solve()`` runs PCG and returns alpha on the device.
+    precision "f64": float64 PCG with a float64 V-cycle (the plain method).
+    precision "mixed": float32 stencil, SpMV and V-cycle smoothing inside PCG
+    with float64 dot products, wrapped in float64 iterative refinement.
+    """
+
+    def __init__(self, image, trimap, epsilon=1e-7, lambda_value=100.0, precision="f64",
+                 num_pre_iter=1, num_post_iter=1, omega=0.8, direct_solve_size=64):
+        self.h, self.w = int(trimap.shape[0]), int(trimap.shape[1])
+        self.precision = precision
+        self.classes, self.tdev = device_classes(trimap)
+        self.op = CfOperator(image, epsilon, 1, "lambda", self.classes, self.tdev, lambda_value)
+        self.b = self.op.rhs()
+        self.M = VCycle(self.op, (self.h, self.w), num_pre_iter, num_post_iter, omega,
+                        direct_solve_size, "f64" if precision == "f64" else "f32")
+        self.stats = {}
+
+    def solve(self, rtol=1e-7, atol=0.0, maxiter=10000, inner_rtol=1e-4, max_outer=20,
+              check_every=8):
+        import ctypes
+        n = self.h * self.w
+        if self.precision == "f64":
+            x, st = cg(self.op, self.b, atol=atol, rtol=rtol, maxiter=maxiter, M=self.M,
+                       check_every=check_every, return_stats=True)
+            self.stats = st
+        else:
+            x = empty(n)
+            stats = _ffi.CgStats()
+            _ffi.check(_ffi.lib().hm_cg_solve_mixed(
+                self.op.handle, self.M.handle, _ffi.ptr(self.b), _ffi.ptr(x), float(rtol),
+                float(inner_rtol), int(max_outer), int(maxiter), ctypes.byref(stats),
+                _ffi.stream()))
+            self.stats = {"iterations": stats.iterations, "norm_b": stats.norm_b,
+                          "norm_r": stats.norm_r}
+        self.x = x
+        return _finalize(x, self.tdev, None, (self.h, self.w))
+
+
+def solve_alpha_lambda(image, trimap, epsilon=1e-7, lambda_value=100.0, rtol=1e-7,
+                       precision="f64", **kwargs):
+    """cf_laplacian -> make_linear_system -> cg(M=vcycle) -> clip, on the device."""
+    solver = LambdaSolver(image, trimap, epsilon, lambda_value, precision)
+    alpha = solver.solve(rtol=rtol, **kwargs)
+    return alpha, solver.stats

@@ -1,0 +1,361 @@
+// Bit-exact assembly of the closed-form matting Laplacian on gfx950.
+//
+// Reproduces the entries of pymatting/laplacian/cf_laplacian.py:5-129 bit for
+// bit. Each window's mean, covariance and cofactor inverse follow the
+// reference's operation order, and every matrix entry is gathered (no atomics)
+// as the sum over the windows containing both pixels in raster order of the
+// window centre -- the order in which the reference's serial scatter loop adds
+// them. This file is compiled with -ffp-contract=off so no multiply-add is
+// fused, matching the reference's arithmetic.
+#include "common.h"
+#include "internal.h"
+
+namespace hm {
+
+__global__ void k_hwc_to_planar_f64(const double* __restrict__ hwc, double* __restrict__ out,
+                                    long long n) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    out[t] = hwc[3 * t];
+    out[n + t] = hwc[3 * t + 1];
+    out[2 * n + t] = hwc[3 * t + 2];
+}
+
+__global__ void k_planar_to_f32(const double* __restrict__ in, float* __restrict__ out,
+                                long long m) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < m) out[t] = (float)in[t];
+}
+
+__global__ void k_trimap_classes(const double* __restrict__ trimap, long long n, double fg_thr,
+                                 double bg_thr, uint8_t* __restrict__ cls,
+                                 unsigned long long* __restrict__ counts) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int fg = 0, bg = 0;
+    if (t < n) {
+        const double v = trimap[t];
+        fg = v >= fg_thr;  // util.py:509
+        bg = v <= bg_thr;  // util.py:510
+        cls[t] = fg ? 1 : (bg ? 2 : 0);
+    }
+    // A pixel can be both only if fg_thr <= bg_thr; the class byte keeps fg then.
+    unsigned long long bf = __ballot(fg), bb = __ballot(bg);
+    if ((threadIdx.x % kWave) == 0) {
+        if (bf) atomicAdd(counts, (unsigned long long)__popcll(bf));
+        if (bb) atomicAdd(counts + 1, (unsigned long long)__popcll(bb));
+    }
+}
+
+// Per-window data, indexed by the window's centre pixel: mean (s/9) per
+// channel, the six inverse-covariance entries, and a validity flag.
+// Layout [10][n].
+__global__ void k_window_exact(const double* __restrict__ I, int h, int w, double eps,
+                               const uint8_t* __restrict__ cls, int skip_known,
+                               double* __restrict__ wc) {
+    const long long n = (long long)h * w;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int y = (int)(t / w), x = (int)(t - (long long)y * w);
+    double* flag = wc + 9 * n;
+    if (x < 1 || x > w - 2 || y < 1 || y > h - 2) {
+        flag[t] = 0.0;
+        return;
+    }
+    if (skip_known && cls) {
+        bool all_known = true;
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) all_known &= cls[t + (long long)dy * w + dx] != 0;
+        if (all_known) {
+            flag[t] = 0.0;
+            return;
+        }
+    }
+    double mean[3];
+    for (int c = 0; c < 3; ++c) {  // cf_laplacian.py:40-50
+        double s = 0.0;
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) s += I[c * n + t + (long long)dy * w + dx];
+        mean[c] = s / 9;
+    }
+    double a00 = eps, a01 = 0.0, a02 = 0.0, a11 = eps, a12 = 0.0, a22 = eps;
+    for (int dy = -1; dy <= 1; ++dy) {  // cf_laplacian.py:53-67
+        for (int dx = -1; dx <= 1; ++dx) {
+            const long long q = t + (long long)dy * w + dx;
+            const double c0 = I[q] - mean[0], c1 = I[n + q] - mean[1], c2 = I[2 * n + q] - mean[2];
+            a00 += c0 * c0;
+            a01 += c0 * c1;
+            a02 += c0 * c2;
+            a11 += c1 * c1;
+            a12 += c1 * c2;
+            a22 += c2 * c2;
+        }
+    }
+    a00 /= 9; a01 /= 9; a02 /= 9; a11 /= 9; a12 /= 9; a22 /= 9;
+    // cf_laplacian.py:76-92: negated determinant and matching cofactors.
+    const double det = a00 * a12 * a12 + a01 * a01 * a22 + a02 * a02 * a11 - a00 * a11 * a22 -
+                       2 * a01 * a02 * a12;
+    const double inv_det = 1.0 / det;
+    wc[t] = mean[0];
+    wc[n + t] = mean[1];
+    wc[2 * n + t] = mean[2];
+    wc[3 * n + t] = (a12 * a12 - a11 * a22) * inv_det;  // m00
+    wc[4 * n + t] = (a01 * a22 - a02 * a12) * inv_det;  // m01
+    wc[5 * n + t] = (a02 * a11 - a01 * a12) * inv_det;  // m02
+    wc[6 * n + t] = (a02 * a02 - a00 * a22) * inv_det;  // m11
+    wc[7 * n + t] = (a00 * a12 - a01 * a02) * inv_det;  // m12
+    wc[8 * n + t] = (a01 * a01 - a00 * a11) * inv_det;  // m22
+    flag[t] = 1.0;
+}
+
+// L_ij for j = i + (dy, dx), summed over windows in raster order
+// (cf_laplacian.py:95-129). Requires j inside the image.
+__device__ __forceinline__ double entry_exact(const double* __restrict__ I,
+                                              const double* __restrict__ wc, int h, int w,
+                                              int yi, int xi, int yj, int xj) {
+    const long long n = (long long)h * w;
+    const long long i = (long long)yi * w + xi, j = (long long)yj * w + xj;
+    const double Ii0 = I[i], Ii1 = I[n + i], Ii2 = I[2 * n + i];
+    const double Ij0 = I[j], Ij1 = I[n + j], Ij2 = I[2 * n + j];
+    const double delta = (i == j) ? 1.0 : 0.0;
+    double value = 0.0;
+    const int ky0 = max(yi, yj) - 1, ky1 = min(yi, yj) + 1;
+    const int kx0 = max(xi, xj) - 1, kx1 = min(xi, xj) + 1;
+    for (int ky = ky0; ky <= ky1; ++ky) {
+        if (ky < 1 || ky > h - 2) continue;
+        for (int kx = kx0; kx <= kx1; ++kx) {
+            if (kx < 1 || kx > w - 2) continue;
+            const long long k = (long long)ky * w + kx;
+            if (wc[9 * n + k] == 0.0) continue;
+            const double mu0 = wc[k], mu1 = wc[n + k], mu2 = wc[2 * n + k];
+            const double m00 = wc[3 * n + k], m01 = wc[4 * n + k], m02 = wc[5 * n + k];
+            const double m11 = wc[6 * n + k], m12 = wc[7 * n + k], m22 = wc[8 * n + k];
+            const double s = Ii0 - mu0, tt = Ii1 - mu1, u = Ii2 - mu2;
+            const double c0 = m00 * s + m01 * tt + m02 * u;
+            const double c1 = m01 * s + m11 * tt + m12 * u;
+            const double c2 = m02 * s + m12 * tt + m22 * u;
+            const double temp = c0 * (Ij0 - mu0) + c1 * (Ij1 - mu1) + c2 * (Ij2 - mu2);
+            value += delta - (1 + temp) / 9;
+        }
+    }
+    return value;
+}
+
+// Assembles 25 slots per pixel. Output element (i, s) lives at
+// values[i * stride_i + s * stride_s]. mode: HM_OP_* adjustments of the
+// stored operator (lambda on the diagonal of known rows, or the
+// hard-constraint masking). indices/indptr (nullable) follow the reference's
+// CSR layout.
+__global__ void k_assemble_exact(const double* __restrict__ I, const double* __restrict__ wc,
+                                 int h, int w, const uint8_t* __restrict__ cls, int mode,
+                                 double lambda, double* __restrict__ values,
+                                 long long stride_i, long long stride_s,
+                                 int64_t* __restrict__ indices, int64_t* __restrict__ indptr) {
+    const long long n = (long long)h * w;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int yi = (int)(i / w), xi = (int)(i - (long long)yi * w);
+    const bool known_i = cls && cls[i] != 0;
+    for (int s = 0; s < 25; ++s) {
+        const int yj = yi + s / 5 - 2, xj = xi + s % 5 - 2;
+        double v = 0.0;
+        int64_t col = 0;
+        if (yj >= 0 && yj < h && xj >= 0 && xj < w) {
+            col = (int64_t)xj + (int64_t)yj * w;
+            const bool known_j = cls && cls[col] != 0;
+            if (mode == HM_OP_MASKED && known_i) {
+                v = (s == 12) ? 1.0 : 0.0;
+            } else if (mode == HM_OP_MASKED && known_j) {
+                v = 0.0;
+            } else {
+                v = entry_exact(I, wc, h, w, yi, xi, yj, xj);
+                if (mode == HM_OP_LAMBDA && s == 12 && known_i) v = v + lambda;
+            }
+        }
+        values[i * stride_i + s * stride_s] = v;
+        if (indices) indices[i * 25 + s] = col;
+    }
+    if (indptr) {
+        indptr[i + 1] = (i + 1) * 25;
+        if (i == 0) indptr[0] = 0;
+    }
+}
+
+// Diagonal of the mode-adjusted operator (float64 and float32 copies).
+__global__ void k_diag_exact(const double* __restrict__ I, const double* __restrict__ wc, int h,
+                             int w, const uint8_t* __restrict__ cls, int mode, double lambda,
+                             double* __restrict__ d64, float* __restrict__ d32) {
+    const long long n = (long long)h * w;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int yi = (int)(i / w), xi = (int)(i - (long long)yi * w);
+    const bool known_i = cls && cls[i] != 0;
+    double v;
+    if (mode == HM_OP_MASKED && known_i) {
+        v = 1.0;
+    } else {
+        v = entry_exact(I, wc, h, w, yi, xi, yi, xi);
+        if (mode == HM_OP_LAMBDA && known_i) v = v + lambda;
+    }
+    d64[i] = v;
+    if (d32) d32[i] = (float)v;
+}
+
+// Right-hand sides. HM_OP_LAMBDA: b = lambda * is_fg (laplacian.py:46-47).
+// HM_OP_MASKED: b_U = -(R m) with m = is_fg on known pixels and 0 on known
+// rows (estimate_alpha_cf.py:66-74); R's row entries are summed in column
+// order like scipy's CSR product.
+__global__ void k_rhs_exact(const double* __restrict__ I, const double* __restrict__ wc, int h,
+                            int w, const uint8_t* __restrict__ cls, int mode, double lambda,
+                            double* __restrict__ b) {
+    const long long n = (long long)h * w;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (mode == HM_OP_LAMBDA) {
+        b[i] = lambda * (double)(cls[i] == 1);
+        return;
+    }
+    if (cls[i] != 0) {
+        b[i] = 0.0;
+        return;
+    }
+    const int yi = (int)(i / w), xi = (int)(i - (long long)yi * w);
+    double sum = 0.0;
+    for (int s = 0; s < 25; ++s) {
+        const int yj = yi + s / 5 - 2, xj = xi + s % 5 - 2;
+        if (yj < 0 || yj >= h || xj < 0 || xj >= w) continue;
+        const long long j = (long long)yj * w + xj;
+        if (cls[j] == 0) continue;
+        sum += entry_exact(I, wc, h, w, yi, xi, yj, xj) * (double)(cls[j] == 1);
+    }
+    b[i] = -sum;
+}
+
+// Value of slot s of row i of the mode-adjusted operator, or 0 outside the image.
+__device__ __forceinline__ double slot_value(const double* __restrict__ I,
+                                             const double* __restrict__ wc, int h, int w,
+                                             const uint8_t* __restrict__ cls, int mode,
+                                             double lambda, int yi, int xi, int s, bool* inside,
+                                             long long* col) {
+    const int yj = yi + s / 5 - 2, xj = xi + s % 5 - 2;
+    *inside = yj >= 0 && yj < h && xj >= 0 && xj < w;
+    if (!*inside) return 0.0;
+    *col = (long long)xj + (long long)yj * w;
+    const long long i = (long long)yi * w + xi;
+    const bool known_i = cls && cls[i] != 0;
+    const bool known_j = cls && cls[*col] != 0;
+    if (mode == HM_OP_MASKED && known_i) return s == 12 ? 1.0 : 0.0;
+    if (mode == HM_OP_MASKED && known_j) return 0.0;
+    double v = entry_exact(I, wc, h, w, yi, xi, yj, xj);
+    if (mode == HM_OP_LAMBDA && s == 12 && known_i) v = v + lambda;
+    return v;
+}
+
+// Canonical CSR (laplacian.py:49-55): in-image slots with a non-zero value,
+// in slot order, which is ascending column order.
+__global__ void k_system_counts(const double* __restrict__ I, const double* __restrict__ wc,
+                                int h, int w, const uint8_t* __restrict__ cls, int mode,
+                                double lambda, int64_t* __restrict__ counts) {
+    const long long n = (long long)h * w;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int yi = (int)(i / w), xi = (int)(i - (long long)yi * w);
+    int64_t c = 0;
+    for (int s = 0; s < 25; ++s) {
+        bool inside;
+        long long col;
+        const double v = slot_value(I, wc, h, w, cls, mode, lambda, yi, xi, s, &inside, &col);
+        c += (inside && v != 0.0);
+    }
+    counts[i] = c;
+}
+
+__global__ void k_system_fill(const double* __restrict__ I, const double* __restrict__ wc, int h,
+                              int w, const uint8_t* __restrict__ cls, int mode, double lambda,
+                              const int64_t* __restrict__ indptr, int64_t* __restrict__ indices,
+                              double* __restrict__ values) {
+    const long long n = (long long)h * w;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int yi = (int)(i / w), xi = (int)(i - (long long)yi * w);
+    int64_t k = indptr[i];
+    for (int s = 0; s < 25; ++s) {
+        bool inside;
+        long long col;
+        const double v = slot_value(I, wc, h, w, cls, mode, lambda, yi, xi, s, &inside, &col);
+        if (inside && v != 0.0) {
+            indices[k] = col;
+            values[k] = v;
+            ++k;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ launchers
+
+int cf_exact_setup(hm_cf* op, hipStream_t s) {
+    const long long n = op->n;
+    const unsigned g = div_up(n, 256);
+    k_window_exact<<<g, 256, 0, s>>>(op->I64, op->h, op->w, op->eps, op->cls, op->skip_known,
+                                     op->wc);
+    HM_LAUNCH_CHECK();
+    k_diag_exact<<<g, 256, 0, s>>>(op->I64, op->wc, op->h, op->w, op->cls, op->mode, op->lambda,
+                                   op->diag64, op->diag32);
+    HM_LAUNCH_CHECK();
+    return HM_OK;
+}
+
+int cf_upload_image(const double* hwc, hm_cf* op, hipStream_t s) {
+    const long long n = op->n;
+    k_hwc_to_planar_f64<<<div_up(n, 256), 256, 0, s>>>(hwc, op->I64, n);
+    HM_LAUNCH_CHECK();
+    k_planar_to_f32<<<div_up(3 * n, 256), 256, 0, s>>>(op->I64, op->I32, 3 * n);
+    HM_LAUNCH_CHECK();
+    return HM_OK;
+}
+
+int cf_assemble(hm_cf* op, double* values, long long stride_i, long long stride_s,
+                int64_t* indices, int64_t* indptr, int mode, hipStream_t s) {
+    k_assemble_exact<<<div_up(op->n, 256), 256, 0, s>>>(op->I64, op->wc, op->h, op->w, op->cls,
+                                                         mode, op->lambda, values, stride_i,
+                                                         stride_s, indices, indptr);
+    HM_LAUNCH_CHECK();
+    return HM_OK;
+}
+
+int cf_rhs(hm_cf* op, double* b, hipStream_t s) {
+    k_rhs_exact<<<div_up(op->n, 256), 256, 0, s>>>(op->I64, op->wc, op->h, op->w, op->cls,
+                                                    op->mode, op->lambda, b);
+    HM_LAUNCH_CHECK();
+    return HM_OK;
+}
+
+int trimap_classes(const double* trimap, long long n, double fg, double bg, uint8_t* cls,
+                   int64_t* counts, hipStream_t s) {
+    HM_HIP(hipMemsetAsync(counts, 0, 2 * sizeof(int64_t), s));
+    k_trimap_classes<<<div_up(n, 256), 256, 0, s>>>(trimap, n, fg, bg, cls,
+                                                     reinterpret_cast<unsigned long long*>(counts));
+    HM_LAUNCH_CHECK();
+    return HM_OK;
+}
+
+}  // namespace hm
+
+namespace hm {
+
+int cf_system_counts(hm_cf* op, int64_t* counts, hipStream_t s) {
+    k_system_counts<<<div_up(op->n, 256), 256, 0, s>>>(op->I64, op->wc, op->h, op->w, op->cls,
+                                                        op->mode, op->lambda, counts);
+    HM_LAUNCH_CHECK();
+    return HM_OK;
+}
+
+int cf_system_fill(hm_cf* op, const int64_t* indptr, int64_t* indices, double* values,
+                   hipStream_t s) {
+    k_system_fill<<<div_up(op->n, 256), 256, 0, s>>>(op->I64, op->wc, op->h, op->w, op->cls,
+                                                      op->mode, op->lambda, indptr, indices,
+                                                      values);
+    HM_LAUNCH_CHECK();
+    return HM_OK;
+}
+
+}  // namespace hm

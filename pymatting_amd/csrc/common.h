@@ -1,0 +1,155 @@
+// Shared definitions for the pymatting_amd HIP library (gfx950 / CDNA4).
+//
+// Everything here is device-agnostic plumbing: status codes, the thread-local
+// error message, deterministic FP64 block reductions and the device-resident
+// state of a conjugate-gradient solve.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+
+#include "../../include/pymatting_amd.h"
+
+namespace hm {
+
+void set_error(const char* fmt, ...);
+
+constexpr int kWave = 64;
+
+// Status propagation for HIP runtime calls inside functions that return int.
+#define HM_HIP(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            ::hm::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),        \
+                            __FILE__, __LINE__);                                          \
+            return HM_ERR_HIP;                                                            \
+        }                                                                                 \
+    } while (0)
+
+#define HM_TRY(expr)                  \
+    do {                              \
+        int s_ = (expr);              \
+        if (s_ != HM_OK) return s_;   \
+    } while (0)
+
+#define HM_LAUNCH_CHECK() HM_HIP(hipGetLastError())
+
+// ---------------------------------------------------------------------------
+// Device-resident state of one PCG solve. Kernels read the scalars they need
+// (alpha, beta) from here and the final block of each reduction writes them,
+// so an iteration needs no host round trip. `done` freezes every later kernel
+// of the solve once the residual test passed.
+struct CgState {
+    double rz;      // <r, z> of the current iterate
+    double pq;      // <p, A p>
+    double alpha;   // rz / pq
+    double beta;    // (1 / rz_old) * rz_new
+    double rr;      // ||r||^2
+    double norm_b;  // ||b||
+    double atol;
+    double rtol;
+    double aux;     // scratch scalar for one-off reductions
+    int done;       // 1 once ||r|| < atol or ||r|| < rtol * ||b||
+    int iters;      // completed iterations
+    int nonfinite;  // set when a reduction produced NaN/Inf
+    int pad;
+};
+
+// ---------------------------------------------------------------------------
+// Deterministic reductions. Each block reduces its values with wavefront
+// shuffles and an LDS stage, writes one partial per value, and the last block
+// to finish (atomic ticket) sums the partials in a fixed order. The counter is
+// reset by that last block, so no memset is needed between launches.
+struct Reducer {
+    double* partials;   // [N][max_blocks]
+    unsigned* counter;  // one counter per reducer
+    int max_blocks;
+};
+
+template <int N>
+__device__ __forceinline__ void wave_sum(double (&v)[N]) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, kWave);
+    }
+}
+
+// Sum over the block; the result is valid in thread 0.
+template <int BS, int N>
+__device__ __forceinline__ void block_sum(double (&v)[N]) {
+    static_assert(BS % kWave == 0 && BS <= 1024, "block size");
+    constexpr int NW = BS / kWave;
+    __shared__ double red_lds[N][NW];
+    wave_sum<N>(v);
+    const int lane = threadIdx.x % kWave;
+    const int wid = threadIdx.x / kWave;
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) red_lds[k][wid] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            double s = 0.0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) s += red_lds[k][w];
+            v[k] = s;
+        }
+    }
+}
+
+__device__ __forceinline__ double load_coherent(const double* p) {
+    unsigned long long u =
+        __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT);
+    return __longlong_as_double(static_cast<long long>(u));
+}
+
+// Returns true in thread 0 of the last block, with the grid-wide sums in out[].
+template <int BS, int N>
+__device__ bool grid_sum(double (&v)[N], const Reducer& red, double (&out)[N]) {
+    const unsigned nblocks = gridDim.x * gridDim.y * gridDim.z;
+    const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    block_sum<BS, N>(v);
+    __shared__ int is_last;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) red.partials[k * red.max_blocks + bid] = v[k];
+        __threadfence();
+        unsigned ticket = atomicAdd(red.counter, 1u);
+        is_last = (ticket == nblocks - 1);
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    __threadfence();
+    double acc[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        acc[k] = 0.0;
+        for (unsigned i = threadIdx.x; i < nblocks; i += BS)
+            acc[k] += load_coherent(red.partials + k * red.max_blocks + i);
+    }
+    block_sum<BS, N>(acc);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) out[k] = acc[k];
+        atomicExch(red.counter, 0u);
+        return true;
+    }
+    return false;
+}
+
+__device__ __forceinline__ bool solve_done(const CgState* st) {
+    return st != nullptr && __hip_atomic_load(&st->done, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
+inline unsigned div_up(long long a, long long b) { return static_cast<unsigned>((a + b - 1) / b); }
+
+}  // namespace hm

@@ -1,0 +1,309 @@
+// Multigrid set-up kernels and general sparse kernels on gfx950:
+//  - Galerkin coarse operators R A P of 25-point stencils
+//    (pymatting/preconditioner/vcycle.py:62-71, coarse_A = downsample.A.upsample),
+//  - diagonal / float32 copies of stored stencils,
+//  - the dense inverse used by the coarsest direct solve (vcycle.py:59-60),
+//  - CSR SpMV for CG on arbitrary matrices (cg.py:68-71, 85-88).
+#include <cstdarg>
+#include <cstring>
+
+#include "common.h"
+#include "engine.h"
+#include "internal.h"
+
+namespace hm {
+
+// ------------------------------------------------------------ device memory
+
+int dev_alloc(void** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 8;
+    HM_HIP(hipMalloc(p, bytes));
+    return HM_OK;
+}
+
+void dev_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+int make_reducer(Reducer* r, int max_blocks, int nvals) {
+    r->max_blocks = max_blocks;
+    HM_TRY(dev_alloc(reinterpret_cast<void**>(&r->partials),
+                     sizeof(double) * (size_t)max_blocks * nvals));
+    HM_TRY(dev_alloc(reinterpret_cast<void**>(&r->counter), sizeof(unsigned) * 4));
+    HM_HIP(hipMemset(r->counter, 0, sizeof(unsigned) * 4));
+    return HM_OK;
+}
+
+void free_reducer(Reducer* r) {
+    dev_free(r->partials);
+    dev_free(r->counter);
+    r->partials = nullptr;
+    r->counter = nullptr;
+}
+
+// ------------------------------------------------------------ Galerkin R A P
+
+// Pairs (e_a, e_b) of fine offsets around coincident coarse points that keep
+// the fine coupling b - a = 2 D + e_b - e_a inside the 5-point reach, for a
+// coarse offset D in {-2..2}. Counts per D: 1, 6, 9, 6, 1.
+struct PairTable {
+    signed char ea[5][9], eb[5][9];
+    int cnt[5];
+};
+
+__constant__ PairTable c_pairs;
+
+static PairTable build_pairs() {
+    PairTable t;
+    std::memset(&t, 0, sizeof(t));
+    for (int D = -2; D <= 2; ++D) {
+        int k = 0;
+        for (int ea = -1; ea <= 1; ++ea)
+            for (int eb = -1; eb <= 1; ++eb) {
+                int d = 2 * D + eb - ea;
+                if (d >= -2 && d <= 2) {
+                    t.ea[D + 2][k] = (signed char)ea;
+                    t.eb[D + 2][k] = (signed char)eb;
+                    ++k;
+                }
+            }
+        t.cnt[D + 2] = k;
+    }
+    return t;
+}
+
+__device__ __forceinline__ double w1(int e) { return e == 0 ? 0.5 : 0.25; }
+
+// One thread per (coarse point, coarse slot).
+__global__ void k_galerkin(const double* __restrict__ Af, int hf, int wf,
+                           double* __restrict__ Ac, int hc, int wc) {
+    const long long nc = (long long)hc * wc, nf = (long long)hf * wf;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 25 * nc) return;
+    const int s = (int)(t / nc);
+    const long long I = t - (long long)s * nc;
+    const int Y = (int)(I / wc), X = (int)(I - (long long)Y * wc);
+    const int Dy = s / 5 - 2, Dx = s % 5 - 2;
+    double sum = 0.0;
+    if (Y + Dy >= 0 && Y + Dy < hc && X + Dx >= 0 && X + Dx < wc) {
+        const int ny = c_pairs.cnt[Dy + 2], nx = c_pairs.cnt[Dx + 2];
+        for (int py = 0; py < ny; ++py) {
+            const int eay = c_pairs.ea[Dy + 2][py], eby = c_pairs.eb[Dy + 2][py];
+            const int ay = 2 * Y + eay, by = 2 * (Y + Dy) + eby;
+            if (ay < 0 || ay >= hf || by < 0 || by >= hf) continue;
+            const double wy = w1(eay) * w1(eby);
+            const int sy = by - ay + 2;
+            for (int px = 0; px < nx; ++px) {
+                const int eax = c_pairs.ea[Dx + 2][px], ebx = c_pairs.eb[Dx + 2][px];
+                const int ax = 2 * X + eax, bx = 2 * (X + Dx) + ebx;
+                if (ax < 0 || ax >= wf || bx < 0 || bx >= wf) continue;
+                const int sx = bx - ax + 2;
+                const long long a = (long long)ay * wf + ax;
+                sum += wy * (w1(eax) * w1(ebx)) * Af[(sy * 5 + sx) * nf + a];
+            }
+        }
+    }
+    Ac[t] = sum;
+}
+
+int dia_galerkin(const hm_dia* fine, hm_dia* coarse, hipStream_t s) {
+    static bool pairs_ready = false;
+    if (!pairs_ready) {
+        PairTable t = build_pairs();
+        HM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_pairs), &t, sizeof(t)));
+        pairs_ready = true;
+    }
+    const long long total = 25 * coarse->n;
+    k_galerkin<<<div_up(total, 256), 256, 0, s>>>(fine->c64, fine->h, fine->w, coarse->c64,
+                                                  coarse->h, coarse->w);
+    HM_LAUNCH_CHECK();
+    return HM_OK;
+}
+
+__global__ void k_dia_finish(const double* __restrict__ c64, float* __restrict__ c32,
+                             double* __restrict__ d64, float* __restrict__ d32, long long n) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 25 * n) return;
+    const double v = c64[t];
+    c32[t] = (float)v;
+    if (t >= 12 * n && t < 13 * n) {
+        d64[t - 12 * n] = v;
+        d32[t - 12 * n] = (float)v;
+    }
+}
+
+int dia_finish(hm_dia* A, hipStream_t s) {
+    k_dia_finish<<<div_up(25 * A->n, 256), 256, 0, s>>>(A->c64, A->c32, A->diag64, A->diag32,
+                                                        A->n);
+    HM_LAUNCH_CHECK();
+    return HM_OK;
+}
+
+// ------------------------------------------------------- coarsest direct solve
+
+// [A | I] (n x 2n) from a stored 25-point stencil or from a dense matrix.
+__global__ void k_aug_from_dia(const double* __restrict__ coef, int h, int w,
+                               double* __restrict__ M) {
+    const int n = h * w, n2 = 2 * n;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n * n2) return;
+    const int r = (int)(t / n2), c = (int)(t - (long long)r * n2);
+    double v = 0.0;
+    if (c < n) {
+        const int ry = r / w, rx = r % w, cy = c / w, cx = c % w;
+        const int dy = cy - ry, dx = cx - rx;
+        if (dy >= -2 && dy <= 2 && dx >= -2 && dx <= 2)
+            v = coef[(long long)((dy + 2) * 5 + dx + 2) * n + r];
+    } else {
+        v = (c - n == r) ? 1.0 : 0.0;
+    }
+    M[t] = v;
+}
+
+__global__ void k_aug_from_dense(const double* __restrict__ A, int n, double* __restrict__ M) {
+    const int n2 = 2 * n;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n * n2) return;
+    const int r = (int)(t / n2), c = (int)(t - (long long)r * n2);
+    M[t] = c < n ? A[(long long)r * n + c] : ((c - n == r) ? 1.0 : 0.0);
+}
+
+// Gauss-Jordan elimination with partial pivoting on [A | I] (global memory),
+// one workgroup. Writes the inverse in float64 and float32.
+__global__ __launch_bounds__(1024) void k_gauss_jordan(int n, double* __restrict__ M,
+                                                       double* __restrict__ inv64,
+                                                       float* __restrict__ inv32,
+                                                       int* __restrict__ status) {
+    const int n2 = 2 * n;
+    __shared__ double fac[kMaxDirect];
+    __shared__ double best_v[1024];
+    __shared__ int best_i[1024];
+    for (int k = 0; k < n; ++k) {
+        double bv = -1.0;
+        int bi = k;
+        for (int r = k + threadIdx.x; r < n; r += blockDim.x) {
+            const double a = fabs(M[(long long)r * n2 + k]);
+            if (a > bv) { bv = a; bi = r; }
+        }
+        best_v[threadIdx.x] = bv;
+        best_i[threadIdx.x] = bi;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int t = 1; t < (int)blockDim.x; ++t)
+                if (best_v[t] > best_v[0]) { best_v[0] = best_v[t]; best_i[0] = best_i[t]; }
+            if (!(best_v[0] > 0.0)) *status = 1;
+        }
+        __syncthreads();
+        const int piv = best_i[0];
+        if (piv != k)
+            for (int c = threadIdx.x; c < n2; c += blockDim.x) {
+                const double t0 = M[(long long)k * n2 + c];
+                M[(long long)k * n2 + c] = M[(long long)piv * n2 + c];
+                M[(long long)piv * n2 + c] = t0;
+            }
+        __syncthreads();
+        const double inv_p = 1.0 / M[(long long)k * n2 + k];
+        __syncthreads();
+        for (int c = threadIdx.x; c < n2; c += blockDim.x) M[(long long)k * n2 + c] *= inv_p;
+        for (int r = threadIdx.x; r < n; r += blockDim.x) fac[r] = M[(long long)r * n2 + k];
+        __syncthreads();
+        for (long long t = threadIdx.x; t < (long long)n * n2; t += blockDim.x) {
+            const int r = (int)(t / n2), c = (int)(t - (long long)r * n2);
+            if (r != k) M[t] -= fac[r] * M[(long long)k * n2 + c];
+        }
+        __syncthreads();
+    }
+    for (long long t = threadIdx.x; t < (long long)n * n; t += blockDim.x) {
+        const int r = (int)(t / n), c = (int)(t - (long long)r * n);
+        const double v = M[(long long)r * n2 + n + c];
+        inv64[t] = v;
+        inv32[t] = (float)v;
+    }
+}
+
+static int gauss_jordan(int n, double* M, double* inv64, float* inv32, hipStream_t s) {
+    int* status = nullptr;
+    HM_TRY(dev_alloc(reinterpret_cast<void**>(&status), sizeof(int)));
+    HM_HIP(hipMemsetAsync(status, 0, sizeof(int), s));
+    k_gauss_jordan<<<1, 1024, 0, s>>>(n, M, inv64, inv32, status);
+    HM_LAUNCH_CHECK();
+    int host_status = 0;
+    HM_HIP(hipMemcpyAsync(&host_status, status, sizeof(int), hipMemcpyDeviceToHost, s));
+    HM_HIP(hipStreamSynchronize(s));
+    dev_free(status);
+    if (host_status) {
+        set_error("coarsest-grid matrix is singular");
+        return HM_ERR_VALUE;
+    }
+    return HM_OK;
+}
+
+int dense_inverse(const hm_dia* A, double* inv64, float* inv32, hipStream_t s) {
+    const int n = (int)A->n;
+    if (n > kMaxDirect) {
+        set_error("direct solve of a %d-point coarsest grid exceeds the %d-point limit", n,
+                  kMaxDirect);
+        return HM_ERR_UNSUPPORTED;
+    }
+    double* M = nullptr;
+    HM_TRY(dev_alloc(reinterpret_cast<void**>(&M), sizeof(double) * (size_t)n * 2 * n));
+    k_aug_from_dia<<<div_up((long long)n * 2 * n, 256), 256, 0, s>>>(A->c64, A->h, A->w, M);
+    HM_LAUNCH_CHECK();
+    int rc = gauss_jordan(n, M, inv64, inv32, s);
+    dev_free(M);
+    return rc;
+}
+
+int dense_inverse_from_dense(const double* A, int n, double* inv64, float* inv32, hipStream_t s) {
+    if (n > kMaxDirect) {
+        set_error("direct solve of a %d x %d matrix exceeds the %d limit", n, n, kMaxDirect);
+        return HM_ERR_UNSUPPORTED;
+    }
+    double* M = nullptr;
+    HM_TRY(dev_alloc(reinterpret_cast<void**>(&M), sizeof(double) * (size_t)n * 2 * n));
+    k_aug_from_dense<<<div_up((long long)n * 2 * n, 256), 256, 0, s>>>(A, n, M);
+    HM_LAUNCH_CHECK();
+    int rc = gauss_jordan(n, M, inv64, inv32, s);
+    dev_free(M);
+    return rc;
+}
+
+// ----------------------------------------------------------------------- CSR
+
+__global__ __launch_bounds__(kBlock) void k_csr_apply(const int64_t* __restrict__ indptr,
+                                                      const int64_t* __restrict__ indices,
+                                                      const double* __restrict__ values,
+                                                      long long n, const double* __restrict__ x,
+                                                      double* __restrict__ y, CgState* st,
+                                                      Reducer red, int kind) {
+    if (solve_done(st)) return;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double acc = 0.0;
+    if (i < n) {
+        double s = 0.0;
+        for (int64_t k = indptr[i]; k < indptr[i + 1]; ++k) s += values[k] * x[indices[k]];
+        y[i] = s;
+        acc = x[i] * s;
+    }
+    if (st != nullptr && red.partials != nullptr) {
+        double v[1] = {acc}, tot[1];
+        if (grid_sum<kBlock, 1>(v, red, tot)) dot_finalize(st, kind, tot[0]);
+    }
+}
+
+int csr_apply(const hm_csr* A, const double* x, double* y, CgState* st, Reducer* red, int kind,
+              hipStream_t s) {
+    const unsigned g = div_up(A->n, kBlock);
+    Reducer r = red ? *red : Reducer{};
+    if (red && (int)g > red->max_blocks) {
+        set_error("reducer too small");
+        return HM_ERR_ARG;
+    }
+    k_csr_apply<<<g, kBlock, 0, s>>>(A->indptr, A->indices, A->values, A->n, x, y, red ? st : nullptr,
+                                     r, kind);
+    HM_LAUNCH_CHECK();
+    return HM_OK;
+}
+
+}  // namespace hm

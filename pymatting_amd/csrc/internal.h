@@ -1,0 +1,106 @@
+// Host-side object layouts and launcher declarations shared by the library's
+// translation units.
+#pragma once
+
+#include <vector>
+
+#include "common.h"
+
+// Device image + closed-form window operator (level 0 of every solve).
+struct hm_cf {
+    int h = 0, w = 0;
+    long long n = 0;
+    double eps = 1e-7;
+    double lambda = 0.0;
+    int mode = HM_OP_LAPLACIAN;
+    int skip_known = 0;
+    double* I64 = nullptr;   // planar [3][n]
+    float* I32 = nullptr;    // planar [3][n]
+    uint8_t* cls = nullptr;  // owned copy of the class map, or null
+    double* wc = nullptr;    // exact per-window data [10][n]
+    double* diag64 = nullptr;
+    float* diag32 = nullptr;
+};
+
+// Stored 25-point stencil (slot-major [25][n]) with its diagonal.
+struct hm_dia {
+    int h = 0, w = 0;
+    long long n = 0;
+    double* c64 = nullptr;
+    float* c32 = nullptr;
+    double* diag64 = nullptr;
+    float* diag32 = nullptr;
+};
+
+struct hm_csr {
+    long long n = 0, nnz = 0;
+    int64_t* indptr = nullptr;
+    int64_t* indices = nullptr;
+    double* values = nullptr;
+};
+
+namespace hm {
+
+// One grid of the V-cycle hierarchy. Level 0 borrows the caller's operator;
+// coarser levels own a Galerkin stencil. Vectors are in the hierarchy's
+// precision (double or float, stored as raw bytes).
+struct VcLevel {
+    int h = 0, w = 0;
+    long long n = 0;
+    hm_dia* A = nullptr;   // owned for levels >= 1 (or level 0 given as DIA: borrowed)
+    bool owns_A = false;
+    void* b = nullptr;     // right-hand side (restricted residual); level 0 uses the input
+    void* x = nullptr;     // correction / smoothed iterate
+    void* t = nullptr;     // scratch for extra Jacobi sweeps
+    void* u = nullptr;     // scratch for extra Jacobi sweeps
+};
+
+}  // namespace hm
+
+struct hm_vc {
+    int prec = HM_F64;
+    int pre = 1, post = 1;
+    double omega = 0.8;
+    int direct_size = 64;
+    hm_cf* cf = nullptr;  // level-0 operator when matrix-free
+    std::vector<hm::VcLevel> levels;  // last level is solved directly
+    double* dense_inv64 = nullptr;    // inverse of the coarsest matrix (n_c x n_c)
+    float* dense_inv32 = nullptr;
+    int n_direct = 0;
+    hm::Reducer red{};
+};
+
+namespace hm {
+
+// cf_exact.hip
+int cf_upload_image(const double* hwc, hm_cf* op, hipStream_t s);
+int cf_exact_setup(hm_cf* op, hipStream_t s);
+int cf_assemble(hm_cf* op, double* values, long long stride_i, long long stride_s,
+                int64_t* indices, int64_t* indptr, int mode, hipStream_t s);
+int cf_rhs(hm_cf* op, double* b, hipStream_t s);
+int cf_system_counts(hm_cf* op, int64_t* counts, hipStream_t s);
+int cf_system_fill(hm_cf* op, const int64_t* indptr, int64_t* indices, double* values,
+                   hipStream_t s);
+
+// scan.hip: out[0] = 0, out[k + 1] = sum(in[0..k]); returns the total.
+int exclusive_offsets(const int64_t* in, int64_t* out, long long n, int64_t* total,
+                      hipStream_t s);
+int trimap_classes(const double* trimap, long long n, double fg, double bg, uint8_t* cls,
+                   int64_t* counts, hipStream_t s);
+
+// grid.hip
+int dia_galerkin(const hm_dia* fine, hm_dia* coarse, hipStream_t s);
+int dia_finish(hm_dia* A, hipStream_t s);  // diagonal + float32 copies
+constexpr int kMaxDirect = 2048;  // largest coarsest grid solved by the dense inverse
+int dense_inverse(const hm_dia* A, double* inv64, float* inv32, hipStream_t s);
+int dense_inverse_from_dense(const double* A, int n, double* inv64, float* inv32, hipStream_t s);
+int csr_apply(const hm_csr* A, const double* x, double* y, CgState* st, Reducer* red, int kind,
+              hipStream_t s);
+
+// Device memory helpers (hipMalloc-backed).
+int dev_alloc(void** p, size_t bytes);
+void dev_free(void* p);
+int make_reducer(Reducer* r, int max_blocks, int nvals);
+void free_reducer(Reducer* r);
+
+}  // namespace hm

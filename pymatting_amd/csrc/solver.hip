@@ -1,0 +1,1329 @@
+// Closed-form operators, V-cycle and PCG drivers, and the C ABI.
+//
+// The PCG follows pymatting/solver/cg.py:4-117 with all scalars on the
+// device: the SpMV kernel's last block forms alpha = rz / <p, Ap>, the update
+// kernel's last block tests ||r|| < atol or ||r|| < rtol ||b|| and raises a
+// `done` flag that turns every later kernel of the solve into a no-op, and
+// the preconditioner's last kernel forms beta = (1 / rz_old) * rz_new. The
+// host enqueues `check_every` iterations at a time and reads the state once
+// per chunk.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+
+#include "common.h"
+#include "engine.h"
+#include "internal.h"
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+namespace hm {
+
+void set_error(const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+}
+
+// ------------------------------------------------------------- launch helpers
+
+template <typename T, class Op, class In, class Out>
+int launch_tile(const Op& op, const In& in, const Out& out, int h, int w, int h2, int w2,
+                const CgState* guard, hipStream_t s) {
+    const dim3 grid = tile_grid(out, h, w, h2, w2);
+    if (out.dot.st != nullptr && (long long)grid.x * grid.y > out.dot.red.max_blocks) {
+        set_error("reduction buffer too small for %u x %u blocks", grid.x, grid.y);
+        return HM_ERR_ARG;
+    }
+    tile_kernel<T, Op, In, Out><<<grid, kBlock, 0, s>>>(op, in, out, h, w, guard);
+    HM_LAUNCH_CHECK();
+    return HM_OK;
+}
+
+template <typename T>
+struct LevelOp {
+    bool is_cf = false;
+    CfOp<T> cf{};
+    DiaOp<T> dia{};
+    const T* D = nullptr;
+    int h = 0, w = 0;
+};
+
+template <typename T>
+const T* pick(const double* a, const float* b);
+template <>
+const double* pick<double>(const double* a, const float*) { return a; }
+template <>
+const float* pick<float>(const double*, const float* b) { return b; }
+
+template <typename T>
+LevelOp<T> cf_level(const hm_cf* op, int force_mode = -1) {
+    LevelOp<T> L;
+    L.is_cf = true;
+    L.cf.d = CfDev{op->h, op->w, op->mode, op->skip_known, op->eps, op->lambda, op->cls};
+    L.cf.I = pick<T>(op->I64, op->I32);
+    L.cf.force_mode = force_mode;
+    L.D = pick<T>(op->diag64, op->diag32);
+    L.h = op->h;
+    L.w = op->w;
+    return L;
+}
+
+template <typename T>
+LevelOp<T> dia_level(const hm_dia* A) {
+    LevelOp<T> L;
+    L.is_cf = false;
+    L.dia = DiaOp<T>{A->h, A->w, pick<T>(A->c64, A->c32)};
+    L.D = pick<T>(A->diag64, A->diag32);
+    L.h = A->h;
+    L.w = A->w;
+    return L;
+}
+
+template <typename T, class In, class Out>
+int run_level(const LevelOp<T>& L, const In& in, const Out& out, const CgState* guard,
+              hipStream_t s) {
+    if (L.is_cf)
+        return launch_tile<T>(L.cf, in, out, L.h, L.w, L.h / 2, L.w / 2, guard, s);
+    return launch_tile<T>(L.dia, in, out, L.h, L.w, L.h / 2, L.w / 2, guard, s);
+}
+
+// ------------------------------------------------------------- vector kernels
+
+constexpr int kVecPerThread = 4;
+
+inline unsigned vec_blocks(long long n) { return div_up(n, (long long)kBlock * kVecPerThread); }
+
+// x += alpha p; r -= alpha q; ||r||^2 and the convergence test (cg.py:90-100).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_update(T* __restrict__ x, const T* __restrict__ p,
+                                                      T* __restrict__ r, const T* __restrict__ q,
+                                                      long long n, CgState* st, Reducer red) {
+    if (solve_done(st)) return;
+    const T alpha = (T)st->alpha;
+    double acc = 0.0;
+    const long long base = (long long)blockIdx.x * kBlock * kVecPerThread + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kVecPerThread; ++k) {
+        const long long i = base + (long long)k * kBlock;
+        if (i < n) {
+            x[i] += alpha * p[i];
+            const T ri = r[i] - alpha * q[i];
+            r[i] = ri;
+            acc += (double)ri * (double)ri;
+        }
+    }
+    double v[1] = {acc}, tot[1];
+    if (grid_sum<kBlock, 1>(v, red, tot)) {
+        const double rr = tot[0];
+        const double norm_r = sqrt(rr);
+        st->rr = rr;
+        st->iters += 1;
+        if (!isfinite(rr)) st->nonfinite = 1;
+        if (norm_r < st->atol || norm_r < st->rtol * st->norm_b) st->done = 1;
+    }
+}
+
+// p = z + beta p (cg.py:112-113).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_pupdate(T* __restrict__ p, const T* __restrict__ z,
+                                                       long long n, const CgState* st) {
+    if (solve_done(st)) return;
+    const T beta = (T)st->beta;
+    const long long base = (long long)blockIdx.x * kBlock * kVecPerThread + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kVecPerThread; ++k) {
+        const long long i = base + (long long)k * kBlock;
+        if (i < n) p[i] = p[i] * beta + z[i];
+    }
+}
+
+// z = r * inv_diag (jacobi.py:24-31), or z = r without a preconditioner; <r, z>.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_diag_pc(const T* __restrict__ r,
+                                                    const double* __restrict__ inv_diag,
+                                                    T* __restrict__ z, long long n, CgState* st,
+                                                    Reducer red, int kind) {
+    if (solve_done(st)) return;
+    double acc = 0.0;
+    const long long base = (long long)blockIdx.x * kBlock * kVecPerThread + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kVecPerThread; ++k) {
+        const long long i = base + (long long)k * kBlock;
+        if (i < n) {
+            const T zi = inv_diag ? (T)((double)r[i] * inv_diag[i]) : r[i];
+            z[i] = zi;
+            acc += (double)r[i] * (double)zi;
+        }
+    }
+    if (st != nullptr) {
+        double v[1] = {acc}, tot[1];
+        if (grid_sum<kBlock, 1>(v, red, tot)) dot_finalize(st, kind, tot[0]);
+    }
+}
+
+// <a, b> (and <a - c, b> when c is given, for the Polak-Ribiere beta of
+// cg.py:104-106) into the state.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dot(const T* __restrict__ a, const T* __restrict__ b,
+                                                const T* __restrict__ c, long long n, CgState* st,
+                                                Reducer red, int kind) {
+    if (kind != kDotAux && solve_done(st)) return;
+    double acc[2] = {0.0, 0.0};
+    const long long base = (long long)blockIdx.x * kBlock * kVecPerThread + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kVecPerThread; ++k) {
+        const long long i = base + (long long)k * kBlock;
+        if (i < n) {
+            acc[0] += (double)a[i] * (double)b[i];
+            if (c) acc[1] += ((double)a[i] - (double)c[i]) * (double)b[i];
+        }
+    }
+    double tot[2];
+    if (grid_sum<kBlock, 2>(acc, red, tot)) {
+        if (c) {  // reorthogonalised beta
+            st->beta = tot[1] / st->rz;
+            st->rz = tot[0];
+        } else {
+            dot_finalize(st, kind, tot[0]);
+        }
+    }
+}
+
+template <typename T>
+__global__ void k_copy(const T* __restrict__ a, T* __restrict__ b, long long n,
+                       const CgState* st) {
+    if (solve_done(st)) return;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+template <typename T>
+__global__ void k_fill(T* __restrict__ a, T v, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// out = base + P e with base = omega b / D (pre = 1), a stored vector, or 0.
+template <typename T, class Base>
+__global__ void k_prolong_add(Base base, InProlong<T, Base> pro, T* __restrict__ out, int h,
+                              int w, const CgState* st) {
+    if (solve_done(st)) return;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)h * w) return;
+    const int y = (int)(i / w), x = (int)(i - (long long)y * w);
+    out[i] = pro.load(x, y, i);
+}
+
+// x = inv(A_c) b on the coarsest grid (n <= 1024), optional <b, x>.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dense_apply(const T* __restrict__ inv,
+                                                        const T* __restrict__ b,
+                                                        T* __restrict__ x, int n, CgState* st,
+                                                        Reducer red, int kind,
+                                                        const CgState* guard) {
+    if (solve_done(guard)) return;
+    double acc = 0.0;
+    for (int i = threadIdx.x + blockIdx.x * kBlock; i < n; i += kBlock * gridDim.x) {
+        T s = 0;
+        for (int j = 0; j < n; ++j) s += inv[(long long)i * n + j] * b[j];
+        x[i] = s;
+        acc += (double)b[i] * (double)s;
+    }
+    if (st != nullptr) {
+        double v[1] = {acc}, tot[1];
+        if (grid_sum<kBlock, 1>(v, red, tot)) dot_finalize(st, kind, tot[0]);
+    }
+}
+
+// Mixed-precision helpers.
+__global__ void k_scale_to_f32(const double* __restrict__ a, double scale, float* __restrict__ b,
+                               long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)(a[i] * scale);
+}
+
+__global__ void k_axpy_f32_to_f64(double* __restrict__ x, double scale,
+                                  const float* __restrict__ e, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += scale * (double)e[i];
+}
+
+// ------------------------------------------------------------------ V-cycle
+
+template <typename T>
+T* vptr(void* p) { return static_cast<T*>(p); }
+
+template <typename T>
+LevelOp<T> vc_level_op(const hm_vc* vc, size_t l) {
+    if (l == 0 && vc->cf) return cf_level<T>(vc->cf);
+    return dia_level<T>(vc->levels[l].A);
+}
+
+template <typename T>
+const T* vc_inv(const hm_vc* vc) { return pick<T>(vc->dense_inv64, vc->dense_inv32); }
+
+template <typename T, class Base>
+int post_stage(const LevelOp<T>& op, const Base& base, const T* b, const T* e, int h2, int w2,
+               int post, T omega, T* target, T* other, DotSink fin, const T* r_top,
+               const CgState* guard, hipStream_t s, const Reducer& red) {
+    DotSink nodot{nullptr, Reducer{}, 0};
+    InProlong<T, Base> pro{base, e, w2, h2};
+    if (post == 0) {
+        k_prolong_add<T, Base><<<div_up((long long)op.h * op.w, 256), 256, 0, s>>>(
+            base, pro, target, op.h, op.w, guard);
+        HM_LAUNCH_CHECK();
+        if (fin.st != nullptr) {
+            const long long n = (long long)op.h * op.w;
+            k_dot<T><<<vec_blocks(n), kBlock, 0, s>>>(r_top, target, nullptr, n, fin.st, red,
+                                                      fin.kind);
+            HM_LAUNCH_CHECK();
+        }
+        return HM_OK;
+    }
+    const T* prev = nullptr;
+    for (int k = 0; k < post; ++k) {
+        T* outp = ((post - 1 - k) % 2 == 0) ? target : other;
+        const DotSink ds = (k == post - 1) ? fin : nodot;
+        OutSmooth<T> sm{b, op.D, omega, outp, ds};
+        if (k == 0)
+            HM_TRY(run_level<T>(op, pro, sm, guard, s));
+        else
+            HM_TRY(run_level<T>(op, InVec<T>{prev}, sm, guard, s));
+        prev = outp;
+    }
+    return HM_OK;
+}
+
+// z = V-cycle(r) (vcycle.py:46-100). If st is given, the final kernel also
+// forms <r, z> and finalises it as `kind`.
+template <typename T>
+int vcycle_apply_t(hm_vc* vc, const T* r, T* z, CgState* st, int kind, const CgState* guard,
+                   hipStream_t s) {
+    const size_t L = vc->levels.size();
+    const T omega = (T)vc->omega;
+    const DotSink nodot{nullptr, Reducer{}, 0};
+    const DotSink dot{st, vc->red, kind};
+    if (L == 1) {
+        k_dense_apply<T><<<1, kBlock, 0, s>>>(vc_inv<T>(vc), r, z, vc->n_direct, st, vc->red,
+                                              kind, guard);
+        HM_LAUNCH_CHECK();
+        return HM_OK;
+    }
+    std::vector<T*> result(L, nullptr), pre_x(L, nullptr);
+    // Descend: pre-smooth, residual, restrict.
+    for (size_t l = 0; l + 1 < L; ++l) {
+        const VcLevel& lev = vc->levels[l];
+        const VcLevel& cl = vc->levels[l + 1];
+        const LevelOp<T> op = vc_level_op<T>(vc, l);
+        const T* b = (l == 0) ? r : vptr<T>(lev.b);
+        const OutRestrict<T> rest{b, vptr<T>(cl.b), cl.w, cl.h, nullptr, nodot};
+        if (vc->pre == 0) {
+            HM_TRY(run_level<T>(op, InZero<T>{}, rest, guard, s));
+        } else if (vc->pre == 1) {
+            HM_TRY(run_level<T>(op, InJac0<T>{b, op.D, omega}, rest, guard, s));
+        } else {
+            T* cur = vptr<T>(lev.x);
+            T* nxt = vptr<T>(lev.t);
+            HM_TRY(run_level<T>(op, InJac0<T>{b, op.D, omega},
+                                OutSmooth<T>{b, op.D, omega, cur, nodot}, guard, s));
+            for (int k = 2; k < vc->pre; ++k) {
+                HM_TRY(run_level<T>(op, InVec<T>{cur}, OutSmooth<T>{b, op.D, omega, nxt, nodot},
+                                    guard, s));
+                std::swap(cur, nxt);
+            }
+            HM_TRY(run_level<T>(op, InVec<T>{cur}, rest, guard, s));
+            pre_x[l] = cur;
+        }
+    }
+    // Coarsest grid: direct solve (vcycle.py:59-60).
+    {
+        const VcLevel& cl = vc->levels[L - 1];
+        k_dense_apply<T><<<1, kBlock, 0, s>>>(vc_inv<T>(vc), vptr<T>(cl.b), vptr<T>(cl.x),
+                                              vc->n_direct, nullptr, Reducer{}, 0, guard);
+        HM_LAUNCH_CHECK();
+        result[L - 1] = vptr<T>(cl.x);
+    }
+    // Ascend: prolongate the correction and post-smooth (vcycle.py:95-100).
+    for (size_t l = L - 1; l-- > 0;) {
+        const VcLevel& lev = vc->levels[l];
+        const VcLevel& cl = vc->levels[l + 1];
+        const LevelOp<T> op = vc_level_op<T>(vc, l);
+        const T* b = (l == 0) ? r : vptr<T>(lev.b);
+        const bool top = (l == 0);
+        T* avail[2];
+        int na = 0;
+        for (void* cand : {lev.x, lev.t, lev.u})
+            if (cand != nullptr && vptr<T>(cand) != pre_x[l] && na < 2) avail[na++] = vptr<T>(cand);
+        if (!top && na < 1) {
+            set_error("internal: missing V-cycle work vectors");
+            return HM_ERR_ARG;
+        }
+        if (vc->post >= 2 && na < (top ? 1 : 2)) {
+            set_error("internal: missing V-cycle work vectors");
+            return HM_ERR_ARG;
+        }
+        T* target = top ? z : avail[0];
+        T* other = top ? (na > 0 ? avail[0] : nullptr) : (na > 1 ? avail[1] : nullptr);
+        const DotSink fin = (top && st) ? dot : nodot;
+        const T* e = result[l + 1];
+        int rc;
+        if (vc->pre == 0)
+            rc = post_stage<T>(op, InZero<T>{}, b, e, cl.h, cl.w, vc->post, omega, target, other,
+                               fin, r, guard, s, vc->red);
+        else if (vc->pre == 1)
+            rc = post_stage<T>(op, InJac0<T>{b, op.D, omega}, b, e, cl.h, cl.w, vc->post, omega,
+                               target, other, fin, r, guard, s, vc->red);
+        else
+            rc = post_stage<T>(op, InVec<T>{pre_x[l]}, b, e, cl.h, cl.w, vc->post, omega,
+                               target, other, fin, r, guard, s, vc->red);
+        HM_TRY(rc);
+        result[l] = target;
+    }
+    return HM_OK;
+}
+
+int vcycle_apply(hm_vc* vc, const void* r, void* z, CgState* st, int kind, const CgState* guard,
+                 hipStream_t s) {
+    if (vc->prec == HM_F64)
+        return vcycle_apply_t<double>(vc, static_cast<const double*>(r), static_cast<double*>(z),
+                                      st, kind, guard, s);
+    return vcycle_apply_t<float>(vc, static_cast<const float*>(r), static_cast<float*>(z), st,
+                                 kind, guard, s);
+}
+
+}  // namespace hm
+
+// ===================================================================== PCG
+
+namespace hm {
+
+struct OpHandle {
+    const hm_cf* cf = nullptr;
+    const hm_dia* dia = nullptr;
+    const hm_csr* csr = nullptr;
+    hm_apply_cb cb = nullptr;  // host callback operator (float64 only)
+    void* cb_ctx = nullptr;
+    long long cb_n = 0;
+    long long n() const { return cf ? cf->n : (dia ? dia->n : (csr ? csr->n : cb_n)); }
+    int h() const { return cf ? cf->h : (dia ? dia->h : 1); }
+    int w() const { return cf ? cf->w : (dia ? dia->w : (int)n()); }
+};
+
+struct PcHandle {
+    int kind = HM_PC_NONE;
+    const double* inv_diag = nullptr;
+    hm_vc* vc = nullptr;
+    const hm_csr* csr = nullptr;
+    hm_apply_cb cb = nullptr;
+    void* cb_ctx = nullptr;
+};
+
+int reducer_blocks(long long n, int h, int w) {
+    long long a = (long long)div_up(w, 64) * div_up(h, 8);
+    long long b = vec_blocks(n);
+    long long c = div_up(n, kBlock);
+    long long m = a > b ? a : b;
+    m = m > c ? m : c;
+    return (int)(m + 64);
+}
+
+// Runs a host callback on device vectors: the stream is drained first so the
+// callback sees finished data, and the callback's own copies complete before
+// it returns.
+inline int host_call(hm_apply_cb cb, void* ctx, const double* x, double* y, hipStream_t s) {
+    HM_HIP(hipStreamSynchronize(s));
+    if (cb(ctx, x, y) != 0) {
+        set_error("host callback failed");
+        return HM_ERR_CALLBACK;
+    }
+    return HM_OK;
+}
+
+template <typename T>
+int op_apply(const OpHandle& A, const T* x, T* y, const DotSink& ds, const CgState* guard,
+             hipStream_t s) {
+    if (A.cf) return run_level<T>(cf_level<T>(A.cf), InVec<T>{x}, OutApply<T>{y, ds}, guard, s);
+    if (A.dia)
+        return run_level<T>(dia_level<T>(A.dia), InVec<T>{x}, OutApply<T>{y, ds}, guard, s);
+    if constexpr (std::is_same<T, double>::value) {
+        if (A.csr) {
+            Reducer red = ds.red;
+            return csr_apply(A.csr, x, y, ds.st, ds.st ? &red : nullptr, ds.kind, s);
+        }
+        HM_TRY(host_call(A.cb, A.cb_ctx, x, y, s));
+        if (ds.st) {
+            const long long n = A.n();
+            k_dot<double><<<vec_blocks(n), kBlock, 0, s>>>(x, y, nullptr, n, ds.st, ds.red,
+                                                           ds.kind);
+            HM_LAUNCH_CHECK();
+        }
+        return HM_OK;
+    }
+    set_error("CSR and callback operators are float64 only");
+    return HM_ERR_UNSUPPORTED;
+}
+
+__global__ void k_sub_from(const double* __restrict__ b, double* __restrict__ r, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) r[i] = b[i] - r[i];
+}
+
+// r = b - A x.
+template <typename T>
+int op_residual(const OpHandle& A, const T* b, const T* x, T* r, hipStream_t s) {
+    const DotSink nodot{nullptr, Reducer{}, 0};
+    if (A.cf)
+        return run_level<T>(cf_level<T>(A.cf), InVec<T>{x}, OutResidual<T>{b, r, nodot}, nullptr,
+                            s);
+    if (A.dia)
+        return run_level<T>(dia_level<T>(A.dia), InVec<T>{x}, OutResidual<T>{b, r, nodot},
+                            nullptr, s);
+    if constexpr (std::is_same<T, double>::value) {
+        HM_TRY(op_apply<double>(A, x, r, nodot, nullptr, s));
+        k_sub_from<<<div_up(A.n(), 256), 256, 0, s>>>(b, r, A.n());
+        HM_LAUNCH_CHECK();
+        return HM_OK;
+    }
+    set_error("CSR and callback operators are float64 only");
+    return HM_ERR_UNSUPPORTED;
+}
+
+template <typename T>
+int pc_apply(const PcHandle& M, const T* r, T* z, CgState* st, const Reducer& red, int kind,
+             const CgState* guard, long long n, hipStream_t s) {
+    if (M.kind == HM_PC_VCYCLE) return vcycle_apply(M.vc, r, z, st, kind, guard, s);
+    if (M.kind == HM_PC_CSR || M.kind == HM_PC_HOST) {
+        if constexpr (std::is_same<T, double>::value) {
+            if (M.kind == HM_PC_CSR)
+                HM_TRY(csr_apply(M.csr, r, z, nullptr, nullptr, 0, s));
+            else
+                HM_TRY(host_call(M.cb, M.cb_ctx, r, z, s));
+            if (st) {
+                k_dot<double><<<vec_blocks(n), kBlock, 0, s>>>(r, z, nullptr, n, st, red, kind);
+                HM_LAUNCH_CHECK();
+            }
+            return HM_OK;
+        }
+        set_error("matrix and callback preconditioners are float64 only");
+        return HM_ERR_UNSUPPORTED;
+    }
+    k_diag_pc<T><<<vec_blocks(n), kBlock, 0, s>>>(r, M.kind == HM_PC_JACOBI ? M.inv_diag : nullptr,
+                                                  z, n, st, red, kind);
+    HM_LAUNCH_CHECK();
+    return HM_OK;
+}
+
+struct DeviceState {
+    CgState* dev = nullptr;
+    CgState* host = nullptr;  // pinned
+    int init() {
+        HM_TRY(dev_alloc(reinterpret_cast<void**>(&dev), sizeof(CgState)));
+        HM_HIP(hipHostMalloc(reinterpret_cast<void**>(&host), sizeof(CgState), hipHostMallocDefault));
+        std::memset(host, 0, sizeof(CgState));
+        return HM_OK;
+    }
+    ~DeviceState() {
+        dev_free(dev);
+        if (host) (void)hipHostFree(host);
+    }
+    int push(hipStream_t s) {
+        HM_HIP(hipMemcpyAsync(dev, host, sizeof(CgState), hipMemcpyHostToDevice, s));
+        return HM_OK;
+    }
+    int pull(hipStream_t s) {
+        HM_HIP(hipMemcpyAsync(host, dev, sizeof(CgState), hipMemcpyDeviceToHost, s));
+        HM_HIP(hipStreamSynchronize(s));
+        return HM_OK;
+    }
+};
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { dev_free(p); }
+    int alloc(size_t bytes) { return dev_alloc(&p, bytes); }
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+struct PcgOptions {
+    double atol = 0.0, rtol = 1e-7;
+    int maxiter = 10000;
+    int reorth = 0;
+    int check_every = 8;
+    hm_iter_cb callback = nullptr;
+    void* cb_ctx = nullptr;
+};
+
+// Preconditioned CG (cg.py:4-117) on device vectors of type T with float64
+// scalars. Returns HM_ERR_NOCONV if maxiter iterations did not converge;
+// stats are filled in either case. With host callbacks (callable A or M, or a
+// per-iteration callback) the state is read after every iteration.
+template <typename T>
+int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
+          const PcgOptions& opt, hm_cg_stats* stats, hipStream_t s) {
+    const long long n = A.n();
+    const int maxiter = opt.maxiter < 0 ? 0 : opt.maxiter;
+    DevBuf r, p, q, z, rold;
+    HM_TRY(r.alloc(sizeof(T) * n));
+    HM_TRY(p.alloc(sizeof(T) * n));
+    HM_TRY(q.alloc(sizeof(T) * n));
+    HM_TRY(z.alloc(sizeof(T) * n));
+    if (opt.reorth) HM_TRY(rold.alloc(sizeof(T) * n));
+    Reducer red{};
+    HM_TRY(make_reducer(&red, reducer_blocks(n, A.h(), A.w()), 2));
+    struct RedGuard { Reducer* r; ~RedGuard() { free_reducer(r); } } rg{&red};
+    DeviceState st;
+    HM_TRY(st.init());
+    st.host->atol = opt.atol;
+    st.host->rtol = opt.rtol;
+    HM_TRY(st.push(s));
+    const unsigned vb = vec_blocks(n);
+    const bool interactive = A.cb || M.kind == HM_PC_HOST || opt.callback;
+
+    // norm_b = ||b|| (cg.py:66)
+    k_dot<T><<<vb, kBlock, 0, s>>>(b, b, nullptr, n, st.dev, red, kDotAux);
+    HM_LAUNCH_CHECK();
+    HM_TRY(st.pull(s));
+    const double norm_b = std::sqrt(st.host->aux);
+    // x = x0 or 0; r = b - A x (cg.py:64-71)
+    if (x0) {
+        k_copy<T><<<div_up(n, 256), 256, 0, s>>>(x0, x, n, nullptr);
+        HM_LAUNCH_CHECK();
+        HM_TRY(op_residual<T>(A, b, x, r.as<T>(), s));
+    } else {
+        k_fill<T><<<div_up(n, 256), 256, 0, s>>>(x, T(0), n);
+        HM_LAUNCH_CHECK();
+        if (A.cb) {  // the reference evaluates A(x) even for x = 0
+            HM_TRY(op_residual<T>(A, b, x, r.as<T>(), s));
+        } else {
+            k_copy<T><<<div_up(n, 256), 256, 0, s>>>(b, r.as<T>(), n, nullptr);
+            HM_LAUNCH_CHECK();
+        }
+    }
+    k_dot<T><<<vb, kBlock, 0, s>>>(r.as<T>(), r.as<T>(), nullptr, n, st.dev, red, kDotAux);
+    HM_LAUNCH_CHECK();
+    HM_TRY(st.pull(s));
+    const double norm_r = std::sqrt(st.host->aux);
+    stats->norm_b = norm_b;
+    stats->norm_r = norm_r;
+    stats->iterations = 0;
+    stats->converged = 0;
+    if (norm_r < opt.atol || norm_r < opt.rtol * norm_b) {  // cg.py:75-76
+        stats->converged = 1;
+        return HM_OK;
+    }
+    st.host->norm_b = norm_b;
+    st.host->rr = norm_r * norm_r;
+    HM_TRY(st.push(s));
+    // z = M r; p = z; rz = <r, z> (cg.py:78-80)
+    HM_TRY(pc_apply<T>(M, r.as<T>(), z.as<T>(), st.dev, red, kDotRzFirst, nullptr, n, s));
+    k_copy<T><<<div_up(n, 256), 256, 0, s>>>(z.as<T>(), p.as<T>(), n, nullptr);
+    HM_LAUNCH_CHECK();
+
+    const int check_every = interactive ? 1 : (opt.check_every < 1 ? 1 : opt.check_every);
+    int enqueued = 0;
+    const DotSink pq{st.dev, red, kDotPQ};
+    while (enqueued < maxiter) {
+        const int k = (maxiter - enqueued < check_every) ? (maxiter - enqueued) : check_every;
+        for (int it = 0; it < k; ++it) {
+            if (opt.reorth) {
+                k_copy<T><<<div_up(n, 256), 256, 0, s>>>(r.as<T>(), rold.as<T>(), n, st.dev);
+                HM_LAUNCH_CHECK();
+            }
+            HM_TRY(op_apply<T>(A, p.as<T>(), q.as<T>(), pq, st.dev, s));
+            k_cg_update<T><<<vb, kBlock, 0, s>>>(x, p.as<T>(), r.as<T>(), q.as<T>(), n, st.dev,
+                                                 red);
+            HM_LAUNCH_CHECK();
+            if (interactive) {
+                HM_TRY(st.pull(s));
+                if constexpr (std::is_same<T, double>::value) {
+                    if (opt.callback &&
+                        opt.callback(opt.cb_ctx, x, r.as<T>(), norm_b, std::sqrt(st.host->rr)) != 0) {
+                        set_error("iteration callback failed");
+                        return HM_ERR_CALLBACK;
+                    }
+                }
+                if (st.host->done || st.host->nonfinite) break;
+            }
+            if (opt.reorth) {
+                HM_TRY(pc_apply<T>(M, r.as<T>(), z.as<T>(), nullptr, red, 0, st.dev, n, s));
+                k_dot<T><<<vb, kBlock, 0, s>>>(r.as<T>(), z.as<T>(), rold.as<T>(), n, st.dev, red,
+                                               kDotRzUpdate);
+                HM_LAUNCH_CHECK();
+            } else {
+                HM_TRY(pc_apply<T>(M, r.as<T>(), z.as<T>(), st.dev, red, kDotRzUpdate, st.dev, n,
+                                   s));
+            }
+            k_cg_pupdate<T><<<vb, kBlock, 0, s>>>(p.as<T>(), z.as<T>(), n, st.dev);
+            HM_LAUNCH_CHECK();
+        }
+        enqueued += k;
+        HM_TRY(st.pull(s));
+        if (st.host->done || st.host->nonfinite) break;
+    }
+    stats->iterations = st.host->iters;
+    stats->norm_r = std::sqrt(st.host->rr);
+    stats->converged = st.host->done;
+    if (!st.host->done) {
+        set_error("Conjugate gradient descent did not converge within %d iterations", maxiter);
+        return HM_ERR_NOCONV;
+    }
+    return HM_OK;
+}
+
+// Mixed precision: float32 inner PCG (float32 stencil, SpMV and V-cycle
+// smoothing; float64 dot products) inside float64 iterative refinement:
+//   r = b - A x (float64), e = A32^-1 (r / ||r||), x += ||r|| e.
+int pcg_mixed(hm_cf* cf, hm_vc* vc32, const double* b, double* x, double rtol,
+              double inner_rtol, int max_outer, int maxiter, hm_cg_stats* stats, hipStream_t s) {
+    const long long n = cf->n;
+    DevBuf r64, r32, e32;
+    HM_TRY(r64.alloc(sizeof(double) * n));
+    HM_TRY(r32.alloc(sizeof(float) * n));
+    HM_TRY(e32.alloc(sizeof(float) * n));
+    Reducer red{};
+    HM_TRY(make_reducer(&red, reducer_blocks(n, cf->h, cf->w), 2));
+    struct RedGuard { Reducer* r; ~RedGuard() { free_reducer(r); } } rg{&red};
+    DeviceState st;
+    HM_TRY(st.init());
+    HM_TRY(st.push(s));
+    const unsigned vb = vec_blocks(n);
+    OpHandle A;
+    A.cf = cf;
+    PcHandle M;
+    M.kind = HM_PC_VCYCLE;
+    M.vc = vc32;
+
+    k_dot<double><<<vb, kBlock, 0, s>>>(b, b, nullptr, n, st.dev, red, kDotAux);
+    HM_LAUNCH_CHECK();
+    HM_TRY(st.pull(s));
+    const double norm_b = std::sqrt(st.host->aux);
+    k_fill<double><<<div_up(n, 256), 256, 0, s>>>(x, 0.0, n);
+    HM_LAUNCH_CHECK();
+    k_copy<double><<<div_up(n, 256), 256, 0, s>>>(b, r64.as<double>(), n, nullptr);
+    HM_LAUNCH_CHECK();
+    int total = 0;
+    stats->norm_b = norm_b;
+    stats->converged = 0;
+    for (int outer = 0; outer <= max_outer; ++outer) {
+        k_dot<double><<<vb, kBlock, 0, s>>>(r64.as<double>(), r64.as<double>(), nullptr, n,
+                                            st.dev, red, kDotAux);
+        HM_LAUNCH_CHECK();
+        HM_TRY(st.pull(s));
+        const double norm_r = std::sqrt(st.host->aux);
+        stats->norm_r = norm_r;
+        if (norm_r < rtol * norm_b) {
+            stats->converged = 1;
+            break;
+        }
+        if (outer == max_outer || total >= maxiter || !std::isfinite(norm_r)) break;
+        k_scale_to_f32<<<div_up(n, 256), 256, 0, s>>>(r64.as<double>(), 1.0 / norm_r,
+                                                      r32.as<float>(), n);
+        HM_LAUNCH_CHECK();
+        hm_cg_stats inner{};
+        PcgOptions io;
+        io.atol = 0.0;
+        io.rtol = inner_rtol;
+        io.maxiter = maxiter - total;
+        int rc = pcg_t<float>(A, M, r32.as<float>(), nullptr, e32.as<float>(), io, &inner, s);
+        if (rc != HM_OK && rc != HM_ERR_NOCONV) return rc;
+        total += inner.iterations;
+        k_axpy_f32_to_f64<<<div_up(n, 256), 256, 0, s>>>(x, norm_r, e32.as<float>(), n);
+        HM_LAUNCH_CHECK();
+        HM_TRY(op_residual<double>(A, b, x, r64.as<double>(), s));
+    }
+    stats->iterations = total;
+    if (!stats->converged) {
+        set_error("Conjugate gradient descent did not converge within %d iterations", maxiter);
+        return HM_ERR_NOCONV;
+    }
+    return HM_OK;
+}
+
+// ------------------------------------------------------------ V-cycle set-up
+
+void vc_free(hm_vc* vc) {
+    for (size_t l = 0; l < vc->levels.size(); ++l) {
+        VcLevel& L = vc->levels[l];
+        if (L.owns_A && L.A) hm_dia_destroy(L.A);
+        dev_free(L.b);
+        dev_free(L.x);
+        dev_free(L.t);
+        dev_free(L.u);
+    }
+    dev_free(vc->dense_inv64);
+    dev_free(vc->dense_inv32);
+    free_reducer(&vc->red);
+    delete vc;
+}
+
+int dia_alloc(int h, int w, hm_dia** out) {
+    hm_dia* A = new hm_dia();
+    A->h = h;
+    A->w = w;
+    A->n = (long long)h * w;
+    int rc = dev_alloc(reinterpret_cast<void**>(&A->c64), sizeof(double) * 25 * A->n);
+    if (rc == HM_OK) rc = dev_alloc(reinterpret_cast<void**>(&A->c32), sizeof(float) * 25 * A->n);
+    if (rc == HM_OK) rc = dev_alloc(reinterpret_cast<void**>(&A->diag64), sizeof(double) * A->n);
+    if (rc == HM_OK) rc = dev_alloc(reinterpret_cast<void**>(&A->diag32), sizeof(float) * A->n);
+    if (rc != HM_OK) {
+        hm_dia_destroy(A);
+        return rc;
+    }
+    *out = A;
+    return HM_OK;
+}
+
+int vc_create(hm_cf* cf, hm_dia* dia, int h, int w, int pre, int post, double omega, int direct,
+              int prec, hipStream_t s, hm_vc** out) {
+    if ((cf == nullptr) == (dia == nullptr)) {
+        set_error("vcycle needs exactly one operator");
+        return HM_ERR_ARG;
+    }
+    const long long n0 = cf ? cf->n : dia->n;
+    if ((long long)h * w != n0) {
+        if ((long long)h * w > direct) {
+            set_error("dimension mismatch: shape (%d, %d) does not match an operator of size %lld",
+                      h, w, n0);
+            return HM_ERR_ARG;
+        }
+    }
+    if (pre < 0 || post < 0) {
+        set_error("num_pre_iter and num_post_iter must be non-negative");
+        return HM_ERR_ARG;
+    }
+    hm_vc* vc = new hm_vc();
+    vc->prec = prec;
+    vc->pre = pre;
+    vc->post = post;
+    vc->omega = omega;
+    vc->direct_size = direct;
+    const size_t es = prec == HM_F64 ? sizeof(double) : sizeof(float);
+    int rc = HM_OK;
+    hm_dia* fine = dia;
+    hm_dia* tmp0 = nullptr;
+    // Level 0.
+    VcLevel L0;
+    L0.h = h;
+    L0.w = w;
+    L0.n = n0;
+    if ((long long)h * w <= direct) {
+        // Direct solve of the given operator (vcycle.py:59-60).
+        if (cf) {
+            rc = hm_dia_from_cf(cf, s, &tmp0);
+            if (rc != HM_OK) { vc_free(vc); return rc; }
+            L0.A = tmp0;
+            L0.owns_A = true;
+        } else {
+            L0.A = dia;
+        }
+        L0.h = L0.A->h;
+        L0.w = L0.A->w;
+        if ((rc = dev_alloc(&L0.x, es * n0)) != HM_OK) { vc->levels.push_back(L0); vc_free(vc); return rc; }
+        vc->levels.push_back(L0);
+    } else {
+        vc->cf = cf;
+        L0.A = dia;  // null when matrix-free
+        if (pre >= 2 || post >= 2) {
+            rc = dev_alloc(&L0.x, es * n0);
+            if (rc == HM_OK) rc = dev_alloc(&L0.t, es * n0);
+            if (rc == HM_OK) rc = dev_alloc(&L0.u, es * n0);
+        }
+        vc->levels.push_back(L0);
+        if (rc != HM_OK) { vc_free(vc); return rc; }
+        if (cf) {
+            rc = hm_dia_from_cf(cf, s, &tmp0);  // exact level-0 coefficients for R A P
+            if (rc != HM_OK) { vc_free(vc); return rc; }
+            fine = tmp0;
+        }
+        int ch = h, cw = w;
+        while (true) {
+            ch /= 2;
+            cw /= 2;
+            VcLevel Lc;
+            Lc.h = ch;
+            Lc.w = cw;
+            Lc.n = (long long)ch * cw;
+            rc = dia_alloc(ch, cw, &Lc.A);
+            if (rc == HM_OK) {
+                Lc.owns_A = true;
+                rc = dia_galerkin(fine, Lc.A, s);
+            }
+            if (rc == HM_OK) rc = dia_finish(Lc.A, s);
+            if (rc == HM_OK) rc = dev_alloc(&Lc.b, es * Lc.n);
+            if (rc == HM_OK) rc = dev_alloc(&Lc.x, es * Lc.n);
+            if (rc == HM_OK) rc = dev_alloc(&Lc.t, es * Lc.n);
+            if (rc == HM_OK) rc = dev_alloc(&Lc.u, es * Lc.n);
+            vc->levels.push_back(Lc);
+            if (rc != HM_OK) break;
+            fine = Lc.A;
+            if (Lc.n <= direct) break;
+        }
+        if (tmp0) {
+            HM_HIP(hipStreamSynchronize(s));
+            hm_dia_destroy(tmp0);
+            tmp0 = nullptr;
+        }
+        if (rc != HM_OK) { vc_free(vc); return rc; }
+    }
+    const hm_dia* last = vc->levels.back().A;
+    vc->n_direct = (int)last->n;
+    rc = dev_alloc(reinterpret_cast<void**>(&vc->dense_inv64), sizeof(double) * last->n * last->n);
+    if (rc == HM_OK)
+        rc = dev_alloc(reinterpret_cast<void**>(&vc->dense_inv32), sizeof(float) * last->n * last->n);
+    if (rc == HM_OK && last->n > 0) rc = dense_inverse(last, vc->dense_inv64, vc->dense_inv32, s);
+    if (rc == HM_OK) rc = make_reducer(&vc->red, reducer_blocks(n0, h, w), 2);
+    if (rc != HM_OK) { vc_free(vc); return rc; }
+    HM_HIP(hipStreamSynchronize(s));
+    *out = vc;
+    return HM_OK;
+}
+
+}  // namespace hm
+
+// ===================================================================== C ABI
+
+using namespace hm;
+
+extern "C" {
+
+const char* hm_last_error(void) { return g_last_error.c_str(); }
+
+int hm_version(void) { return 1; }
+
+int hm_trimap_classes(const double* trimap, int64_t n, double fg_threshold, double bg_threshold,
+                      uint8_t* cls, int64_t* counts, void* stream) {
+    if (!trimap || !cls || !counts || n < 0) {
+        set_error("hm_trimap_classes: null pointer or negative size");
+        return HM_ERR_ARG;
+    }
+    return trimap_classes(trimap, n, fg_threshold, bg_threshold, cls, counts,
+                          static_cast<hipStream_t>(stream));
+}
+
+int hm_cf_create(const double* image_hwc, int h, int w, double epsilon, int radius,
+                 const uint8_t* cls, double lambda_value, int mode, void* stream, hm_cf** out) {
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (!image_hwc || !out || h < 1 || w < 1) {
+        set_error("hm_cf_create: invalid image");
+        return HM_ERR_ARG;
+    }
+    if (radius != 1) {
+        set_error("only radius=1 (5x5 stencil) is implemented on the GPU, got radius=%d", radius);
+        return HM_ERR_UNSUPPORTED;
+    }
+    if (mode < HM_OP_LAPLACIAN || mode > HM_OP_MASKED || (mode != HM_OP_LAPLACIAN && !cls)) {
+        set_error("hm_cf_create: invalid mode or missing class map");
+        return HM_ERR_ARG;
+    }
+    hm_cf* op = new hm_cf();
+    op->h = h;
+    op->w = w;
+    op->n = (long long)h * w;
+    op->eps = epsilon;
+    op->lambda = lambda_value;
+    op->mode = mode;
+    op->skip_known = (mode == HM_OP_LAPLACIAN && cls) ? 1 : 0;
+    const long long n = op->n;
+    int rc = dev_alloc(reinterpret_cast<void**>(&op->I64), sizeof(double) * 3 * n);
+    if (rc == HM_OK) rc = dev_alloc(reinterpret_cast<void**>(&op->I32), sizeof(float) * 3 * n);
+    if (rc == HM_OK) rc = dev_alloc(reinterpret_cast<void**>(&op->wc), sizeof(double) * 10 * n);
+    if (rc == HM_OK) rc = dev_alloc(reinterpret_cast<void**>(&op->diag64), sizeof(double) * n);
+    if (rc == HM_OK) rc = dev_alloc(reinterpret_cast<void**>(&op->diag32), sizeof(float) * n);
+    if (rc == HM_OK && cls) {
+        rc = dev_alloc(reinterpret_cast<void**>(&op->cls), n);
+        if (rc == HM_OK) {
+            hipError_t e = hipMemcpyAsync(op->cls, cls, n, hipMemcpyDeviceToDevice, s);
+            if (e != hipSuccess) {
+                set_error("hipMemcpyAsync: %s", hipGetErrorString(e));
+                rc = HM_ERR_HIP;
+            }
+        }
+    }
+    if (rc == HM_OK) rc = cf_upload_image(image_hwc, op, s);
+    if (rc == HM_OK) rc = cf_exact_setup(op, s);
+    if (rc != HM_OK) {
+        hm_cf_destroy(op);
+        return rc;
+    }
+    *out = op;
+    return HM_OK;
+}
+
+int hm_cf_destroy(hm_cf* op) {
+    if (!op) return HM_OK;
+    dev_free(op->I64);
+    dev_free(op->I32);
+    dev_free(op->cls);
+    dev_free(op->wc);
+    dev_free(op->diag64);
+    dev_free(op->diag32);
+    delete op;
+    return HM_OK;
+}
+
+int hm_cf_assemble_csr(hm_cf* op, double* values, int64_t* indices, int64_t* indptr,
+                       void* stream) {
+    if (!op || !values) {
+        set_error("hm_cf_assemble_csr: null argument");
+        return HM_ERR_ARG;
+    }
+    return cf_assemble(op, values, 25, 1, indices, indptr, op->mode,
+                       static_cast<hipStream_t>(stream));
+}
+
+int hm_cf_apply(hm_cf* op, int prec, const void* x, void* y, void* stream) {
+    if (!op || !x || !y) {
+        set_error("hm_cf_apply: null argument");
+        return HM_ERR_ARG;
+    }
+    OpHandle A;
+    A.cf = op;
+    const DotSink nodot{nullptr, Reducer{}, 0};
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (prec == HM_F64)
+        return op_apply<double>(A, static_cast<const double*>(x), static_cast<double*>(y), nodot,
+                                nullptr, s);
+    return op_apply<float>(A, static_cast<const float*>(x), static_cast<float*>(y), nodot, nullptr,
+                           s);
+}
+
+int hm_cf_diagonal(hm_cf* op, double* d, void* stream) {
+    if (!op || !d) {
+        set_error("hm_cf_diagonal: null argument");
+        return HM_ERR_ARG;
+    }
+    HM_HIP(hipMemcpyAsync(d, op->diag64, sizeof(double) * op->n, hipMemcpyDeviceToDevice,
+                          static_cast<hipStream_t>(stream)));
+    return HM_OK;
+}
+
+int hm_cf_rhs(hm_cf* op, double* b, void* stream) {
+    if (!op || !b || op->mode == HM_OP_LAPLACIAN) {
+        set_error("hm_cf_rhs: needs a lambda or masked operator");
+        return HM_ERR_ARG;
+    }
+    return cf_rhs(op, b, static_cast<hipStream_t>(stream));
+}
+
+int hm_dia_create(const double* coef, int h, int w, void* stream, hm_dia** out) {
+    if (!coef || !out || h < 1 || w < 1) {
+        set_error("hm_dia_create: invalid argument");
+        return HM_ERR_ARG;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hm_dia* A = nullptr;
+    HM_TRY(dia_alloc(h, w, &A));
+    hipError_t e = hipMemcpyAsync(A->c64, coef, sizeof(double) * 25 * A->n, hipMemcpyDeviceToDevice, s);
+    int rc = e == hipSuccess ? dia_finish(A, s) : HM_ERR_HIP;
+    if (e != hipSuccess) set_error("hipMemcpyAsync: %s", hipGetErrorString(e));
+    if (rc != HM_OK) {
+        hm_dia_destroy(A);
+        return rc;
+    }
+    *out = A;
+    return HM_OK;
+}
+
+int hm_dia_from_cf(hm_cf* op, void* stream, hm_dia** out) {
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hm_dia* A = nullptr;
+    HM_TRY(dia_alloc(op->h, op->w, &A));
+    int rc = cf_assemble(op, A->c64, 1, A->n, nullptr, nullptr, op->mode, s);
+    if (rc == HM_OK) rc = dia_finish(A, s);
+    if (rc != HM_OK) {
+        hm_dia_destroy(A);
+        return rc;
+    }
+    *out = A;
+    return HM_OK;
+}
+
+int hm_dia_destroy(hm_dia* A) {
+    if (!A) return HM_OK;
+    dev_free(A->c64);
+    dev_free(A->c32);
+    dev_free(A->diag64);
+    dev_free(A->diag32);
+    delete A;
+    return HM_OK;
+}
+
+int hm_dia_apply(hm_dia* A, int prec, const void* x, void* y, void* stream) {
+    if (!A || !x || !y) {
+        set_error("hm_dia_apply: null argument");
+        return HM_ERR_ARG;
+    }
+    OpHandle H;
+    H.dia = A;
+    const DotSink nodot{nullptr, Reducer{}, 0};
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (prec == HM_F64)
+        return op_apply<double>(H, static_cast<const double*>(x), static_cast<double*>(y), nodot,
+                                nullptr, s);
+    return op_apply<float>(H, static_cast<const float*>(x), static_cast<float*>(y), nodot, nullptr,
+                           s);
+}
+
+int hm_dia_coefficients(hm_dia* A, double* coef, void* stream) {
+    HM_HIP(hipMemcpyAsync(coef, A->c64, sizeof(double) * 25 * A->n, hipMemcpyDeviceToDevice,
+                          static_cast<hipStream_t>(stream)));
+    return HM_OK;
+}
+
+int hm_csr_create(int64_t n, int64_t nnz, const int64_t* indptr, const int64_t* indices,
+                  const double* values, hm_csr** out) {
+    if (n < 0 || nnz < 0 || !indptr || !out) {
+        set_error("hm_csr_create: invalid argument");
+        return HM_ERR_ARG;
+    }
+    hm_csr* A = new hm_csr();
+    A->n = n;
+    A->nnz = nnz;
+    int rc = dev_alloc(reinterpret_cast<void**>(&A->indptr), sizeof(int64_t) * (n + 1));
+    if (rc == HM_OK) rc = dev_alloc(reinterpret_cast<void**>(&A->indices), sizeof(int64_t) * nnz);
+    if (rc == HM_OK) rc = dev_alloc(reinterpret_cast<void**>(&A->values), sizeof(double) * nnz);
+    if (rc == HM_OK) {
+        if (hipMemcpy(A->indptr, indptr, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToDevice) != hipSuccess ||
+            (nnz > 0 && hipMemcpy(A->indices, indices, sizeof(int64_t) * nnz, hipMemcpyDeviceToDevice) != hipSuccess) ||
+            (nnz > 0 && hipMemcpy(A->values, values, sizeof(double) * nnz, hipMemcpyDeviceToDevice) != hipSuccess)) {
+            set_error("hm_csr_create: copy failed");
+            rc = HM_ERR_HIP;
+        }
+    }
+    if (rc != HM_OK) {
+        hm_csr_destroy(A);
+        return rc;
+    }
+    *out = A;
+    return HM_OK;
+}
+
+int hm_csr_destroy(hm_csr* A) {
+    if (!A) return HM_OK;
+    dev_free(A->indptr);
+    dev_free(A->indices);
+    dev_free(A->values);
+    delete A;
+    return HM_OK;
+}
+
+int hm_csr_apply(hm_csr* A, const double* x, double* y, void* stream) {
+    return csr_apply(A, x, y, nullptr, nullptr, 0, static_cast<hipStream_t>(stream));
+}
+
+int hm_vcycle_create(hm_cf* cf, hm_dia* dia, int h, int w, int num_pre_iter, int num_post_iter,
+                     double omega, int direct_solve_size, int prec, void* stream, hm_vc** out) {
+    if (!out) {
+        set_error("hm_vcycle_create: null output");
+        return HM_ERR_ARG;
+    }
+    return vc_create(cf, dia, h, w, num_pre_iter, num_post_iter, omega, direct_solve_size, prec,
+                     static_cast<hipStream_t>(stream), out);
+}
+
+int hm_vcycle_create_dense(const double* A_dense, int n, int prec, void* stream, hm_vc** out) {
+    if (!A_dense || !out || n < 1) {
+        set_error("hm_vcycle_create_dense: invalid argument");
+        return HM_ERR_ARG;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hm_vc* vc = new hm_vc();
+    vc->prec = prec;
+    VcLevel L0;
+    L0.h = 1;
+    L0.w = n;
+    L0.n = n;
+    vc->levels.push_back(L0);
+    vc->n_direct = n;
+    int rc = dev_alloc(reinterpret_cast<void**>(&vc->dense_inv64), sizeof(double) * (size_t)n * n);
+    if (rc == HM_OK)
+        rc = dev_alloc(reinterpret_cast<void**>(&vc->dense_inv32), sizeof(float) * (size_t)n * n);
+    if (rc == HM_OK) rc = dense_inverse_from_dense(A_dense, n, vc->dense_inv64, vc->dense_inv32, s);
+    if (rc == HM_OK) rc = make_reducer(&vc->red, reducer_blocks(n, 1, n), 2);
+    if (rc != HM_OK) {
+        vc_free(vc);
+        return rc;
+    }
+    *out = vc;
+    return HM_OK;
+}
+
+int hm_vcycle_destroy(hm_vc* vc) {
+    if (vc) vc_free(vc);
+    return HM_OK;
+}
+
+int hm_vcycle_levels(hm_vc* vc) { return vc ? (int)vc->levels.size() : 0; }
+
+int hm_vcycle_apply(hm_vc* vc, const void* r, void* z, void* stream) {
+    if (!vc || !r || !z) {
+        set_error("hm_vcycle_apply: null argument");
+        return HM_ERR_ARG;
+    }
+    return vcycle_apply(vc, r, z, nullptr, 0, nullptr, static_cast<hipStream_t>(stream));
+}
+
+int hm_copy(void* dst, const void* src, int64_t bytes, int kind, void* stream) {
+    if (bytes < 0 || (bytes > 0 && (!dst || !src))) {
+        set_error("hm_copy: invalid argument");
+        return HM_ERR_ARG;
+    }
+    if (bytes == 0) return HM_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyDeviceToHost
+                                      : (kind == 1 ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice);
+    HM_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, k, s));
+    if (kind != 2) HM_HIP(hipStreamSynchronize(s));
+    return HM_OK;
+}
+
+int hm_cg_solve(const hm_cg_args* a, hm_cg_stats* stats, void* stream) {
+    if (!a || !stats) {
+        set_error("hm_cg_solve: null argument");
+        return HM_ERR_ARG;
+    }
+    const int nops = (a->cf != nullptr) + (a->dia != nullptr) + (a->csr != nullptr) +
+                     (a->a_cb != nullptr);
+    if (nops != 1 || !a->b || !a->x || (a->a_cb && a->n < 0)) {
+        set_error("hm_cg_solve: needs exactly one operator and b, x");
+        return HM_ERR_ARG;
+    }
+    const int k = a->pc_kind;
+    if (k < HM_PC_NONE || k > HM_PC_HOST || (k == HM_PC_JACOBI && !a->inv_diag) ||
+        (k == HM_PC_VCYCLE && !a->vc) || (k == HM_PC_CSR && !a->pc_csr) ||
+        (k == HM_PC_HOST && !a->m_cb)) {
+        set_error("hm_cg_solve: invalid preconditioner");
+        return HM_ERR_ARG;
+    }
+    if (k == HM_PC_VCYCLE && a->vc->prec != HM_F64) {
+        set_error("hm_cg_solve: the V-cycle must be float64 for a float64 solve");
+        return HM_ERR_ARG;
+    }
+    OpHandle A;
+    A.cf = a->cf;
+    A.dia = a->dia;
+    A.csr = a->csr;
+    A.cb = a->a_cb;
+    A.cb_ctx = a->a_ctx;
+    A.cb_n = a->n;
+    if (k == HM_PC_VCYCLE && a->vc->levels[0].n != A.n()) {
+        set_error("dimension mismatch: V-cycle of size %lld for an operator of size %lld",
+                  a->vc->levels[0].n, A.n());
+        return HM_ERR_ARG;
+    }
+    PcHandle M;
+    M.kind = k;
+    M.inv_diag = a->inv_diag;
+    M.vc = a->vc;
+    M.csr = a->pc_csr;
+    M.cb = a->m_cb;
+    M.cb_ctx = a->m_ctx;
+    PcgOptions opt;
+    opt.atol = a->atol;
+    opt.rtol = a->rtol;
+    opt.maxiter = a->maxiter;
+    opt.reorth = a->reorthogonalize;
+    opt.check_every = a->check_every;
+    opt.callback = a->callback;
+    opt.cb_ctx = a->cb_ctx;
+    return pcg_t<double>(A, M, a->b, a->x0, a->x, opt, stats, static_cast<hipStream_t>(stream));
+}
+
+int hm_cg_solve_mixed(hm_cf* cf64, hm_vc* vc32, const double* b, double* x, double rtol,
+                      double inner_rtol, int max_outer, int maxiter, hm_cg_stats* stats,
+                      void* stream) {
+    if (!cf64 || !vc32 || !b || !x || !stats || vc32->prec != HM_F32) {
+        set_error("hm_cg_solve_mixed: needs a float64 operator and a float32 V-cycle");
+        return HM_ERR_ARG;
+    }
+    return pcg_mixed(cf64, vc32, b, x, rtol, inner_rtol, max_outer, maxiter, stats,
+                     static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int hm_cf_system_indptr(hm_cf* op, int64_t* indptr, int64_t* nnz, void* stream) {
+    if (!op || !indptr || !nnz) {
+        set_error("hm_cf_system_indptr: null argument");
+        return HM_ERR_ARG;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    DevBuf counts;
+    HM_TRY(counts.alloc(sizeof(int64_t) * op->n));
+    HM_TRY(cf_system_counts(op, counts.as<int64_t>(), s));
+    return exclusive_offsets(counts.as<int64_t>(), indptr, op->n, nnz, s);
+}
+
+int hm_cf_system_fill(hm_cf* op, const int64_t* indptr, int64_t* indices, double* values,
+                      void* stream) {
+    if (!op || !indptr || !indices || !values) {
+        set_error("hm_cf_system_fill: null argument");
+        return HM_ERR_ARG;
+    }
+    return cf_system_fill(op, indptr, indices, values, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"
+
+namespace hm {
+
+__global__ void k_diag_apply(const double* __restrict__ inv_diag, const double* __restrict__ r,
+                             double* __restrict__ z, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) z[i] = r[i] * inv_diag[i];
+}
+
+// alpha = clip(x on unknown pixels, trimap on known pixels, 0, 1)
+// (estimate_alpha_cf.py:72-76); lambda path: clip(x, 0, 1).
+__global__ void k_alpha_finalize(const double* __restrict__ x, const double* __restrict__ trimap,
+                                 const uint8_t* __restrict__ cls, double* __restrict__ alpha,
+                                 long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v = (cls && cls[i] != 0) ? trimap[i] : x[i];
+    v = v < 0.0 ? 0.0 : v;
+    v = v > 1.0 ? 1.0 : v;
+    alpha[i] = v;
+}
+
+}  // namespace hm
+
+extern "C" {
+
+int hm_diag_apply(const double* inv_diag, const double* r, double* z, int64_t n, void* stream) {
+    if (!inv_diag || !r || !z || n < 0) {
+        set_error("hm_diag_apply: invalid argument");
+        return HM_ERR_ARG;
+    }
+    if (n == 0) return HM_OK;
+    k_diag_apply<<<div_up(n, 256), 256, 0, static_cast<hipStream_t>(stream)>>>(inv_diag, r, z, n);
+    HM_LAUNCH_CHECK();
+    return HM_OK;
+}
+
+int hm_alpha_finalize(const double* x, const double* trimap, const uint8_t* cls, double* alpha,
+                      int64_t n, void* stream) {
+    if (!x || !alpha || n < 0 || (cls && !trimap)) {
+        set_error("hm_alpha_finalize: invalid argument");
+        return HM_ERR_ARG;
+    }
+    if (n == 0) return HM_OK;
+    k_alpha_finalize<<<div_up(n, 256), 256, 0, static_cast<hipStream_t>(stream)>>>(x, trimap, cls,
+                                                                                   alpha, n);
+    HM_LAUNCH_CHECK();
+    return HM_OK;
+}
+
+}  // extern "C"

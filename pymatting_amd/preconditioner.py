@@ -1,0 +1,151 @@
+"""Preconditioners on the GPU.
+
+Drop-ins for pymatting/preconditioner/jacobi.py:1-31 (jacobi) and
+pymatting/preconditioner/vcycle.py:103-153 (vcycle). Both return callables
+(numpy or torch vectors in, the same type out) that ``cg`` recognises and
+runs inside its device loop. pymatting's ``ichol`` (a sequential incomplete
+Cholesky) has no GPU implementation here; ``vcycle`` is the GPU default.
+"""
+import ctypes
+
+import numpy as np
+import scipy.sparse
+import torch
+
+from pymatting_amd import _ffi
+from pymatting_amd._device import as_device, empty, output_like
+from pymatting_amd.laplacian import CfOperator, DiaOperator, device_operator
+
+
+def _diagonal_device(A):
+    op = device_operator(A)
+    if op is not None:
+        return op.diagonal_device()
+    if scipy.sparse.issparse(A):
+        d = A.diagonal()
+    else:
+        d = np.diag(np.asarray(A))
+    return as_device(np.asarray(d, dtype=np.float64))
+
+
+class Jacobi:
+    """x -> x / diag(A) (jacobi.py:24-31)."""
+
+    def __init__(self, A):
+        self.inv_diag = 1.0 / _diagonal_device(A)
+        self.n = self.inv_diag.numel()
+
+    def __call__(self, x):
+        xd = as_device(x).reshape(-1)
+        out = empty(self.n)
+        _ffi.check(_ffi.lib().hm_diag_apply(_ffi.ptr(self.inv_diag), _ffi.ptr(xd), _ffi.ptr(out),
+                                            self.n, _ffi.stream()))
+        return output_like(out, x)
+
+
+def jacobi(A):
+    """jacobi.py:1-31: the Jacobi (diagonal) preconditioner of A."""
+    return Jacobi(A)
+
+
+class VCycle:
+    """Geometric multigrid V-cycle (vcycle.py:46-153) built on the device.
+
+    Levels: (h, w) -> (h//2, w//2) -> ... until h*w <= direct_solve_size,
+    Galerkin coarse operators R A P, damped Jacobi smoothing, and a dense
+    direct solve on the coarsest grid.
+    """
+
+    def __init__(self, A, shape, num_pre_iter=1, num_post_iter=1, omega=0.8,
+                 direct_solve_size=64, precision="f64"):
+        self.shape = (int(shape[0]), int(shape[1]))
+        h, w = self.shape
+        self.precision = precision
+        prec = _ffi.HM_F64 if precision == "f64" else _ffi.HM_F32
+        op = device_operator(A)
+        handle = ctypes.c_void_p()
+        L = _ffi.lib()
+        self._keep = []
+        if h * w <= direct_solve_size and not isinstance(op, CfOperator):
+            # vcycle.py:59-60 solves the given matrix directly.
+            if op is not None:
+                dense = self._dense_from_dia(op)
+            elif scipy.sparse.issparse(A):
+                dense = A.toarray()
+            else:
+                dense = np.asarray(A, dtype=np.float64)
+            dense = as_device(np.asarray(dense, dtype=np.float64))
+            self.n = dense.shape[0]
+            _ffi.check(L.hm_vcycle_create_dense(_ffi.ptr(dense), self.n, prec, _ffi.stream(),
+                                                ctypes.byref(handle)))
+        else:
+            if op is None:
+                op = DiaOperator.from_sparse(A, self.shape)
+            self.n = op.n
+            self._keep.append(op)
+            cf = op.handle if isinstance(op, CfOperator) else None
+            dia = op.handle if isinstance(op, DiaOperator) else None
+            _ffi.check(L.hm_vcycle_create(cf, dia, h, w, int(num_pre_iter), int(num_post_iter),
+                                          float(omega), int(direct_solve_size), prec,
+                                          _ffi.stream(), ctypes.byref(handle)))
+        self.handle = handle
+        self.levels = L.hm_vcycle_levels(handle)
+
+    @staticmethod
+    def _dense_from_dia(op):
+        h, w = op.grid_shape
+        n = op.n
+        coef = op.coef.cpu().numpy() if isinstance(op, DiaOperator) else None
+        if coef is None:
+            raise ValueError("dense direct solve needs a stored operator")
+        dense = np.zeros((n, n))
+        for s in range(25):
+            dy, dx = s // 5 - 2, s % 5 - 2
+            for i in range(n):
+                y, x = divmod(i, w)
+                if 0 <= y + dy < h and 0 <= x + dx < w:
+                    dense[i, (y + dy) * w + x + dx] = coef[s, i]
+        return dense
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            _ffi.lib().hm_vcycle_destroy(h)
+            self.handle = None
+
+    def apply(self, r, out=None):
+        dt = torch.float64 if self.precision == "f64" else torch.float32
+        out = out if out is not None else empty(self.n, dt)
+        _ffi.check(_ffi.lib().hm_vcycle_apply(self.handle, _ffi.ptr(r), _ffi.ptr(out),
+                                              _ffi.stream()))
+        return out
+
+    def __call__(self, r):
+        dt = torch.float64 if self.precision == "f64" else torch.float32
+        rd = as_device(r, dt).reshape(-1)
+        if rd.numel() != self.n:
+            raise ValueError("dimension mismatch: V-cycle of size %d, vector of size %d"
+                             % (self.n, rd.numel()))
+        return output_like(self.apply(rd), r)
+
+
+def vcycle(A, shape, num_pre_iter=1, num_post_iter=1, omega=0.8, direct_solve_size=64,
+           cache=None):
+    """vcycle.py:103-153: returns the V-cycle preconditioner of A on `shape`.
+
+    `cache` (a dict) keeps the hierarchy per shape across calls, like the
+    reference's per-shape cache of coarse operators.
+    """
+    key = (tuple(int(s) for s in shape), num_pre_iter, num_post_iter, omega, direct_solve_size)
+    if cache is not None and key in cache:
+        return cache[key]
+    M = VCycle(A, shape, num_pre_iter, num_post_iter, omega, direct_solve_size)
+    if cache is not None:
+        cache[key] = M
+    return M
+
+
+def ichol(A, *args, **kwargs):
+    """pymatting's thresholded incomplete Cholesky is sequential and has no GPU form here."""
+    raise NotImplementedError("ichol has no GPU implementation in pymatting_amd; "
+                              "use vcycle (the default of estimate_alpha_cf) or jacobi")

@@ -1,0 +1,54 @@
+"""GPU parity of the alpha solves against the oracle and the reference's
+tolerances (tests/test_estimate_alpha.py, tests/test_simple_api.py)."""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+import oracle.cf as ocf
+
+pytestmark = pytest.mark.gpu
+
+
+def test_estimate_alpha_cf_lemur():
+    import pymatting_amd as pm
+    d = load_golden("lemur_0p1")
+    alpha = pm.estimate_alpha_cf(d["image"], d["trimap"])
+    assert np.linalg.norm(alpha - d["good_alpha"]) < 1.68
+    # Same solution as the oracle's (ichol-preconditioned) solve, both tight.
+    ref = ocf.estimate_alpha_cf(d["image"], d["trimap"], cg_kwargs={"rtol": 1e-12})
+    tight = pm.estimate_alpha_cf(d["image"], d["trimap"], cg_kwargs={"rtol": 1e-12})
+    assert np.abs(tight - ref).max() < 1e-6
+
+
+def test_estimate_alpha_cf_jacobi_factory_and_test32():
+    import pymatting_amd as pm
+    d = load_golden("test32")
+    alpha = pm.estimate_alpha_cf(d["image"], d["trimap"], preconditioner=pm.jacobi)
+    assert np.linalg.norm(alpha - d["true_alpha"]) < 3.1
+    ref = ocf.estimate_alpha_cf(d["image"], d["trimap"], cg_kwargs={"rtol": 1e-12})
+    tight = pm.estimate_alpha_cf(d["image"], d["trimap"], cg_kwargs={"rtol": 1e-12})
+    assert np.abs(tight - ref).max() < 1e-6
+
+
+def test_estimate_alpha_cf_errors():
+    import pymatting_amd as pm
+    image = np.random.default_rng(0).random((8, 8, 3))
+    with pytest.raises(ValueError):
+        pm.estimate_alpha_cf(image, np.full((8, 8), 0.5))
+    with pytest.raises(TypeError):
+        pm.cf_laplacian(image.astype(np.float32))
+
+
+def test_lambda_solve_config1_fp64_and_mixed():
+    import pymatting_amd as pm
+    from pymatting_amd import synthetic
+    image, trimap, _ = synthetic.config1()
+    ref_alpha, ref_x = ocf.solve_alpha_lambda(image, trimap)
+    alpha, st = pm.solve_alpha_lambda(image, trimap, precision="f64")
+    alpha = alpha.cpu().numpy()
+    print("fp64 iterations", st, "max|d|", np.abs(alpha - ref_alpha).max(), flush=True)
+    assert np.abs(alpha - ref_alpha).max() <= 1e-6
+    alpha32, st32 = pm.solve_alpha_lambda(image, trimap, precision="mixed")
+    alpha32 = alpha32.cpu().numpy()
+    print("mixed", st32, "max|d|", np.abs(alpha32 - ref_alpha).max(), flush=True)
+    assert np.abs(alpha32 - ref_alpha).max() <= 1e-4

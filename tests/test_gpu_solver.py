@@ -1,0 +1,101 @@
+"""GPU parity of cg, jacobi and vcycle against the oracle and the reference's
+known answers (docstrings, tests/test_cg.py, tests/test_preconditioners.py)."""
+import numpy as np
+import pytest
+import scipy.sparse
+
+from conftest import load_golden
+import oracle.cf as ocf
+import oracle.solver as osol
+
+pytestmark = pytest.mark.gpu
+
+
+def test_docstring_examples():
+    import pymatting_amd as pm
+    A = np.array([[3.0, 1.0], [1.0, 2.0]])
+    assert np.allclose(pm.cg(A, np.array([4.0, 3.0]), M=pm.jacobi(A)), [1.0, 1.0])
+    B = np.array([[2.0, 3.0], [3.0, 5.0]])
+    assert np.allclose(pm.jacobi(B)(np.array([1.0, 2.0])), [0.5, 0.4])
+    assert np.allclose(pm.vcycle(B, (2, 2))(np.array([1.0, 2.0])), [-1.0, 1.0])
+
+
+def test_cg_random_spd_all_modes():
+    import pymatting_amd as pm
+    np.random.seed(0)
+    for n in [1, 5, 10]:
+        x_true = np.random.rand(n)
+        A = np.random.rand(n, n)
+        A += A.T + n * np.eye(n)
+        b = A.dot(x_true)
+        Mj = np.diag(1.0 / np.diag(A))
+        for x0 in [None, np.random.rand(n)]:
+            for M in [None, Mj, lambda v: Mj.dot(v)]:
+                for cb in [None, pm.CounterCallback(), pm.ProgressCallback()]:
+                    x = pm.cg(A, b, x0=x0, M=M, rtol=0, atol=1e-10, callback=cb)
+                    assert np.linalg.norm(A.dot(x) - b) < 1e-10
+                    if cb is not None:
+                        assert 0 < cb.n < 20
+    # Callable operator.
+    A = np.diag(np.arange(1.0, 6.0))
+    x = pm.cg(lambda v: A.dot(v), np.ones(5), rtol=1e-12)
+    assert np.allclose(x, 1.0 / np.arange(1.0, 6.0))
+
+
+def test_cg_maxiter_raises():
+    import pymatting_amd as pm
+    A = np.diag(np.arange(1.0, 50.0))
+    with pytest.raises(ValueError, match="did not converge within 3 iterations"):
+        pm.cg(A, np.ones(49), maxiter=3)
+
+
+def _lemur_system():
+    d = load_golden("lemur_0p125")
+    return d["image"], d["trimap"]
+
+
+def test_vcycle_matches_oracle():
+    import pymatting_amd as pm
+    image, trimap = _lemur_system()
+    A, b = pm.make_linear_system(pm.cf_laplacian(image), trimap)
+    Ar, br = ocf.make_linear_system(ocf.cf_laplacian(image), trimap)
+    for pre, post in [(1, 1), (2, 2), (0, 1), (1, 0), (3, 2)]:
+        M = pm.vcycle(A, trimap.shape, num_pre_iter=pre, num_post_iter=post)
+        Mr = osol.vcycle(Ar, trimap.shape, num_pre_iter=pre, num_post_iter=post)
+        rng = np.random.default_rng(pre * 10 + post)
+        r = rng.standard_normal(b.size)
+        z, zr = M(r), Mr(r)
+        assert np.abs(z - zr).max() <= 1e-10 * np.abs(zr).max(), (pre, post)
+    # Stored-stencil input (plain scipy matrix) builds the same hierarchy.
+    Md = pm.vcycle(scipy.sparse.csr_matrix(Ar), trimap.shape)
+    r = np.random.default_rng(5).standard_normal(b.size)
+    assert np.abs(Md(r) - osol.vcycle(Ar, trimap.shape)(r)).max() <= 1e-10 * np.abs(r).max() * 10
+
+
+def test_iteration_ceilings_and_oracle_counts():
+    import pymatting_amd as pm
+    image, trimap = _lemur_system()
+    A, b = pm.make_linear_system(pm.cf_laplacian(image), trimap)
+    Ar, br = ocf.make_linear_system(ocf.cf_laplacian(image), trimap)
+    expected = {"none": 378, "jacobi": 197, "vcycle": 48}
+    for name, mk, mkr in [("none", lambda A: None, lambda A: None),
+                          ("jacobi", pm.jacobi, osol.jacobi),
+                          ("vcycle", lambda A: pm.vcycle(A, trimap.shape),
+                           lambda A: osol.vcycle(A, trimap.shape))]:
+        cb, cbr = pm.CounterCallback(), osol.CounterCallback()
+        x = pm.cg(A, b, M=mk(A), atol=1e-6, rtol=0, maxiter=10000, callback=cb)
+        xr = osol.cg(Ar, br, M=mkr(Ar), atol=1e-6, rtol=0, maxiter=10000, callback=cbr)
+        assert np.linalg.norm(br - Ar.dot(x)) <= 1e-6
+        assert cb.n <= expected[name]
+        assert abs(cb.n - cbr.n) <= 1, (name, cb.n, cbr.n)
+        assert np.abs(x - xr).max() < 1e-6
+
+
+def test_dia_operator_and_csr_paths_agree():
+    import pymatting_amd as pm
+    image, trimap = _lemur_system()
+    A, b = pm.make_linear_system(pm.cf_laplacian(image), trimap)
+    plain = scipy.sparse.csr_matrix(A)  # no device operator: CSR upload path
+    x1 = pm.cg(A, b, M=pm.vcycle(A, trimap.shape), rtol=1e-10)
+    x2 = pm.cg(plain, b, M=pm.vcycle(plain, trimap.shape), rtol=1e-10)
+    assert np.abs(x1 - x2).max() < 1e-8
