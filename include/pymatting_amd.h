@@ -191,6 +191,27 @@ int hm_fb_ml(const void* image, const void* alpha, int h0, int w0, int n_levels,
              int n_big, int small_size, double gradient_weight, int prec, void* F, void* B,
              void* stream);
 
+/* ---- measurement (no reference counterpart) ----
+ * hm_prof_set(1) starts recording HIP events around every launch of the
+ * library's kernel classes (on the stream each kernel runs on) and discards
+ * earlier records; hm_prof_query sums the recorded durations of one class
+ * (HM_PROF_* below) for one precision (0 float64, 1 float32), with the number
+ * of launches and the grid points they covered. */
+#define HM_PROF_CF_APPLY 0
+#define HM_PROF_CF_RESTRICT 1
+#define HM_PROF_CF_SMOOTH 2
+#define HM_PROF_CF_RESIDUAL 3
+#define HM_PROF_DIA_APPLY 4
+#define HM_PROF_DIA_RESTRICT 5
+#define HM_PROF_DIA_SMOOTH 6
+#define HM_PROF_DIA_RESIDUAL 7
+#define HM_PROF_CG_UPDATE 8
+#define HM_PROF_CG_PUPDATE 9
+#define HM_PROF_DENSE 10
+#define HM_PROF_FG_SWEEP 11
+int hm_prof_set(int enabled);
+int hm_prof_query(int kind, int prec, double* total_ms, int64_t* launches, int64_t* points);
+
 #ifdef __cplusplus
 }
 #endif

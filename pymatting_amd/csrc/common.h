@@ -150,6 +150,22 @@ __device__ __forceinline__ bool solve_done(const CgState* st) {
                                               __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 
+// Optional per-kernel-class timing with HIP events (hm_prof_*). When enabled,
+// each launch of a class is bracketed by events on its own stream; the
+// events are read after the measured region.
+enum ProfKind {
+    kProfCfApply = 0, kProfCfRestrict, kProfCfSmooth, kProfCfResidual,
+    kProfDiaApply, kProfDiaRestrict, kProfDiaSmooth, kProfDiaResidual,
+    kProfCgUpdate, kProfCgPupdate, kProfDense, kProfFgSweep, kProfCount
+};
+
+struct ProfScope {
+    ProfScope(int kind, long long points, int prec, hipStream_t s);
+    ~ProfScope();
+    int slot;
+    hipStream_t stream;
+};
+
 inline unsigned div_up(long long a, long long b) { return static_cast<unsigned>((a + b - 1) / b); }
 
 }  // namespace hm

@@ -53,6 +53,7 @@ struct CfShared {
 
 template <typename T>
 struct CfOp {
+    static constexpr int kProfBase = kProfCfApply;
     CfDev d;
     const T* I;  // planar [3][n]
     int force_mode = -1;  // >= 0 overrides d.mode (the masked right-hand side uses plain L)
@@ -183,6 +184,7 @@ struct NoShared {};
 // Stored 25-point stencil; coef is slot-major [25][n], slot (dy+2)*5 + (dx+2).
 template <typename T>
 struct DiaOp {
+    static constexpr int kProfBase = kProfDiaApply;
     int h, w;
     const T* coef;
 
@@ -304,6 +306,7 @@ __device__ __forceinline__ void dot_finalize(CgState* st, int kind, double s) {
 template <typename T>
 struct OutApply {
     HM_TILE_GEOM(64, 8)
+    static constexpr int kProf = 0;
     T* y;
     DotSink dot;
     __device__ __forceinline__ void point(long long g, int, int, T Ax, T xi, double& acc) const {
@@ -317,6 +320,7 @@ struct OutApply {
 template <typename T>
 struct OutResidual {
     HM_TILE_GEOM(64, 8)
+    static constexpr int kProf = 3;
     const T* b;
     T* r;
     DotSink dot;
@@ -330,6 +334,7 @@ struct OutResidual {
 template <typename T>
 struct OutSmooth {
     HM_TILE_GEOM(64, 8)
+    static constexpr int kProf = 2;
     const T* b;
     const T* D;
     T omega;
@@ -348,6 +353,7 @@ struct OutSmooth {
 // full-weighting stencil to bc (vcycle.py:77-80).
 template <typename T>
 struct OutRestrict {
+    static constexpr int kProf = 1;
     static constexpr int CX = 32, CY = 4;
     static constexpr int OX = 2 * CX + 1, OY = 2 * CY + 1;
     const T* b;

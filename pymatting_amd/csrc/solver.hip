@@ -43,6 +43,7 @@ int launch_tile(const Op& op, const In& in, const Out& out, int h, int w, int h2
         set_error("reduction buffer too small for %u x %u blocks", grid.x, grid.y);
         return HM_ERR_ARG;
     }
+    ProfScope prof(Op::kProfBase + Out::kProf, (long long)h * w, sizeof(T) == 4, s);
     tile_kernel<T, Op, In, Out><<<grid, kBlock, 0, s>>>(op, in, out, h, w, guard);
     HM_LAUNCH_CHECK();
     return HM_OK;
@@ -640,9 +641,12 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
                 HM_LAUNCH_CHECK();
             }
             HM_TRY(op_apply<T>(A, p.as<T>(), q.as<T>(), pq, st.dev, s));
-            k_cg_update<T><<<vb, kBlock, 0, s>>>(x, p.as<T>(), r.as<T>(), q.as<T>(), n, st.dev,
-                                                 red);
-            HM_LAUNCH_CHECK();
+            {
+                ProfScope prof_u(kProfCgUpdate, n, sizeof(T) == 4, s);
+                k_cg_update<T><<<vb, kBlock, 0, s>>>(x, p.as<T>(), r.as<T>(), q.as<T>(), n,
+                                                     st.dev, red);
+                HM_LAUNCH_CHECK();
+            }
             if (interactive) {
                 HM_TRY(st.pull(s));
                 if constexpr (std::is_same<T, double>::value) {
@@ -663,8 +667,11 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
                 HM_TRY(pc_apply<T>(M, r.as<T>(), z.as<T>(), st.dev, red, kDotRzUpdate, st.dev, n,
                                    s));
             }
-            k_cg_pupdate<T><<<vb, kBlock, 0, s>>>(p.as<T>(), z.as<T>(), n, st.dev);
-            HM_LAUNCH_CHECK();
+            {
+                ProfScope prof_p(kProfCgPupdate, n, sizeof(T) == 4, s);
+                k_cg_pupdate<T><<<vb, kBlock, 0, s>>>(p.as<T>(), z.as<T>(), n, st.dev);
+                HM_LAUNCH_CHECK();
+            }
         }
         enqueued += k;
         HM_TRY(st.pull(s));
@@ -1323,6 +1330,76 @@ int hm_alpha_finalize(const double* x, const double* trimap, const uint8_t* cls,
     k_alpha_finalize<<<div_up(n, 256), 256, 0, static_cast<hipStream_t>(stream)>>>(x, trimap, cls,
                                                                                    alpha, n);
     HM_LAUNCH_CHECK();
+    return HM_OK;
+}
+
+}  // extern "C"
+
+// ----------------------------------------------------------------- profiling
+
+namespace hm {
+namespace {
+struct ProfRecord {
+    hipEvent_t start, stop;
+    int kind;
+    int prec;
+    long long points;
+};
+bool g_prof_on = false;
+std::vector<ProfRecord> g_prof;
+std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_pool;
+size_t g_prof_used = 0;
+}  // namespace
+
+ProfScope::ProfScope(int kind, long long points, int prec, hipStream_t s) : slot(-1), stream(s) {
+    if (!g_prof_on) return;
+    if (g_prof_used == g_prof_pool.size()) {
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+        g_prof_pool.emplace_back(a, b);
+    }
+    ProfRecord r{g_prof_pool[g_prof_used].first, g_prof_pool[g_prof_used].second, kind, prec,
+                 points};
+    ++g_prof_used;
+    if (hipEventRecord(r.start, s) != hipSuccess) return;
+    slot = (int)g_prof.size();
+    g_prof.push_back(r);
+}
+
+ProfScope::~ProfScope() {
+    if (slot >= 0) (void)hipEventRecord(g_prof[slot].stop, stream);
+}
+
+}  // namespace hm
+
+extern "C" {
+
+int hm_prof_set(int enabled) {
+    hm::g_prof_on = enabled != 0;
+    hm::g_prof.clear();
+    hm::g_prof_used = 0;
+    return HM_OK;
+}
+
+int hm_prof_query(int kind, int prec, double* total_ms, int64_t* launches, int64_t* points) {
+    if (!total_ms || !launches || !points) {
+        set_error("hm_prof_query: null argument");
+        return HM_ERR_ARG;
+    }
+    double ms = 0.0;
+    int64_t cnt = 0, pts = 0;
+    for (const auto& r : hm::g_prof) {
+        if (r.kind != kind || r.prec != prec) continue;
+        HM_HIP(hipEventSynchronize(r.stop));
+        float e = 0.f;
+        HM_HIP(hipEventElapsedTime(&e, r.start, r.stop));
+        ms += e;
+        ++cnt;
+        pts += r.points;
+    }
+    *total_ms = ms;
+    *launches = cnt;
+    *points = pts;
     return HM_OK;
 }
 
