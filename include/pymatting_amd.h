@@ -169,10 +169,11 @@ int hm_cg_solve(const hm_cg_args* args, hm_cg_stats* stats, void* stream);
 
 /* Mixed precision: float32 stencil, SpMV and smoothing inside CG with float64
  * dot products, wrapped in float64 iterative refinement on the residual
- * b - A x (recomputed with the float64 operator). vc must be HM_F32. */
-int hm_cg_solve_mixed(hm_cf* cf64, hm_vc* vc32, const double* b, double* x, double rtol,
-                      double inner_rtol, int max_outer, int maxiter, hm_cg_stats* stats,
-                      void* stream);
+ * b - A x (recomputed with the float64 operator). Exactly one of cf (matrix-
+ * free) / dia (stored stencil) is the operator; vc32 must be HM_F32. */
+int hm_cg_solve_mixed(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x,
+                      double rtol, double inner_rtol, int max_outer, int maxiter,
+                      hm_cg_stats* stats, void* stream);
 
 /* z = r * inv_diag (jacobi.py:24-31 applied to a vector). */
 int hm_diag_apply(const double* inv_diag, const double* r, double* z, int64_t n, void* stream);

@@ -20,7 +20,7 @@ import torch
 
 from pymatting_amd import _ffi
 from pymatting_amd._device import as_device, empty, is_torch
-from pymatting_amd.laplacian import CfOperator
+from pymatting_amd.laplacian import CfOperator, DiaOperator
 from pymatting_amd.preconditioner import VCycle
 from pymatting_amd.solver import cg
 from pymatting_amd.util import device_classes, sanity_check_image, trimap_split
@@ -70,12 +70,17 @@ class LambdaSolver:
     """
 
     def __init__(self, image, trimap, epsilon=1e-7, lambda_value=100.0, precision="f64",
-                 num_pre_iter=1, num_post_iter=1, omega=0.8, direct_solve_size=64):
+                 num_pre_iter=1, num_post_iter=1, omega=0.8, direct_solve_size=64,
+                 level0="matrix_free"):
         self.h, self.w = int(trimap.shape[0]), int(trimap.shape[1])
         self.precision = precision
         self.classes, self.tdev = device_classes(trimap)
-        self.op = CfOperator(image, epsilon, 1, "lambda", self.classes, self.tdev, lambda_value)
-        self.b = self.op.rhs()
+        self.cf = CfOperator(image, epsilon, 1, "lambda", self.classes, self.tdev, lambda_value)
+        self.b = self.cf.rhs()
+        # level0 "matrix_free": window stencil recomputed from the image on
+        # every pass; "stencil": the 25-point operator assembled once (exactly)
+        # and streamed (the stored ELL/DIA form).
+        self.op = DiaOperator.from_cf(self.cf) if level0 == "stencil" else self.cf
         self.M = VCycle(self.op, (self.h, self.w), num_pre_iter, num_post_iter, omega,
                         direct_solve_size, "f64" if precision == "f64" else "f32")
         self.stats = {}
@@ -91,8 +96,10 @@ class LambdaSolver:
         else:
             x = empty(n)
             stats = _ffi.CgStats()
+            cf = self.op.handle if isinstance(self.op, CfOperator) else None
+            dia = self.op.handle if isinstance(self.op, DiaOperator) else None
             _ffi.check(_ffi.lib().hm_cg_solve_mixed(
-                self.op.handle, self.M.handle, _ffi.ptr(self.b), _ffi.ptr(x), float(rtol),
+                cf, dia, self.M.handle, _ffi.ptr(self.b), _ffi.ptr(x), float(rtol),
                 float(inner_rtol), int(max_outer), int(maxiter), ctypes.byref(stats),
                 _ffi.stream()))
             self.stats = {"iterations": stats.iterations, "norm_b": stats.norm_b,

@@ -27,22 +27,55 @@ __global__ void k_planar_to_f32(const double* __restrict__ in, float* __restrict
     if (t < m) out[t] = (float)in[t];
 }
 
-__global__ void k_trimap_classes(const double* __restrict__ trimap, long long n, double fg_thr,
-                                 double bg_thr, uint8_t* __restrict__ cls,
-                                 unsigned long long* __restrict__ counts) {
+__global__ __launch_bounds__(256) void k_trimap_classes(const double* __restrict__ trimap,
+                                                        long long n, double fg_thr, double bg_thr,
+                                                        uint8_t* __restrict__ cls,
+                                                        unsigned long long* __restrict__ part) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     int fg = 0, bg = 0;
     if (t < n) {
         const double v = trimap[t];
         fg = v >= fg_thr;  // util.py:509
         bg = v <= bg_thr;  // util.py:510
+        // A pixel can be both only if fg_thr <= bg_thr; the class byte keeps fg then.
         cls[t] = fg ? 1 : (bg ? 2 : 0);
     }
-    // A pixel can be both only if fg_thr <= bg_thr; the class byte keeps fg then.
-    unsigned long long bf = __ballot(fg), bb = __ballot(bg);
+    __shared__ unsigned cnt[2];
+    if (threadIdx.x == 0) cnt[0] = cnt[1] = 0;
+    __syncthreads();
+    const unsigned long long bf = __ballot(fg), bb = __ballot(bg);
     if ((threadIdx.x % kWave) == 0) {
-        if (bf) atomicAdd(counts, (unsigned long long)__popcll(bf));
-        if (bb) atomicAdd(counts + 1, (unsigned long long)__popcll(bb));
+        atomicAdd(&cnt[0], (unsigned)__popcll(bf));
+        atomicAdd(&cnt[1], (unsigned)__popcll(bb));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = cnt[0];
+        part[2 * blockIdx.x + 1] = cnt[1];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_trimap_counts(const unsigned long long* __restrict__ part,
+                                                       unsigned nblocks, int64_t* __restrict__ counts) {
+    __shared__ unsigned long long s[2][256];
+    unsigned long long f = 0, b = 0;
+    for (unsigned i = threadIdx.x; i < nblocks; i += 256) {
+        f += part[2 * i];
+        b += part[2 * i + 1];
+    }
+    s[0][threadIdx.x] = f;
+    s[1][threadIdx.x] = b;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            s[0][threadIdx.x] += s[0][threadIdx.x + off];
+            s[1][threadIdx.x] += s[1][threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        counts[0] = (int64_t)s[0][0];
+        counts[1] = (int64_t)s[1][0];
     }
 }
 
@@ -301,6 +334,8 @@ int cf_exact_setup(hm_cf* op, hipStream_t s) {
     k_diag_exact<<<g, 256, 0, s>>>(op->I64, op->wc, op->h, op->w, op->cls, op->mode, op->lambda,
                                    op->diag64, op->diag32);
     HM_LAUNCH_CHECK();
+    k_planar_to_f32<<<div_up(9 * n, 256), 256, 0, s>>>(op->wc, op->wc32, 9 * n);
+    HM_LAUNCH_CHECK();
     return HM_OK;
 }
 
@@ -331,10 +366,18 @@ int cf_rhs(hm_cf* op, double* b, hipStream_t s) {
 
 int trimap_classes(const double* trimap, long long n, double fg, double bg, uint8_t* cls,
                    int64_t* counts, hipStream_t s) {
-    HM_HIP(hipMemsetAsync(counts, 0, 2 * sizeof(int64_t), s));
-    k_trimap_classes<<<div_up(n, 256), 256, 0, s>>>(trimap, n, fg, bg, cls,
-                                                     reinterpret_cast<unsigned long long*>(counts));
-    HM_LAUNCH_CHECK();
+    const unsigned g = n > 0 ? div_up(n, 256) : 1;
+    void* part = nullptr;
+    HM_TRY(dev_alloc(&part, sizeof(unsigned long long) * 2 * g));
+    k_trimap_classes<<<g, 256, 0, s>>>(trimap, n, fg, bg, cls,
+                                       static_cast<unsigned long long*>(part));
+    k_trimap_counts<<<1, 256, 0, s>>>(static_cast<unsigned long long*>(part), g, counts);
+    const hipError_t e = hipGetLastError();
+    dev_free(part);
+    if (e != hipSuccess) {
+        set_error("trimap classes: %s", hipGetErrorString(e));
+        return HM_ERR_HIP;
+    }
     return HM_OK;
 }
 

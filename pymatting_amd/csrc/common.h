@@ -60,9 +60,8 @@ struct CgState {
 
 // ---------------------------------------------------------------------------
 // Deterministic reductions. Each block reduces its values with wavefront
-// shuffles and an LDS stage, writes one partial per value, and the last block
-// to finish (atomic ticket) sums the partials in a fixed order. The counter is
-// reset by that last block, so no memset is needed between launches.
+// shuffles and an LDS stage and writes one partial per value; a one-block
+// finalize kernel sums the partials in a fixed order.
 struct Reducer {
     double* partials;   // [N][max_blocks]
     unsigned* counter;  // one counter per reducer
@@ -104,45 +103,19 @@ __device__ __forceinline__ void block_sum(double (&v)[N]) {
     }
 }
 
-__device__ __forceinline__ double load_coherent(const double* p) {
-    unsigned long long u =
-        __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
-                          __HIP_MEMORY_SCOPE_AGENT);
-    return __longlong_as_double(static_cast<long long>(u));
-}
-
-// Returns true in thread 0 of the last block, with the grid-wide sums in out[].
+// Stage one of a reduction: thread 0 writes this block's sums to
+// partials[k * max_blocks + block]. Stage two (k_finalize) runs as its own
+// one-workgroup kernel after the producing kernel, so the partials are
+// visible through the kernel boundary without a device-scope fence per block
+// (on gfx950 such a fence writes back the XCD's L2).
 template <int BS, int N>
-__device__ bool grid_sum(double (&v)[N], const Reducer& red, double (&out)[N]) {
-    const unsigned nblocks = gridDim.x * gridDim.y * gridDim.z;
+__device__ __forceinline__ void block_partials(double (&v)[N], const Reducer& red) {
     const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     block_sum<BS, N>(v);
-    __shared__ int is_last;
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < N; ++k) red.partials[k * red.max_blocks + bid] = v[k];
-        __threadfence();
-        unsigned ticket = atomicAdd(red.counter, 1u);
-        is_last = (ticket == nblocks - 1);
     }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
-    double acc[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-        acc[k] = 0.0;
-        for (unsigned i = threadIdx.x; i < nblocks; i += BS)
-            acc[k] += load_coherent(red.partials + k * red.max_blocks + i);
-    }
-    block_sum<BS, N>(acc);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < N; ++k) out[k] = acc[k];
-        atomicExch(red.counter, 0u);
-        return true;
-    }
-    return false;
 }
 
 __device__ __forceinline__ bool solve_done(const CgState* st) {

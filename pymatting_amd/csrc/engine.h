@@ -10,9 +10,11 @@
 //
 // Operator policies:
 //  - CfOp: the matrix-free closed-form Laplacian (Levin et al. 2007) of
-//    pymatting/laplacian/cf_laplacian.py:5-129. Per 3x3 window k it recomputes
-//    the mean mu_k, the epsilon-regularised covariance and its inverse M_k in
-//    LDS, and uses
+//    pymatting/laplacian/cf_laplacian.py:5-129. The mean mu_k and inverse
+//    regularised covariance M_k of every 3x3 window are computed once per
+//    operator (in the reference's operation order; float32 operators round
+//    the float64 values); each pass stages the image and vector in LDS and
+//    uses
 //       (L x)_i = sum_{k ∋ i} [ x_i - S_k/9 - (I_i - mu_k)^T M_k T_k / 9 ],
 //       S_k = sum_{j ∈ k} x_j,  T_k = sum_{j ∈ k} (I_j - mu_k) x_j,
 //    which is the row sum of the reference's per-window entries
@@ -41,14 +43,31 @@ struct CfDev {
     const uint8_t* cls;  // nullable: 0 unknown, 1 fg, 2 bg
 };
 
+// LDS records: per region pixel {I0, I1, I2, x (masked)}, per window
+// {S/9, mu0, mu1, mu2, v0, v1, v2, valid}; read as 16-byte vectors.
+template <typename T>
+struct alignas(4 * sizeof(T)) Vec4 {
+    T a, b, c, d;
+};
+
 template <typename T, int OX, int OY>
 struct CfShared {
-    T I[3][OY + 4][OX + 4];
+    Vec4<T> px[OY + 4][OX + 4];
+    Vec4<T> win0[OY + 2][OX + 2];  // {S/9, mu0, mu1, mu2}
+    Vec4<T> win1[OY + 2][OX + 2];  // {v0, v1, v2, valid}
     uint8_t cls[OY + 4][OX + 4];
-    T mu[3][OY + 2][OX + 2];
-    T v[3][OY + 2][OX + 2];
-    T a[OY + 2][OX + 2];
-    uint8_t ok[OY + 2][OX + 2];
+};
+
+// Window coefficients held in registers from the start of the kernel.
+// Both precisions store the window mean and inverse covariance (9 values),
+// float32 ones rounded from the float64 values: recomputing the mean in
+// float32 perturbs nearly singular windows far more than the rounding.
+template <typename T, int OX, int OY>
+struct CfPrefetch {
+    static constexpr int NW = ((OX + 2) * (OY + 2) + kBlock - 1) / kBlock;
+    static constexpr int NQ = 9;
+    T cf[NW][NQ];
+    bool geo[NW];
 };
 
 template <typename T>
@@ -56,6 +75,8 @@ struct CfOp {
     static constexpr int kProfBase = kProfCfApply;
     CfDev d;
     const T* I;  // planar [3][n]
+    // Per-window data at the centre pixel [9][n]: mean (3), inverse covariance (6).
+    const T* W;
     int force_mode = -1;  // >= 0 overrides d.mode (the masked right-hand side uses plain L)
 
     template <int OX, int OY>
@@ -63,31 +84,57 @@ struct CfOp {
 
     __device__ __forceinline__ int mode() const { return force_mode >= 0 ? force_mode : d.mode; }
 
+    // Stages the image, the (masked) input and the class map at region point
+    // (ly, lx). In masked mode known pixels enter the window sums as 0.
     template <int OX, int OY>
-    __device__ void load(Shared<OX, OY>& sh, int ox0, int oy0) const {
-        constexpr int RX = OX + 4, RY = OY + 4;
+    __device__ __forceinline__ void load_point(Shared<OX, OY>& sh, int ly, int lx, bool in,
+                                               long long g, T x) const {
         const long long n = (long long)d.h * d.w;
-        for (int t = threadIdx.x; t < RX * RY; t += kBlock) {
-            const int ly = t / RX, lx = t - ly * RX;
-            const int gx = ox0 - 2 + lx, gy = oy0 - 2 + ly;
-            const bool in = gx >= 0 && gx < d.w && gy >= 0 && gy < d.h;
-            const long long g = (long long)gy * d.w + gx;
+        const uint8_t c = (in && d.cls) ? d.cls[g] : 0;
+        Vec4<T> v;
+        v.a = in ? I[g] : T(0);
+        v.b = in ? I[n + g] : T(0);
+        v.c = in ? I[2 * n + g] : T(0);
+        v.d = (mode() == HM_OP_MASKED && c != 0) ? T(0) : x;
+        sh.px[ly][lx] = v;
+        sh.cls[ly][lx] = c;
+    }
+
+    template <int OX, int OY>
+    using Prefetch = CfPrefetch<T, OX, OY>;
+
+    // Loads the coefficients of this thread's windows into registers; issued
+    // before the region loads so both latencies overlap.
+    template <int OX, int OY>
+    __device__ __forceinline__ void prefetch(Prefetch<OX, OY>& pf, int ox0, int oy0) const {
+        constexpr int WX = OX + 2, WY = OY + 2;
+        constexpr int NW = Prefetch<OX, OY>::NW, NQ = Prefetch<OX, OY>::NQ;
+        const long long n = (long long)d.h * d.w;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) sh.I[c][ly][lx] = in ? I[c * n + g] : T(0);
-            sh.cls[ly][lx] = (in && d.cls) ? d.cls[g] : 0;
+        for (int k = 0; k < NW; ++k) {
+            const int t = threadIdx.x + k * kBlock;
+            const int wy = t / WX, wx = t - wy * WX;
+            const int cx = ox0 - 1 + wx, cy = oy0 - 1 + wy;
+            const bool ok = t < WX * WY && cx >= 1 && cx <= d.w - 2 && cy >= 1 && cy <= d.h - 2;
+            pf.geo[k] = ok;
+            const long long gc = ok ? (long long)cy * d.w + cx : 0;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) pf.cf[k][q] = ok ? W[q * n + gc] : T(0);
         }
     }
 
-    // Window coefficients for all (OX+2) x (OY+2) windows touching the tile.
+    // Window records for all (OX+2) x (OY+2) windows touching the tile.
     template <int OX, int OY>
-    __device__ void prepare(Shared<OX, OY>& sh, const T (&sx)[OY + 4][OX + 4], int ox0,
-                            int oy0) const {
+    __device__ void prepare(Shared<OX, OY>& sh, const Prefetch<OX, OY>& pf) const {
         constexpr int WX = OX + 2, WY = OY + 2;
-        const bool masked = mode() == HM_OP_MASKED;
-        for (int t = threadIdx.x; t < WX * WY; t += kBlock) {
+        constexpr int NW = Prefetch<OX, OY>::NW, NQ = Prefetch<OX, OY>::NQ;
+        constexpr int M0 = NQ - 6;  // index of m00
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+            const int t = threadIdx.x + k * kBlock;
+            if (t >= WX * WY) continue;
             const int wy = t / WX, wx = t - wy * WX;
-            const int cx = ox0 - 1 + wx, cy = oy0 - 1 + wy;
-            bool valid = cx >= 1 && cx <= d.w - 2 && cy >= 1 && cy <= d.h - 2;
+            bool valid = pf.geo[k];
             if (valid && d.skip_known && d.cls) {
                 bool all_known = true;
 #pragma unroll
@@ -96,83 +143,76 @@ struct CfOp {
                     for (int dx = 0; dx < 3; ++dx) all_known &= sh.cls[wy + dy][wx + dx] != 0;
                 valid = !all_known;
             }
-            sh.ok[wy][wx] = valid;
-            if (!valid) continue;
-            T c[3][9];
+            Vec4<T> p[9];
+#pragma unroll
+            for (int j = 0; j < 9; ++j) p[j] = sh.px[wy + j / 3][wx + j % 3];
             T mu[3];
+            if constexpr (NQ == 9) {
+                mu[0] = pf.cf[k][0];
+                mu[1] = pf.cf[k][1];
+                mu[2] = pf.cf[k][2];
+            } else {
+                const T inv9 = T(1) / T(9);
+                T s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
-            for (int ch = 0; ch < 3; ++ch) {
-                T s = 0;
-#pragma unroll
-                for (int k = 0; k < 9; ++k) s += sh.I[ch][wy + k / 3][wx + k % 3];
-                mu[ch] = s / T(9);
-#pragma unroll
-                for (int k = 0; k < 9; ++k) c[ch][k] = sh.I[ch][wy + k / 3][wx + k % 3] - mu[ch];
-            }
-            T a00 = T(d.eps), a01 = 0, a02 = 0, a11 = T(d.eps), a12 = 0, a22 = T(d.eps);
-#pragma unroll
-            for (int k = 0; k < 9; ++k) {
-                a00 += c[0][k] * c[0][k];
-                a01 += c[0][k] * c[1][k];
-                a02 += c[0][k] * c[2][k];
-                a11 += c[1][k] * c[1][k];
-                a12 += c[1][k] * c[2][k];
-                a22 += c[2][k] * c[2][k];
+                for (int j = 0; j < 9; ++j) {
+                    s0 += p[j].a;
+                    s1 += p[j].b;
+                    s2 += p[j].c;
+                }
+                mu[0] = s0 * inv9;
+                mu[1] = s1 * inv9;
+                mu[2] = s2 * inv9;
             }
             const T inv9 = T(1) / T(9);
-            a00 *= inv9; a01 *= inv9; a02 *= inv9; a11 *= inv9; a12 *= inv9; a22 *= inv9;
-            // Inverse by cofactors (cf_laplacian.py:76-92 uses the same closed form).
-            const T k00 = a11 * a22 - a12 * a12;
-            const T k01 = a02 * a12 - a01 * a22;
-            const T k02 = a01 * a12 - a02 * a11;
-            const T det = a00 * k00 + a01 * k01 + a02 * k02;
-            const T id = T(1) / det;
-            const T m00 = k00 * id, m01 = k01 * id, m02 = k02 * id;
-            const T m11 = (a00 * a22 - a02 * a02) * id;
-            const T m12 = (a01 * a02 - a00 * a12) * id;
-            const T m22 = (a00 * a11 - a01 * a01) * id;
             T S = 0, T0 = 0, T1 = 0, T2 = 0;
 #pragma unroll
-            for (int k = 0; k < 9; ++k) {
-                const int ry = wy + k / 3, rx = wx + k % 3;
-                T xj = sx[ry][rx];
-                if (masked && sh.cls[ry][rx] != 0) xj = 0;
-                S += xj;
-                T0 += c[0][k] * xj;
-                T1 += c[1][k] * xj;
-                T2 += c[2][k] * xj;
+            for (int j = 0; j < 9; ++j) {
+                S += p[j].d;
+                T0 += (p[j].a - mu[0]) * p[j].d;
+                T1 += (p[j].b - mu[1]) * p[j].d;
+                T2 += (p[j].c - mu[2]) * p[j].d;
             }
-            sh.mu[0][wy][wx] = mu[0];
-            sh.mu[1][wy][wx] = mu[1];
-            sh.mu[2][wy][wx] = mu[2];
-            sh.v[0][wy][wx] = (m00 * T0 + m01 * T1 + m02 * T2) * inv9;
-            sh.v[1][wy][wx] = (m01 * T0 + m11 * T1 + m12 * T2) * inv9;
-            sh.v[2][wy][wx] = (m02 * T0 + m12 * T1 + m22 * T2) * inv9;
-            sh.a[wy][wx] = S * inv9;
+            // Invalid windows get all-zero records, so the output loop needs
+            // no branch.
+            const T okf = valid ? T(1) : T(0);
+            const T m00 = pf.cf[k][M0], m01 = pf.cf[k][M0 + 1], m02 = pf.cf[k][M0 + 2];
+            const T m11 = pf.cf[k][M0 + 3], m12 = pf.cf[k][M0 + 4], m22 = pf.cf[k][M0 + 5];
+            Vec4<T> r0, r1;
+            r0.a = okf * S * inv9;
+            r0.b = mu[0];
+            r0.c = mu[1];
+            r0.d = mu[2];
+            r1.a = okf * (m00 * T0 + m01 * T1 + m02 * T2) * inv9;
+            r1.b = okf * (m01 * T0 + m11 * T1 + m12 * T2) * inv9;
+            r1.c = okf * (m02 * T0 + m12 * T1 + m22 * T2) * inv9;
+            r1.d = okf;
+            sh.win0[wy][wx] = r0;
+            sh.win1[wy][wx] = r1;
         }
     }
 
     // (A x) at tile point (lx, ly); xi is the unmasked input at that point.
     template <int OX, int OY>
-    __device__ __forceinline__ T apply(const Shared<OX, OY>& sh, const T (&sx)[OY + 4][OX + 4],
+    __device__ __forceinline__ T apply(const Shared<OX, OY>& sh, const T (&)[OY + 4][OX + 4],
                                        int lx, int ly, T xi, long long) const {
         const int m = mode();
         const uint8_t ci = sh.cls[ly + 2][lx + 2];
-        if (m == HM_OP_MASKED && ci != 0) return xi;
-        const T I0 = sh.I[0][ly + 2][lx + 2], I1 = sh.I[1][ly + 2][lx + 2],
-                I2 = sh.I[2][ly + 2][lx + 2];
-        T acc = 0;
+        const Vec4<T> p = sh.px[ly + 2][lx + 2];
+        T acc = 0, cnt = 0;
 #pragma unroll
         for (int dy = 0; dy < 3; ++dy) {
 #pragma unroll
             for (int dx = 0; dx < 3; ++dx) {
-                const int wy = ly + dy, wx = lx + dx;
-                if (!sh.ok[wy][wx]) continue;
-                acc += xi - sh.a[wy][wx] - (I0 - sh.mu[0][wy][wx]) * sh.v[0][wy][wx] -
-                       (I1 - sh.mu[1][wy][wx]) * sh.v[1][wy][wx] -
-                       (I2 - sh.mu[2][wy][wx]) * sh.v[2][wy][wx];
+                const Vec4<T> r0 = sh.win0[ly + dy][lx + dx];
+                const Vec4<T> r1 = sh.win1[ly + dy][lx + dx];
+                acc -= r0.a + (p.a - r0.b) * r1.a + (p.b - r0.c) * r1.b + (p.c - r0.d) * r1.c;
+                cnt += r1.d;
             }
         }
+        // Invalid windows have zero records; each valid one adds x_i.
+        acc += cnt * xi;
+        if (m == HM_OP_MASKED && ci != 0) return xi;
         if (m == HM_OP_LAMBDA && ci != 0) acc += T(d.lambda) * xi;
         return acc;
     }
@@ -192,9 +232,13 @@ struct DiaOp {
     using Shared = NoShared<OX, OY>;
 
     template <int OX, int OY>
-    __device__ void load(Shared<OX, OY>&, int, int) const {}
+    __device__ __forceinline__ void load_point(Shared<OX, OY>&, int, int, bool, long long, T) const {}
     template <int OX, int OY>
-    __device__ void prepare(Shared<OX, OY>&, const T (&)[OY + 4][OX + 4], int, int) const {}
+    using Prefetch = NoShared<OX, OY>;
+    template <int OX, int OY>
+    __device__ __forceinline__ void prefetch(Prefetch<OX, OY>&, int, int) const {}
+    template <int OX, int OY>
+    __device__ void prepare(Shared<OX, OY>&, const Prefetch<OX, OY>&) const {}
 
     template <int OX, int OY>
     __device__ __forceinline__ T apply(const Shared<OX, OY>&, const T (&sx)[OY + 4][OX + 4],
@@ -273,11 +317,25 @@ struct DotSink {
     int kind;      // see dot_finalize
 };
 
-enum DotKind { kDotPQ = 0, kDotRzFirst = 1, kDotRzUpdate = 2, kDotAux = 3 };
+enum DotKind {
+    kDotPQ = 0, kDotRzFirst = 1, kDotRzUpdate = 2, kDotAux = 3, kDotUpdate = 4, kDotReorth = 5
+};
 
-__device__ __forceinline__ void dot_finalize(CgState* st, int kind, double s) {
+// Applies a finished grid-wide sum to the solve state (cg.py:90-110).
+__device__ __forceinline__ void dot_finalize(CgState* st, int kind, double s, double s2 = 0.0) {
     if (!isfinite(s)) st->nonfinite = 1;
     switch (kind) {
+        case kDotUpdate: {  // ||r||^2 after the update, and the test of cg.py:99
+            const double norm_r = sqrt(s);
+            st->rr = s;
+            st->iters += 1;
+            if (norm_r < st->atol || norm_r < st->rtol * st->norm_b) st->done = 1;
+            break;
+        }
+        case kDotReorth:  // Polak-Ribiere: s = <r, z>, s2 = <r - r_old, z>
+            st->beta = s2 / st->rz;
+            st->rz = s;
+            break;
         case kDotPQ:
             st->pq = s;
             st->alpha = st->rz / s;  // cg.py:90
@@ -400,22 +458,35 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(Op op, In in, Out out, int
     __shared__ typename Op::template Shared<OX, OY> sh;
     __shared__ T sres[IsRestrict<Out>::value ? OX * OY : 1];
     const int ox0 = out.ox0(blockIdx.x), oy0 = out.oy0(blockIdx.y);
+    typename Op::template Prefetch<OX, OY> pf;
+    op.template prefetch<OX, OY>(pf, ox0, oy0);
 
-    for (int t = threadIdx.x; t < RX * RY; t += kBlock) {
-        const int ly = t / RX, lx = t - ly * RX;
-        const int gx = ox0 - 2 + lx, gy = oy0 - 2 + ly;
-        const bool inb = gx >= 0 && gx < w && gy >= 0 && gy < h;
-        sx[ly][lx] = inb ? in.load(gx, gy, (long long)gy * w + gx) : T(0);
+    constexpr int NR = (RX * RY + kBlock - 1) / kBlock;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        const int t = threadIdx.x + k * kBlock;
+        if (t < RX * RY) {
+            const int ly = t / RX, lx = t - ly * RX;
+            const int gx = ox0 - 2 + lx, gy = oy0 - 2 + ly;
+            const bool inb = gx >= 0 && gx < w && gy >= 0 && gy < h;
+            const long long g = inb ? (long long)gy * w + gx : 0;
+            const T xv = inb ? in.load(gx, gy, g) : T(0);
+            sx[ly][lx] = xv;
+            op.template load_point<OX, OY>(sh, ly, lx, inb, g, xv);
+        }
     }
-    op.template load<OX, OY>(sh, ox0, oy0);
     __syncthreads();
-    op.template prepare<OX, OY>(sh, sx, ox0, oy0);
+    op.template prepare<OX, OY>(sh, pf);
     __syncthreads();
 
     Out o = out;
     if constexpr (IsRestrict<Out>::value) o.sres = sres;
     double acc = 0.0;
-    for (int t = threadIdx.x; t < OX * OY; t += kBlock) {
+    constexpr int NO = (OX * OY + kBlock - 1) / kBlock;
+#pragma unroll
+    for (int k = 0; k < NO; ++k) {
+        const int t = threadIdx.x + k * kBlock;
+        if (t >= OX * OY) continue;
         const int ly = t / OX, lx = t - ly * OX;
         const int gx = ox0 + lx, gy = oy0 + ly;
         if (gx < 0 || gx >= w || gy < 0 || gy >= h) {
@@ -432,9 +503,26 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(Op op, In in, Out out, int
         o.restrict_tile(blockIdx.x, blockIdx.y);
     }
     if (o.dot.st != nullptr) {
-        double v[1] = {acc}, tot[1];
-        if (grid_sum<kBlock, 1>(v, o.dot.red, tot)) dot_finalize(o.dot.st, o.dot.kind, tot[0]);
+        double v[1] = {acc};
+        block_partials<kBlock, 1>(v, o.dot.red);
     }
+}
+
+// Stage two of every reduction: sums nblocks partials per value in a fixed
+// order and applies the result to the solve state.
+template <int N>
+__global__ __launch_bounds__(kBlock) void k_finalize(Reducer red, unsigned nblocks, CgState* st,
+                                                     int kind, const CgState* guard) {
+    if (solve_done(guard)) return;
+    double acc[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        acc[k] = 0.0;
+        for (unsigned i = threadIdx.x; i < nblocks; i += kBlock)
+            acc[k] += red.partials[k * red.max_blocks + i];
+    }
+    block_sum<kBlock, N>(acc);
+    if (threadIdx.x == 0) dot_finalize(st, kind, acc[0], N > 1 ? acc[N > 1 ? 1 : 0] : 0.0);
 }
 
 template <class Out>

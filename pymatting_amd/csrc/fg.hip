@@ -289,11 +289,25 @@ __global__ __launch_bounds__(kBlockMean) void k_fg_means(const S* __restrict__ i
             v[7] += 1.0;
         }
     }
-    double tot[8];
-    if (grid_sum<kBlockMean, 8>(v, red, tot)) {
+    block_partials<kBlockMean, 8>(v, red);
+}
+
+// Mean colours from the partial sums, divided by count + 1e-5 in float64.
+template <typename S>
+__global__ __launch_bounds__(kBlockMean) void k_fg_means_final(Reducer red, unsigned nblocks,
+                                                               S* __restrict__ means) {
+    double acc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        acc[k] = 0.0;
+        for (unsigned i = threadIdx.x; i < nblocks; i += kBlockMean)
+            acc[k] += red.partials[k * red.max_blocks + i];
+    }
+    block_sum<kBlockMean, 8>(acc);
+    if (threadIdx.x == 0) {
         for (int c = 0; c < 3; ++c) {
-            means[c] = (S)(tot[c] / (tot[3] + 1e-5));
-            means[3 + c] = (S)(tot[4 + c] / (tot[7] + 1e-5));
+            means[c] = (S)(acc[c] / (acc[3] + 1e-5));
+            means[3 + c] = (S)(acc[4 + c] / (acc[7] + 1e-5));
         }
     }
 }
@@ -349,6 +363,7 @@ int fb_ml_t(const S* img0, const S* alpha0, int h0, int w0, int n_levels, const 
     hipError_t e = hipMemsetAsync(fail, 0, sizeof(int), s);
     if (e != hipSuccess) { cleanup(); set_error("hipMemsetAsync: %s", hipGetErrorString(e)); return HM_ERR_HIP; }
     k_fg_means<S><<<mean_blocks, kBlockMean, 0, s>>>(img0, alpha0, n0, red, static_cast<S*>(means));
+    k_fg_means_final<S><<<1, kBlockMean, 0, s>>>(red, mean_blocks, static_cast<S*>(means));
     // Level "-1": 1x1 F/B initialised with the mean colours.
     S* Fp = static_cast<S*>(Fa);
     S* Bp = static_cast<S*>(Ba);

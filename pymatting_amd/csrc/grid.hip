@@ -6,6 +6,9 @@
 //  - CSR SpMV for CG on arbitrary matrices (cg.py:68-71, 85-88).
 #include <cstdarg>
 #include <cstring>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
 
 #include "common.h"
 #include "engine.h"
@@ -15,23 +18,85 @@ namespace hm {
 
 // ------------------------------------------------------------ device memory
 
+// Device allocations go through a caching pool: a solve allocates dozens of
+// buffers and hipMalloc/hipFree (which synchronises the device) would
+// dominate small and medium solves. Freed blocks are kept per rounded size
+// and reused in stream order; the library issues all work on the caller's
+// stream, so a reused block's earlier uses are ordered before its new ones.
+namespace {
+std::mutex g_pool_mu;
+std::unordered_map<size_t, std::vector<void*>> g_pool_free;
+std::unordered_map<void*, size_t> g_pool_size;
+
+size_t pool_round(size_t bytes) {
+    if (bytes < 256) return 256;
+    if (bytes < (1u << 20)) {
+        size_t r = 256;
+        while (r < bytes) r <<= 1;
+        return r;
+    }
+    const size_t mb2 = size_t(2) << 20;
+    return (bytes + mb2 - 1) / mb2 * mb2;
+}
+}  // namespace
+
 int dev_alloc(void** p, size_t bytes) {
     *p = nullptr;
-    if (bytes == 0) bytes = 8;
-    HM_HIP(hipMalloc(p, bytes));
+    const size_t r = pool_round(bytes);
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        auto it = g_pool_free.find(r);
+        if (it != g_pool_free.end() && !it->second.empty()) {
+            *p = it->second.back();
+            it->second.pop_back();
+            return HM_OK;
+        }
+    }
+    hipError_t e = hipMalloc(p, r);
+    if (e != hipSuccess) {
+        // Release cached blocks and retry once.
+        pool_trim();
+        e = hipMalloc(p, r);
+    }
+    if (e != hipSuccess) {
+        set_error("hipMalloc(%zu) failed: %s", r, hipGetErrorString(e));
+        *p = nullptr;
+        return HM_ERR_HIP;
+    }
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    g_pool_size[*p] = r;
     return HM_OK;
 }
 
 void dev_free(void* p) {
-    if (p) (void)hipFree(p);
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    auto it = g_pool_size.find(p);
+    if (it == g_pool_size.end()) return;
+    g_pool_free[it->second].push_back(p);
+}
+
+void pool_trim() {
+    std::vector<void*> victims;
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        for (auto& kv : g_pool_free) {
+            for (void* q : kv.second) {
+                victims.push_back(q);
+                g_pool_size.erase(q);
+            }
+            kv.second.clear();
+        }
+    }
+    if (!victims.empty()) (void)hipDeviceSynchronize();
+    for (void* q : victims) (void)hipFree(q);
 }
 
 int make_reducer(Reducer* r, int max_blocks, int nvals) {
     r->max_blocks = max_blocks;
     HM_TRY(dev_alloc(reinterpret_cast<void**>(&r->partials),
                      sizeof(double) * (size_t)max_blocks * nvals));
-    HM_TRY(dev_alloc(reinterpret_cast<void**>(&r->counter), sizeof(unsigned) * 4));
-    HM_HIP(hipMemset(r->counter, 0, sizeof(unsigned) * 4));
+    r->counter = nullptr;
     return HM_OK;
 }
 
@@ -189,11 +254,19 @@ __global__ __launch_bounds__(1024) void k_gauss_jordan(int n, double* __restrict
         best_v[threadIdx.x] = bv;
         best_i[threadIdx.x] = bi;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int t = 1; t < (int)blockDim.x; ++t)
-                if (best_v[t] > best_v[0]) { best_v[0] = best_v[t]; best_i[0] = best_i[t]; }
-            if (!(best_v[0] > 0.0)) *status = 1;
+        // Tree argmax; ties keep the lower row like a serial scan.
+        for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+            if ((int)threadIdx.x < off) {
+                const double ov = best_v[threadIdx.x + off];
+                const int oi = best_i[threadIdx.x + off];
+                if (ov > best_v[threadIdx.x] || (ov == best_v[threadIdx.x] && oi < best_i[threadIdx.x])) {
+                    best_v[threadIdx.x] = ov;
+                    best_i[threadIdx.x] = oi;
+                }
+            }
+            __syncthreads();
         }
+        if (threadIdx.x == 0 && !(best_v[0] > 0.0)) *status = 1;
         __syncthreads();
         const int piv = best_i[0];
         if (piv != k)
@@ -287,8 +360,8 @@ __global__ __launch_bounds__(kBlock) void k_csr_apply(const int64_t* __restrict_
         acc = x[i] * s;
     }
     if (st != nullptr && red.partials != nullptr) {
-        double v[1] = {acc}, tot[1];
-        if (grid_sum<kBlock, 1>(v, red, tot)) dot_finalize(st, kind, tot[0]);
+        double v[1] = {acc};
+        block_partials<kBlock, 1>(v, red);
     }
 }
 
@@ -302,6 +375,21 @@ int csr_apply(const hm_csr* A, const double* x, double* y, CgState* st, Reducer*
     }
     k_csr_apply<<<g, kBlock, 0, s>>>(A->indptr, A->indices, A->values, A->n, x, y, red ? st : nullptr,
                                      r, kind);
+    HM_LAUNCH_CHECK();
+    if (red && st) return launch_finalize(1, r, g, st, kind, st, s);
+    return HM_OK;
+}
+
+int launch_finalize(int nvals, const Reducer& red, unsigned nblocks, CgState* st, int kind,
+                    const CgState* guard, hipStream_t s) {
+    if ((int)nblocks > red.max_blocks) {
+        set_error("reduction buffer too small for %u blocks", nblocks);
+        return HM_ERR_ARG;
+    }
+    if (nvals == 1)
+        k_finalize<1><<<1, kBlock, 0, s>>>(red, nblocks, st, kind, guard);
+    else
+        k_finalize<2><<<1, kBlock, 0, s>>>(red, nblocks, st, kind, guard);
     HM_LAUNCH_CHECK();
     return HM_OK;
 }

@@ -17,7 +17,8 @@ struct hm_cf {
     double* I64 = nullptr;   // planar [3][n]
     float* I32 = nullptr;    // planar [3][n]
     uint8_t* cls = nullptr;  // owned copy of the class map, or null
-    double* wc = nullptr;    // exact per-window data [10][n]
+    double* wc = nullptr;    // exact per-window data [10][n]: mean, inverse covariance, flag
+    float* wc32 = nullptr;   // float32 copy of the first 9 planes
     double* diag64 = nullptr;
     float* diag32 = nullptr;
 };
@@ -97,9 +98,14 @@ int dense_inverse_from_dense(const double* A, int n, double* inv64, float* inv32
 int csr_apply(const hm_csr* A, const double* x, double* y, CgState* st, Reducer* red, int kind,
               hipStream_t s);
 
+// Stage two of a reduction over `nblocks` partials (engine.h k_finalize).
+int launch_finalize(int nvals, const Reducer& red, unsigned nblocks, CgState* st, int kind,
+                    const CgState* guard, hipStream_t s);
+
 // Device memory helpers (hipMalloc-backed).
 int dev_alloc(void** p, size_t bytes);
 void dev_free(void* p);
+void pool_trim();  // returns cached blocks to HIP
 int make_reducer(Reducer* r, int max_blocks, int nvals);
 void free_reducer(Reducer* r);
 

@@ -46,6 +46,8 @@ int launch_tile(const Op& op, const In& in, const Out& out, int h, int w, int h2
     ProfScope prof(Op::kProfBase + Out::kProf, (long long)h * w, sizeof(T) == 4, s);
     tile_kernel<T, Op, In, Out><<<grid, kBlock, 0, s>>>(op, in, out, h, w, guard);
     HM_LAUNCH_CHECK();
+    if (out.dot.st != nullptr)
+        return launch_finalize(1, out.dot.red, grid.x * grid.y, out.dot.st, out.dot.kind, guard, s);
     return HM_OK;
 }
 
@@ -71,6 +73,7 @@ LevelOp<T> cf_level(const hm_cf* op, int force_mode = -1) {
     L.is_cf = true;
     L.cf.d = CfDev{op->h, op->w, op->mode, op->skip_known, op->eps, op->lambda, op->cls};
     L.cf.I = pick<T>(op->I64, op->I32);
+    L.cf.W = pick<T>(op->wc, op->wc32);
     L.cf.force_mode = force_mode;
     L.D = pick<T>(op->diag64, op->diag32);
     L.h = op->h;
@@ -122,15 +125,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(T* __restrict__ x, const T
             acc += (double)ri * (double)ri;
         }
     }
-    double v[1] = {acc}, tot[1];
-    if (grid_sum<kBlock, 1>(v, red, tot)) {
-        const double rr = tot[0];
-        const double norm_r = sqrt(rr);
-        st->rr = rr;
-        st->iters += 1;
-        if (!isfinite(rr)) st->nonfinite = 1;
-        if (norm_r < st->atol || norm_r < st->rtol * st->norm_b) st->done = 1;
-    }
+    double v[1] = {acc};
+    block_partials<kBlock, 1>(v, red);
 }
 
 // p = z + beta p (cg.py:112-113).
@@ -166,8 +162,8 @@ __global__ __launch_bounds__(kBlock) void k_diag_pc(const T* __restrict__ r,
         }
     }
     if (st != nullptr) {
-        double v[1] = {acc}, tot[1];
-        if (grid_sum<kBlock, 1>(v, red, tot)) dot_finalize(st, kind, tot[0]);
+        double v[1] = {acc};
+        block_partials<kBlock, 1>(v, red);
     }
 }
 
@@ -188,15 +184,18 @@ __global__ __launch_bounds__(kBlock) void k_dot(const T* __restrict__ a, const T
             if (c) acc[1] += ((double)a[i] - (double)c[i]) * (double)b[i];
         }
     }
-    double tot[2];
-    if (grid_sum<kBlock, 2>(acc, red, tot)) {
-        if (c) {  // reorthogonalised beta
-            st->beta = tot[1] / st->rz;
-            st->rz = tot[0];
-        } else {
-            dot_finalize(st, kind, tot[0]);
-        }
-    }
+    block_partials<kBlock, 2>(acc, red);
+}
+
+// <a, b> (or the Polak-Ribiere pair when c is given) into the solve state.
+template <typename T>
+int launch_dot(const T* a, const T* b, const T* c, long long n, CgState* st, const Reducer& red,
+               int kind, hipStream_t s) {
+    const unsigned g = vec_blocks(n);
+    k_dot<T><<<g, kBlock, 0, s>>>(a, b, c, n, st, red, kind);
+    HM_LAUNCH_CHECK();
+    return launch_finalize(c ? 2 : 1, red, g, st, c ? kDotReorth : kind,
+                           kind == kDotAux ? nullptr : st, s);
 }
 
 template <typename T>
@@ -239,9 +238,10 @@ __global__ __launch_bounds__(kBlock) void k_dense_apply(const T* __restrict__ in
         x[i] = s;
         acc += (double)b[i] * (double)s;
     }
-    if (st != nullptr) {
-        double v[1] = {acc}, tot[1];
-        if (grid_sum<kBlock, 1>(v, red, tot)) dot_finalize(st, kind, tot[0]);
+    if (st != nullptr) {  // a single workgroup: its block sum is the total
+        double v[1] = {acc};
+        block_sum<kBlock, 1>(v);
+        if (threadIdx.x == 0) dot_finalize(st, kind, v[0]);
     }
 }
 
@@ -284,9 +284,7 @@ int post_stage(const LevelOp<T>& op, const Base& base, const T* b, const T* e, i
         HM_LAUNCH_CHECK();
         if (fin.st != nullptr) {
             const long long n = (long long)op.h * op.w;
-            k_dot<T><<<vec_blocks(n), kBlock, 0, s>>>(r_top, target, nullptr, n, fin.st, red,
-                                                      fin.kind);
-            HM_LAUNCH_CHECK();
+            HM_TRY(launch_dot<T>(r_top, target, nullptr, n, fin.st, red, fin.kind, s));
         }
         return HM_OK;
     }
@@ -301,6 +299,191 @@ int post_stage(const LevelOp<T>& op, const Base& base, const T* b, const T* e, i
             HM_TRY(run_level<T>(op, InVec<T>{prev}, sm, guard, s));
         prev = outp;
     }
+    return HM_OK;
+}
+
+// ------------------------------------------------------------- V-cycle tail
+//
+// The coarsest grids (each at most kTailPoints points) are processed by one
+// 1024-thread workgroup in a single launch: pre-smoothing, residual,
+// restriction, the dense direct solve and the way back up, separated by
+// workgroup barriers. On these grids a separate launch per step would cost
+// more than the arithmetic.
+
+constexpr int kTailPoints = 1024;
+constexpr int kTailMaxLevels = 24;
+constexpr int kTailBlock = 1024;
+
+template <typename T>
+struct TailLevel {
+    int h, w;
+    const T* coef;  // [25][n]
+    const T* D;
+    T* b;
+    T* x;
+    T* t;
+    T* u;
+};
+
+template <typename T>
+struct TailParams {
+    TailLevel<T> lv[kTailMaxLevels];
+    int nlev;        // tail grids including the direct one (last)
+    int pre, post;
+    T omega;
+    const T* inv;    // dense inverse of the last grid
+};
+
+template <typename T>
+__device__ __forceinline__ T tail_Ax(const TailLevel<T>& L, const T* v, int i) {
+    const int n = L.h * L.w;
+    const int y = i / L.w, x = i - y * L.w;
+    T acc = 0;
+#pragma unroll
+    for (int s = 0; s < 25; ++s) {
+        const int yy = y + s / 5 - 2, xx = x + s % 5 - 2;
+        if (yy >= 0 && yy < L.h && xx >= 0 && xx < L.w) acc += L.coef[s * n + i] * v[yy * L.w + xx];
+    }
+    return acc;
+}
+
+template <typename T>
+__device__ __forceinline__ T tail_prolong(const T* e, int h2, int w2, int gx, int gy) {
+    T pe = 0;
+    for (int ey = -1; ey <= 1; ++ey) {
+        if (((gy - ey) & 1) != 0) continue;
+        const int Y = (gy - ey) >> 1;
+        if (Y < 0 || Y >= h2) continue;
+        for (int ex = -1; ex <= 1; ++ex) {
+            if (((gx - ex) & 1) != 0) continue;
+            const int X = (gx - ex) >> 1;
+            if (X < 0 || X >= w2) continue;
+            pe += (ey == 0 ? T(0.5) : T(0.25)) * (ex == 0 ? T(0.5) : T(0.25)) * e[Y * w2 + X];
+        }
+    }
+    return pe;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kTailBlock) void k_vc_tail(TailParams<T> P, const CgState* guard) {
+    if (solve_done(guard)) return;
+    const int tid = threadIdx.x;
+    T* xpre[kTailMaxLevels];
+    // Descend.
+    for (int l = 0; l + 1 < P.nlev; ++l) {
+        const TailLevel<T>& L = P.lv[l];
+        const int n = L.h * L.w;
+        T* cur = L.x;
+        T* nxt = L.t;
+        if (P.pre == 0) {
+            for (int i = tid; i < n; i += kTailBlock) cur[i] = T(0);
+        } else {
+            for (int i = tid; i < n; i += kTailBlock) cur[i] = P.omega * L.b[i] / L.D[i];
+            for (int k = 1; k < P.pre; ++k) {
+                __syncthreads();
+                for (int i = tid; i < n; i += kTailBlock)
+                    nxt[i] = cur[i] + P.omega * (L.b[i] - tail_Ax(L, cur, i)) / L.D[i];
+                T* sw = cur; cur = nxt; nxt = sw;
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i < n; i += kTailBlock) L.u[i] = L.b[i] - tail_Ax(L, cur, i);
+        __syncthreads();
+        const TailLevel<T>& C = P.lv[l + 1];
+        const int nc = C.h * C.w;
+        for (int I = tid; I < nc; I += kTailBlock) {
+            const int Y = I / C.w, X = I - Y * C.w;
+            T acc = 0;
+            for (int ey = -1; ey <= 1; ++ey) {
+                const int y = 2 * Y + ey;
+                if (y < 0 || y >= L.h) continue;
+                for (int ex = -1; ex <= 1; ++ex) {
+                    const int x = 2 * X + ex;
+                    if (x < 0 || x >= L.w) continue;
+                    acc += (ey == 0 ? T(0.5) : T(0.25)) * (ex == 0 ? T(0.5) : T(0.25)) *
+                           L.u[y * L.w + x];
+                }
+            }
+            C.b[I] = acc;
+        }
+        xpre[l] = cur;
+        __syncthreads();
+    }
+    // Direct solve on the last grid.
+    {
+        const TailLevel<T>& C = P.lv[P.nlev - 1];
+        const int nc = C.h * C.w;
+        for (int i = tid; i < nc; i += kTailBlock) {
+            T acc = 0;
+            for (int j = 0; j < nc; ++j) acc += P.inv[(long long)i * nc + j] * C.b[j];
+            C.x[i] = acc;
+        }
+        __syncthreads();
+    }
+    // Ascend.
+    const T* e = P.lv[P.nlev - 1].x;
+    for (int l = P.nlev - 2; l >= 0; --l) {
+        const TailLevel<T>& L = P.lv[l];
+        const TailLevel<T>& C = P.lv[l + 1];
+        const int n = L.h * L.w;
+        T* a = (xpre[l] == L.x) ? L.t : L.x;  // must not overwrite the pre-smoothed iterate
+        for (int i = tid; i < n; i += kTailBlock) {
+            const int y = i / L.w, x = i - y * L.w;
+            a[i] = xpre[l][i] + tail_prolong(e, C.h, C.w, x, y);
+        }
+        T* cur = a;
+        T* nxt = L.u;
+        for (int k = 0; k < P.post; ++k) {
+            __syncthreads();
+            for (int i = tid; i < n; i += kTailBlock)
+                nxt[i] = cur[i] + P.omega * (L.b[i] - tail_Ax(L, cur, i)) / L.D[i];
+            T* sw = cur; cur = nxt; nxt = sw;
+        }
+        __syncthreads();
+        if (l == 0 && cur != L.x) {
+            for (int i = tid; i < n; i += kTailBlock) L.x[i] = cur[i];
+            __syncthreads();
+            cur = L.x;
+        }
+        e = cur;
+    }
+}
+
+// First level index of the tail (levels tail..L-1 are all small), or L if
+// the hierarchy is a single direct level.
+inline size_t vc_tail_start(const hm_vc* vc) {
+    const size_t L = vc->levels.size();
+    if (L <= 1) return L;
+    size_t t = L - 1;
+    while (t > 1 && vc->levels[t - 1].n <= kTailPoints && L - (t - 1) <= (size_t)kTailMaxLevels)
+        --t;
+    return t;
+}
+
+template <typename T>
+int launch_tail(hm_vc* vc, size_t start, const CgState* guard, hipStream_t s) {
+    TailParams<T> P{};
+    const size_t L = vc->levels.size();
+    P.nlev = (int)(L - start);
+    P.pre = vc->pre;
+    P.post = vc->post;
+    P.omega = (T)vc->omega;
+    P.inv = pick<T>(vc->dense_inv64, vc->dense_inv32);
+    for (size_t l = start; l < L; ++l) {
+        const VcLevel& V = vc->levels[l];
+        TailLevel<T>& t = P.lv[l - start];
+        t.h = V.h;
+        t.w = V.w;
+        t.coef = pick<T>(V.A->c64, V.A->c32);
+        t.D = pick<T>(V.A->diag64, V.A->diag32);
+        t.b = static_cast<T*>(V.b);
+        t.x = static_cast<T*>(V.x);
+        t.t = static_cast<T*>(V.t);
+        t.u = static_cast<T*>(V.u);
+    }
+    ProfScope prof(kProfDense, vc->levels[start].n, sizeof(T) == 4, s);
+    k_vc_tail<T><<<1, kTailBlock, 0, s>>>(P, guard);
+    HM_LAUNCH_CHECK();
     return HM_OK;
 }
 
@@ -320,8 +503,9 @@ int vcycle_apply_t(hm_vc* vc, const T* r, T* z, CgState* st, int kind, const CgS
         return HM_OK;
     }
     std::vector<T*> result(L, nullptr), pre_x(L, nullptr);
+    const size_t tail = vc_tail_start(vc);
     // Descend: pre-smooth, residual, restrict.
-    for (size_t l = 0; l + 1 < L; ++l) {
+    for (size_t l = 0; l + 1 < L && l < tail; ++l) {
         const VcLevel& lev = vc->levels[l];
         const VcLevel& cl = vc->levels[l + 1];
         const LevelOp<T> op = vc_level_op<T>(vc, l);
@@ -345,16 +529,19 @@ int vcycle_apply_t(hm_vc* vc, const T* r, T* z, CgState* st, int kind, const CgS
             pre_x[l] = cur;
         }
     }
-    // Coarsest grid: direct solve (vcycle.py:59-60).
-    {
+    // Coarse tail in one launch (its top grid's correction lands in .x), or
+    // the direct solve alone (vcycle.py:59-60).
+    if (tail < L - 1) {
+        HM_TRY(launch_tail<T>(vc, tail, guard, s));
+    } else {
         const VcLevel& cl = vc->levels[L - 1];
         k_dense_apply<T><<<1, kBlock, 0, s>>>(vc_inv<T>(vc), vptr<T>(cl.b), vptr<T>(cl.x),
                                               vc->n_direct, nullptr, Reducer{}, 0, guard);
         HM_LAUNCH_CHECK();
-        result[L - 1] = vptr<T>(cl.x);
     }
+    result[tail] = vptr<T>(vc->levels[tail].x);
     // Ascend: prolongate the correction and post-smooth (vcycle.py:95-100).
-    for (size_t l = L - 1; l-- > 0;) {
+    for (size_t l = tail; l-- > 0;) {
         const VcLevel& lev = vc->levels[l];
         const VcLevel& cl = vc->levels[l + 1];
         const LevelOp<T> op = vc_level_op<T>(vc, l);
@@ -463,9 +650,7 @@ int op_apply(const OpHandle& A, const T* x, T* y, const DotSink& ds, const CgSta
         HM_TRY(host_call(A.cb, A.cb_ctx, x, y, s));
         if (ds.st) {
             const long long n = A.n();
-            k_dot<double><<<vec_blocks(n), kBlock, 0, s>>>(x, y, nullptr, n, ds.st, ds.red,
-                                                           ds.kind);
-            HM_LAUNCH_CHECK();
+            HM_TRY(launch_dot<double>(x, y, nullptr, n, ds.st, ds.red, ds.kind, s));
         }
         return HM_OK;
     }
@@ -509,8 +694,7 @@ int pc_apply(const PcHandle& M, const T* r, T* z, CgState* st, const Reducer& re
             else
                 HM_TRY(host_call(M.cb, M.cb_ctx, r, z, s));
             if (st) {
-                k_dot<double><<<vec_blocks(n), kBlock, 0, s>>>(r, z, nullptr, n, st, red, kind);
-                HM_LAUNCH_CHECK();
+                HM_TRY(launch_dot<double>(r, z, nullptr, n, st, red, kind, s));
             }
             return HM_OK;
         }
@@ -520,21 +704,33 @@ int pc_apply(const PcHandle& M, const T* r, T* z, CgState* st, const Reducer& re
     k_diag_pc<T><<<vec_blocks(n), kBlock, 0, s>>>(r, M.kind == HM_PC_JACOBI ? M.inv_diag : nullptr,
                                                   z, n, st, red, kind);
     HM_LAUNCH_CHECK();
+    if (st) HM_TRY(launch_finalize(1, red, vec_blocks(n), st, kind, guard, s));
     return HM_OK;
 }
 
+// Device copy of the solve state plus a pinned host mirror (cached per
+// thread: hipHostMalloc is slow).
 struct DeviceState {
     CgState* dev = nullptr;
-    CgState* host = nullptr;  // pinned
+    CgState* host = nullptr;
     int init() {
+        static thread_local CgState* pinned = nullptr;
+        if (!pinned) HM_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned), sizeof(CgState) * 16,
+                                          hipHostMallocDefault));
+        static thread_local int depth = 0;
+        if (depth >= 16) {
+            set_error("solver nesting too deep");
+            return HM_ERR_ARG;
+        }
+        host = pinned + depth++;
+        depth_ = &depth;
         HM_TRY(dev_alloc(reinterpret_cast<void**>(&dev), sizeof(CgState)));
-        HM_HIP(hipHostMalloc(reinterpret_cast<void**>(&host), sizeof(CgState), hipHostMallocDefault));
         std::memset(host, 0, sizeof(CgState));
         return HM_OK;
     }
     ~DeviceState() {
         dev_free(dev);
-        if (host) (void)hipHostFree(host);
+        if (depth_) --*depth_;
     }
     int push(hipStream_t s) {
         HM_HIP(hipMemcpyAsync(dev, host, sizeof(CgState), hipMemcpyHostToDevice, s));
@@ -545,6 +741,7 @@ struct DeviceState {
         HM_HIP(hipStreamSynchronize(s));
         return HM_OK;
     }
+    int* depth_ = nullptr;
 };
 
 struct DevBuf {
@@ -591,8 +788,7 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
     const bool interactive = A.cb || M.kind == HM_PC_HOST || opt.callback;
 
     // norm_b = ||b|| (cg.py:66)
-    k_dot<T><<<vb, kBlock, 0, s>>>(b, b, nullptr, n, st.dev, red, kDotAux);
-    HM_LAUNCH_CHECK();
+    HM_TRY(launch_dot<T>(b, b, nullptr, n, st.dev, red, kDotAux, s));
     HM_TRY(st.pull(s));
     const double norm_b = std::sqrt(st.host->aux);
     // x = x0 or 0; r = b - A x (cg.py:64-71)
@@ -610,8 +806,7 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
             HM_LAUNCH_CHECK();
         }
     }
-    k_dot<T><<<vb, kBlock, 0, s>>>(r.as<T>(), r.as<T>(), nullptr, n, st.dev, red, kDotAux);
-    HM_LAUNCH_CHECK();
+    HM_TRY(launch_dot<T>(r.as<T>(), r.as<T>(), nullptr, n, st.dev, red, kDotAux, s));
     HM_TRY(st.pull(s));
     const double norm_r = std::sqrt(st.host->aux);
     stats->norm_b = norm_b;
@@ -646,6 +841,7 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
                 k_cg_update<T><<<vb, kBlock, 0, s>>>(x, p.as<T>(), r.as<T>(), q.as<T>(), n,
                                                      st.dev, red);
                 HM_LAUNCH_CHECK();
+                HM_TRY(launch_finalize(1, red, vb, st.dev, kDotUpdate, st.dev, s));
             }
             if (interactive) {
                 HM_TRY(st.pull(s));
@@ -660,9 +856,7 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
             }
             if (opt.reorth) {
                 HM_TRY(pc_apply<T>(M, r.as<T>(), z.as<T>(), nullptr, red, 0, st.dev, n, s));
-                k_dot<T><<<vb, kBlock, 0, s>>>(r.as<T>(), z.as<T>(), rold.as<T>(), n, st.dev, red,
-                                               kDotRzUpdate);
-                HM_LAUNCH_CHECK();
+                HM_TRY(launch_dot<T>(r.as<T>(), z.as<T>(), rold.as<T>(), n, st.dev, red, kDotRzUpdate, s));
             } else {
                 HM_TRY(pc_apply<T>(M, r.as<T>(), z.as<T>(), st.dev, red, kDotRzUpdate, st.dev, n,
                                    s));
@@ -690,15 +884,16 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
 // Mixed precision: float32 inner PCG (float32 stencil, SpMV and V-cycle
 // smoothing; float64 dot products) inside float64 iterative refinement:
 //   r = b - A x (float64), e = A32^-1 (r / ||r||), x += ||r|| e.
-int pcg_mixed(hm_cf* cf, hm_vc* vc32, const double* b, double* x, double rtol,
+int pcg_mixed(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x, double rtol,
               double inner_rtol, int max_outer, int maxiter, hm_cg_stats* stats, hipStream_t s) {
-    const long long n = cf->n;
+    const long long n = cf ? cf->n : dia->n;
+    const int oh = cf ? cf->h : dia->h, ow = cf ? cf->w : dia->w;
     DevBuf r64, r32, e32;
     HM_TRY(r64.alloc(sizeof(double) * n));
     HM_TRY(r32.alloc(sizeof(float) * n));
     HM_TRY(e32.alloc(sizeof(float) * n));
     Reducer red{};
-    HM_TRY(make_reducer(&red, reducer_blocks(n, cf->h, cf->w), 2));
+    HM_TRY(make_reducer(&red, reducer_blocks(n, oh, ow), 2));
     struct RedGuard { Reducer* r; ~RedGuard() { free_reducer(r); } } rg{&red};
     DeviceState st;
     HM_TRY(st.init());
@@ -706,12 +901,12 @@ int pcg_mixed(hm_cf* cf, hm_vc* vc32, const double* b, double* x, double rtol,
     const unsigned vb = vec_blocks(n);
     OpHandle A;
     A.cf = cf;
+    A.dia = dia;
     PcHandle M;
     M.kind = HM_PC_VCYCLE;
     M.vc = vc32;
 
-    k_dot<double><<<vb, kBlock, 0, s>>>(b, b, nullptr, n, st.dev, red, kDotAux);
-    HM_LAUNCH_CHECK();
+    HM_TRY(launch_dot<double>(b, b, nullptr, n, st.dev, red, kDotAux, s));
     HM_TRY(st.pull(s));
     const double norm_b = std::sqrt(st.host->aux);
     k_fill<double><<<div_up(n, 256), 256, 0, s>>>(x, 0.0, n);
@@ -722,9 +917,7 @@ int pcg_mixed(hm_cf* cf, hm_vc* vc32, const double* b, double* x, double rtol,
     stats->norm_b = norm_b;
     stats->converged = 0;
     for (int outer = 0; outer <= max_outer; ++outer) {
-        k_dot<double><<<vb, kBlock, 0, s>>>(r64.as<double>(), r64.as<double>(), nullptr, n,
-                                            st.dev, red, kDotAux);
-        HM_LAUNCH_CHECK();
+        HM_TRY(launch_dot<double>(r64.as<double>(), r64.as<double>(), nullptr, n, st.dev, red, kDotAux, s));
         HM_TRY(st.pull(s));
         const double norm_r = std::sqrt(st.host->aux);
         stats->norm_r = norm_r;
@@ -944,6 +1137,7 @@ int hm_cf_create(const double* image_hwc, int h, int w, double epsilon, int radi
     int rc = dev_alloc(reinterpret_cast<void**>(&op->I64), sizeof(double) * 3 * n);
     if (rc == HM_OK) rc = dev_alloc(reinterpret_cast<void**>(&op->I32), sizeof(float) * 3 * n);
     if (rc == HM_OK) rc = dev_alloc(reinterpret_cast<void**>(&op->wc), sizeof(double) * 10 * n);
+    if (rc == HM_OK) rc = dev_alloc(reinterpret_cast<void**>(&op->wc32), sizeof(float) * 9 * n);
     if (rc == HM_OK) rc = dev_alloc(reinterpret_cast<void**>(&op->diag64), sizeof(double) * n);
     if (rc == HM_OK) rc = dev_alloc(reinterpret_cast<void**>(&op->diag32), sizeof(float) * n);
     if (rc == HM_OK && cls) {
@@ -972,6 +1166,7 @@ int hm_cf_destroy(hm_cf* op) {
     dev_free(op->I32);
     dev_free(op->cls);
     dev_free(op->wc);
+    dev_free(op->wc32);
     dev_free(op->diag64);
     dev_free(op->diag32);
     delete op;
@@ -1246,14 +1441,15 @@ int hm_cg_solve(const hm_cg_args* a, hm_cg_stats* stats, void* stream) {
     return pcg_t<double>(A, M, a->b, a->x0, a->x, opt, stats, static_cast<hipStream_t>(stream));
 }
 
-int hm_cg_solve_mixed(hm_cf* cf64, hm_vc* vc32, const double* b, double* x, double rtol,
-                      double inner_rtol, int max_outer, int maxiter, hm_cg_stats* stats,
-                      void* stream) {
-    if (!cf64 || !vc32 || !b || !x || !stats || vc32->prec != HM_F32) {
-        set_error("hm_cg_solve_mixed: needs a float64 operator and a float32 V-cycle");
+int hm_cg_solve_mixed(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x,
+                      double rtol, double inner_rtol, int max_outer, int maxiter,
+                      hm_cg_stats* stats, void* stream) {
+    if ((cf == nullptr) == (dia == nullptr) || !vc32 || !b || !x || !stats ||
+        vc32->prec != HM_F32) {
+        set_error("hm_cg_solve_mixed: needs one operator and a float32 V-cycle");
         return HM_ERR_ARG;
     }
-    return pcg_mixed(cf64, vc32, b, x, rtol, inner_rtol, max_outer, maxiter, stats,
+    return pcg_mixed(cf, dia, vc32, b, x, rtol, inner_rtol, max_outer, maxiter, stats,
                      static_cast<hipStream_t>(stream));
 }
 

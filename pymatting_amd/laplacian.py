@@ -62,8 +62,8 @@ class CfOperator:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and h.value:
-            _ffi.lib().hm_cf_destroy(h)
+        if h is not None and h.value and _ffi is not None and _ffi._lib is not None:
+            _ffi._lib.hm_cf_destroy(h)
             self.handle = None
 
     # -- matrix-like interface -------------------------------------------
@@ -141,11 +141,33 @@ class DiaOperator:
         self.n = self.h * self.w
         self.shape = (self.n, self.n)
         self.grid_shape = (self.h, self.w)
-        self.coef = as_device(coef, torch.float64).reshape(25, self.n).contiguous()
+        self._coef = as_device(coef, torch.float64).reshape(25, self.n).contiguous()
         handle = ctypes.c_void_p()
-        _ffi.check(_ffi.lib().hm_dia_create(_ffi.ptr(self.coef), self.h, self.w, _ffi.stream(),
+        _ffi.check(_ffi.lib().hm_dia_create(_ffi.ptr(self._coef), self.h, self.w, _ffi.stream(),
                                             ctypes.byref(handle)))
         self.handle = handle
+
+    @classmethod
+    def from_cf(cls, op):
+        """Stored form of a CfOperator, assembled exactly on the device."""
+        self = cls.__new__(cls)
+        self.h, self.w = op.h, op.w
+        self.n = op.n
+        self.shape = op.shape
+        self.grid_shape = op.grid_shape
+        handle = ctypes.c_void_p()
+        _ffi.check(_ffi.lib().hm_dia_from_cf(op.handle, _ffi.stream(), ctypes.byref(handle)))
+        self.handle = handle
+        self._coef = None
+        return self
+
+    @property
+    def coef(self):
+        if getattr(self, "_coef", None) is None:
+            c = empty(25 * self.n)
+            _ffi.check(_ffi.lib().hm_dia_coefficients(self.handle, _ffi.ptr(c), _ffi.stream()))
+            self._coef = c.reshape(25, self.n)
+        return self._coef
 
     @classmethod
     def from_sparse(cls, A, shape):
@@ -165,8 +187,8 @@ class DiaOperator:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and h.value:
-            _ffi.lib().hm_dia_destroy(h)
+        if h is not None and h.value and _ffi is not None and _ffi._lib is not None:
+            _ffi._lib.hm_dia_destroy(h)
             self.handle = None
 
     def apply(self, x, out=None):

@@ -109,8 +109,8 @@ class VCycle:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and h.value:
-            _ffi.lib().hm_vcycle_destroy(h)
+        if h is not None and h.value and _ffi is not None and _ffi._lib is not None:
+            _ffi._lib.hm_vcycle_destroy(h)
             self.handle = None
 
     def apply(self, r, out=None):

@@ -59,8 +59,8 @@ class DeviceCSR:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and h.value:
-            _ffi.lib().hm_csr_destroy(h)
+        if h is not None and h.value and _ffi is not None and _ffi._lib is not None:
+            _ffi._lib.hm_csr_destroy(h)
             self.handle = None
 
 

@@ -40,15 +40,21 @@ def test_estimate_alpha_cf_errors():
 
 
 def test_lambda_solve_config1_fp64_and_mixed():
+    """320x240 workload. The float64 GPU solve follows the oracle's iterates
+    (same rtol 1e-7, alpha within 1e-6). An rtol-1e-7 solution of this system
+    is only determined to ~1.5e-3 in alpha (oracle rtol 1e-7 vs 1e-12), so the
+    mixed form is checked against a tightly converged oracle solve, run to a
+    residual at which alpha is determined well below the 1e-4 bound."""
     import pymatting_amd as pm
     from pymatting_amd import synthetic
     image, trimap, _ = synthetic.config1()
     ref_alpha, ref_x = ocf.solve_alpha_lambda(image, trimap)
     alpha, st = pm.solve_alpha_lambda(image, trimap, precision="f64")
     alpha = alpha.cpu().numpy()
-    print("fp64 iterations", st, "max|d|", np.abs(alpha - ref_alpha).max(), flush=True)
+    print("fp64", st, "max|d|", np.abs(alpha - ref_alpha).max(), flush=True)
     assert np.abs(alpha - ref_alpha).max() <= 1e-6
-    alpha32, st32 = pm.solve_alpha_lambda(image, trimap, precision="mixed")
+    tight, _ = ocf.solve_alpha_lambda(image, trimap, rtol=1e-11)
+    alpha32, st32 = pm.solve_alpha_lambda(image, trimap, precision="mixed", rtol=1e-9)
     alpha32 = alpha32.cpu().numpy()
-    print("mixed", st32, "max|d|", np.abs(alpha32 - ref_alpha).max(), flush=True)
-    assert np.abs(alpha32 - ref_alpha).max() <= 1e-4
+    print("mixed", st32, "max|d| vs tight", np.abs(alpha32 - tight).max(), flush=True)
+    assert np.abs(alpha32 - tight).max() <= 1e-4
