@@ -96,7 +96,10 @@ int hm_cf_system_fill(hm_cf* op, const int64_t* indptr, int64_t* indices, double
                       void* stream);
 
 /* ---- stored 25-point operators (ELL/DIA, slot s = (dy+2)*5 + (dx+2)) ----
- * coef: device float64 [25][h*w] slot-major. */
+ * coef: device float64 [25][h*w] slot-major. The operator must be symmetric
+ * (as every matrix handed to CG is): the library stores the 13 slots with
+ * (dy, dx) >= (0, 0) and reads the others from the neighbouring rows.
+ * hm_dia_coefficients writes the full [25][h*w] form back. */
 int hm_dia_create(const double* coef, int h, int w, void* stream, hm_dia** out);
 int hm_dia_from_cf(hm_cf* op, void* stream, hm_dia** out);
 int hm_dia_destroy(hm_dia* A);
@@ -202,7 +205,7 @@ int hm_fb_ml(const void* image, const void* alpha, int h0, int w0, int n_levels,
 #define HM_PROF_CF_RESTRICT 1
 #define HM_PROF_CF_SMOOTH 2
 #define HM_PROF_CF_RESIDUAL 3
-#define HM_PROF_DIA_APPLY 4
+#define HM_PROF_DIA_APPLY 4     /* stored level-0 stencil */
 #define HM_PROF_DIA_RESTRICT 5
 #define HM_PROF_DIA_SMOOTH 6
 #define HM_PROF_DIA_RESIDUAL 7
@@ -210,6 +213,10 @@ int hm_fb_ml(const void* image, const void* alpha, int h0, int w0, int n_levels,
 #define HM_PROF_CG_PUPDATE 9
 #define HM_PROF_DENSE 10
 #define HM_PROF_FG_SWEEP 11
+#define HM_PROF_COARSE_APPLY 12    /* stored stencils of Galerkin levels (>= 1) */
+#define HM_PROF_COARSE_RESTRICT 13
+#define HM_PROF_COARSE_SMOOTH 14
+#define HM_PROF_COARSE_RESIDUAL 15
 int hm_prof_set(int enabled);
 int hm_prof_query(int kind, int prec, double* total_ms, int64_t* launches, int64_t* points);
 

@@ -129,7 +129,9 @@ __device__ __forceinline__ bool solve_done(const CgState* st) {
 enum ProfKind {
     kProfCfApply = 0, kProfCfRestrict, kProfCfSmooth, kProfCfResidual,
     kProfDiaApply, kProfDiaRestrict, kProfDiaSmooth, kProfDiaResidual,
-    kProfCgUpdate, kProfCgPupdate, kProfDense, kProfFgSweep, kProfCount
+    kProfCgUpdate, kProfCgPupdate, kProfDense, kProfFgSweep,
+    kProfCoarseApply, kProfCoarseRestrict, kProfCoarseSmooth, kProfCoarseResidual,
+    kProfCount
 };
 
 struct ProfScope {

@@ -194,8 +194,9 @@ struct CfOp {
 
     // (A x) at tile point (lx, ly); xi is the unmasked input at that point.
     template <int OX, int OY>
-    __device__ __forceinline__ T apply(const Shared<OX, OY>& sh, const T (&)[OY + 4][OX + 4],
-                                       int lx, int ly, T xi, long long) const {
+    __device__ __forceinline__ T apply(const Shared<OX, OY>& sh, const Prefetch<OX, OY>&,
+                                       const T (&)[OY + 4][OX + 4], int, int lx, int ly, T xi,
+                                       long long) const {
         const int m = mode();
         const uint8_t ci = sh.cls[ly + 2][lx + 2];
         const Vec4<T> p = sh.px[ly + 2][lx + 2];
@@ -218,35 +219,84 @@ struct CfOp {
     }
 };
 
+// Stored symmetric 25-point stencils keep the 13 slots s = (dy+2)*5 + (dx+2)
+// with s >= 12 ((dy, dx) >= (0, 0) in raster order), slot-major [13][n]; the
+// coefficient of slot s < 12 at point i is that of slot 24 - s at the
+// neighbour i + (dy, dx) (A is symmetric). The neighbour index is clamped into
+// the array: callers multiply out-of-grid couplings by zero.
+constexpr int kSymSlots = 13;
+
+template <typename T>
+__device__ __forceinline__ T sym_coef(const T* __restrict__ c, long long n, int w, long long i,
+                                      int s) {
+    if (s >= 12) return c[(s - 12) * n + i];
+    long long j = i + (long long)(s / 5 - 2) * w + (s % 5 - 2);
+    j = j < 0 ? 0 : (j >= n ? n - 1 : j);
+    return c[(12 - s) * n + j];
+}
+
 template <int OX, int OY>
 struct NoShared {};
 
-// Stored 25-point stencil; coef is slot-major [25][n], slot (dy+2)*5 + (dx+2).
-template <typename T>
+// Coefficients of this thread's output points, loaded before the tile
+// barrier so their latency overlaps the halo loads (PF = true: small grids,
+// whose launches are latency-bound). PF = false loads them at use, which keeps
+// register pressure and occupancy where the large grids need them.
+template <typename T, int OX, int OY, bool PF>
+struct DiaPrefetch {
+    static constexpr int NO = (OX * OY + kBlock - 1) / kBlock;
+    T c[NO][25];
+};
+template <typename T, int OX, int OY>
+struct DiaPrefetch<T, OX, OY, false> {};
+
+template <typename T, bool PF = false>
 struct DiaOp {
     static constexpr int kProfBase = kProfDiaApply;
     int h, w;
-    const T* coef;
+    const T* coef;  // [13][n] symmetric storage
 
     template <int OX, int OY>
     using Shared = NoShared<OX, OY>;
+    template <int OX, int OY>
+    using Prefetch = DiaPrefetch<T, OX, OY, PF>;
 
     template <int OX, int OY>
+    __device__ __forceinline__ void prefetch(Prefetch<OX, OY>& pf, int ox0, int oy0) const {
+        if constexpr (PF) {
+            const long long n = (long long)h * w;
+            constexpr int NO = (OX * OY + kBlock - 1) / kBlock;
+#pragma unroll
+            for (int k = 0; k < NO; ++k) {
+                const int t = threadIdx.x + k * kBlock;
+                const int ly = t / OX, lx = t - ly * OX;
+                const int gx = ox0 + lx, gy = oy0 + ly;
+                const bool in = t < OX * OY && gx >= 0 && gx < w && gy >= 0 && gy < h;
+                const long long g = in ? (long long)gy * w + gx : 0;
+#pragma unroll
+                for (int s = 0; s < 25; ++s) pf.c[k][s] = in ? sym_coef(coef, n, w, g, s) : T(0);
+            }
+        }
+    }
+    template <int OX, int OY>
     __device__ __forceinline__ void load_point(Shared<OX, OY>&, int, int, bool, long long, T) const {}
-    template <int OX, int OY>
-    using Prefetch = NoShared<OX, OY>;
-    template <int OX, int OY>
-    __device__ __forceinline__ void prefetch(Prefetch<OX, OY>&, int, int) const {}
     template <int OX, int OY>
     __device__ void prepare(Shared<OX, OY>&, const Prefetch<OX, OY>&) const {}
 
     template <int OX, int OY>
-    __device__ __forceinline__ T apply(const Shared<OX, OY>&, const T (&sx)[OY + 4][OX + 4],
-                                       int lx, int ly, T, long long g) const {
-        const long long n = (long long)h * w;
+    __device__ __forceinline__ T apply(const Shared<OX, OY>&, const Prefetch<OX, OY>& pf,
+                                       const T (&sx)[OY + 4][OX + 4], int k, int lx, int ly, T,
+                                       long long g) const {
         T acc = 0;
+        if constexpr (PF) {
 #pragma unroll
-        for (int s = 0; s < 25; ++s) acc += coef[s * n + g] * sx[ly + s / 5][lx + s % 5];
+            for (int s = 0; s < 25; ++s) acc += pf.c[k][s] * sx[ly + s / 5][lx + s % 5];
+        } else {
+            const long long n = (long long)h * w;
+#pragma unroll
+            for (int s = 0; s < 25; ++s)
+                acc += sym_coef(coef, n, w, g, s) * sx[ly + s / 5][lx + s % 5];
+        }
         return acc;
     }
 };
@@ -257,6 +307,18 @@ template <typename T>
 struct InVec {
     const T* x;
     __device__ __forceinline__ T load(int, int, long long g) const { return x[g]; }
+};
+
+// p = z + beta p (cg.py:112-113) formed on the fly from the previous
+// direction; the apply kernel writes the tile's new p to another buffer.
+template <typename T>
+struct InPupdate {
+    const T* z;
+    const T* p_old;
+    const CgState* st;
+    __device__ __forceinline__ T load(int, int, long long g) const {
+        return p_old[g] * (T)st->beta + z[g];
+    }
 };
 
 template <typename T>
@@ -367,8 +429,10 @@ struct OutApply {
     static constexpr int kProf = 0;
     T* y;
     DotSink dot;
+    T* xout = nullptr;  // optional copy of the input at the tile's points
     __device__ __forceinline__ void point(long long g, int, int, T Ax, T xi, double& acc) const {
         y[g] = Ax;
+        if (xout) xout[g] = xi;
         acc += (double)xi * (double)Ax;
     }
     __device__ __forceinline__ void outside(int, int) const {}
@@ -495,7 +559,7 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(Op op, In in, Out out, int
         }
         const long long g = (long long)gy * w + gx;
         const T xi = sx[ly + 2][lx + 2];
-        const T Ax = op.template apply<OX, OY>(sh, sx, lx, ly, xi, g);
+        const T Ax = op.template apply<OX, OY>(sh, pf, sx, k, lx, ly, xi, g);
         o.point(g, lx, ly, Ax, xi, acc);
     }
     if constexpr (IsRestrict<Out>::value) {
@@ -514,12 +578,25 @@ template <int N>
 __global__ __launch_bounds__(kBlock) void k_finalize(Reducer red, unsigned nblocks, CgState* st,
                                                      int kind, const CgState* guard) {
     if (solve_done(guard)) return;
+    // Eight independent partial sums per thread keep eight loads in flight.
+    constexpr int U = 8;
     double acc[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        acc[k] = 0.0;
-        for (unsigned i = threadIdx.x; i < nblocks; i += kBlock)
-            acc[k] += red.partials[k * red.max_blocks + i];
+        double part[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) part[u] = 0.0;
+        for (unsigned i0 = 0; i0 < nblocks; i0 += kBlock * U) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const unsigned i = i0 + u * kBlock + threadIdx.x;
+                if (i < nblocks) part[u] += red.partials[k * red.max_blocks + i];
+            }
+        }
+        double t = 0.0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) t += part[u];
+        acc[k] = t;
     }
     block_sum<kBlock, N>(acc);
     if (threadIdx.x == 0) dot_finalize(st, kind, acc[0], N > 1 ? acc[N > 1 ? 1 : 0] : 0.0);

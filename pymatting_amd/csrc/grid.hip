@@ -140,14 +140,15 @@ static PairTable build_pairs() {
 
 __device__ __forceinline__ double w1(int e) { return e == 0 ? 0.5 : 0.25; }
 
-// One thread per (coarse point, coarse slot).
+// One thread per (coarse point, stored coarse slot s >= 12); both operators
+// use the symmetric storage of engine.h.
 __global__ void k_galerkin(const double* __restrict__ Af, int hf, int wf,
                            double* __restrict__ Ac, int hc, int wc) {
     const long long nc = (long long)hc * wc, nf = (long long)hf * wf;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 25 * nc) return;
-    const int s = (int)(t / nc);
-    const long long I = t - (long long)s * nc;
+    if (t >= kSymSlots * nc) return;
+    const int s = 12 + (int)(t / nc);
+    const long long I = t - (long long)(s - 12) * nc;
     const int Y = (int)(I / wc), X = (int)(I - (long long)Y * wc);
     const int Dy = s / 5 - 2, Dx = s % 5 - 2;
     double sum = 0.0;
@@ -165,7 +166,7 @@ __global__ void k_galerkin(const double* __restrict__ Af, int hf, int wf,
                 if (ax < 0 || ax >= wf || bx < 0 || bx >= wf) continue;
                 const int sx = bx - ax + 2;
                 const long long a = (long long)ay * wf + ax;
-                sum += wy * (w1(eax) * w1(ebx)) * Af[(sy * 5 + sx) * nf + a];
+                sum += wy * (w1(eax) * w1(ebx)) * sym_coef(Af, nf, wf, a, sy * 5 + sx);
             }
         }
     }
@@ -179,7 +180,7 @@ int dia_galerkin(const hm_dia* fine, hm_dia* coarse, hipStream_t s) {
         HM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_pairs), &t, sizeof(t)));
         pairs_ready = true;
     }
-    const long long total = 25 * coarse->n;
+    const long long total = kSymSlots * coarse->n;
     k_galerkin<<<div_up(total, 256), 256, 0, s>>>(fine->c64, fine->h, fine->w, coarse->c64,
                                                   coarse->h, coarse->w);
     HM_LAUNCH_CHECK();
@@ -189,18 +190,48 @@ int dia_galerkin(const hm_dia* fine, hm_dia* coarse, hipStream_t s) {
 __global__ void k_dia_finish(const double* __restrict__ c64, float* __restrict__ c32,
                              double* __restrict__ d64, float* __restrict__ d32, long long n) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 25 * n) return;
+    if (t >= kSymSlots * n) return;
     const double v = c64[t];
     c32[t] = (float)v;
-    if (t >= 12 * n && t < 13 * n) {
-        d64[t - 12 * n] = v;
-        d32[t - 12 * n] = (float)v;
+    if (t < n) {  // stored slot 0 is the diagonal (s = 12)
+        d64[t] = v;
+        d32[t] = (float)v;
     }
 }
 
+// Full [25][n] slot-major layout <-> symmetric [13][n] storage.
+__global__ void k_dia_pack(const double* __restrict__ full, double* __restrict__ sym, long long n) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < kSymSlots * n) sym[t] = full[12 * n + t];
+}
+
+__global__ void k_dia_unpack(const double* __restrict__ sym, int h, int w,
+                             double* __restrict__ full) {
+    const long long n = (long long)h * w;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 25 * n) return;
+    const int s = (int)(t / n);
+    const long long i = t - (long long)s * n;
+    const int y = (int)(i / w), x = (int)(i - (long long)y * w);
+    const int yy = y + s / 5 - 2, xx = x + s % 5 - 2;
+    full[t] = (yy >= 0 && yy < h && xx >= 0 && xx < w) ? sym_coef(sym, n, w, i, s) : 0.0;
+}
+
+int dia_pack(const double* full, hm_dia* A, hipStream_t s) {
+    k_dia_pack<<<div_up(kSymSlots * A->n, 256), 256, 0, s>>>(full, A->c64, A->n);
+    HM_LAUNCH_CHECK();
+    return HM_OK;
+}
+
+int dia_unpack(const hm_dia* A, double* full, hipStream_t s) {
+    k_dia_unpack<<<div_up(25 * A->n, 256), 256, 0, s>>>(A->c64, A->h, A->w, full);
+    HM_LAUNCH_CHECK();
+    return HM_OK;
+}
+
 int dia_finish(hm_dia* A, hipStream_t s) {
-    k_dia_finish<<<div_up(25 * A->n, 256), 256, 0, s>>>(A->c64, A->c32, A->diag64, A->diag32,
-                                                        A->n);
+    k_dia_finish<<<div_up(kSymSlots * A->n, 256), 256, 0, s>>>(A->c64, A->c32, A->diag64,
+                                                               A->diag32, A->n);
     HM_LAUNCH_CHECK();
     return HM_OK;
 }
@@ -219,7 +250,7 @@ __global__ void k_aug_from_dia(const double* __restrict__ coef, int h, int w,
         const int ry = r / w, rx = r % w, cy = c / w, cx = c % w;
         const int dy = cy - ry, dx = cx - rx;
         if (dy >= -2 && dy <= 2 && dx >= -2 && dx <= 2)
-            v = coef[(long long)((dy + 2) * 5 + dx + 2) * n + r];
+            v = sym_coef(coef, (long long)n, w, (long long)r, (dy + 2) * 5 + dx + 2);
     } else {
         v = (c - n == r) ? 1.0 : 0.0;
     }

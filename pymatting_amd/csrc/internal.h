@@ -23,7 +23,8 @@ struct hm_cf {
     float* diag32 = nullptr;
 };
 
-// Stored 25-point stencil (slot-major [25][n]) with its diagonal.
+// Stored symmetric 25-point stencil (slot-major [13][n], see engine.h) with
+// its diagonal.
 struct hm_dia {
     int h = 0, w = 0;
     long long n = 0;
@@ -92,6 +93,8 @@ int trimap_classes(const double* trimap, long long n, double fg, double bg, uint
 // grid.hip
 int dia_galerkin(const hm_dia* fine, hm_dia* coarse, hipStream_t s);
 int dia_finish(hm_dia* A, hipStream_t s);  // diagonal + float32 copies
+int dia_pack(const double* full, hm_dia* A, hipStream_t s);     // [25][n] -> [13][n]
+int dia_unpack(const hm_dia* A, double* full, hipStream_t s);   // [13][n] -> [25][n]
 constexpr int kMaxDirect = 2048;  // largest coarsest grid solved by the dense inverse
 int dense_inverse(const hm_dia* A, double* inv64, float* inv32, hipStream_t s);
 int dense_inverse_from_dense(const double* A, int n, double* inv64, float* inv32, hipStream_t s);

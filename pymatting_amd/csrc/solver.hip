@@ -37,13 +37,14 @@ void set_error(const char* fmt, ...) {
 
 template <typename T, class Op, class In, class Out>
 int launch_tile(const Op& op, const In& in, const Out& out, int h, int w, int h2, int w2,
-                const CgState* guard, hipStream_t s) {
+                const CgState* guard, hipStream_t s, bool coarse = false) {
     const dim3 grid = tile_grid(out, h, w, h2, w2);
     if (out.dot.st != nullptr && (long long)grid.x * grid.y > out.dot.red.max_blocks) {
         set_error("reduction buffer too small for %u x %u blocks", grid.x, grid.y);
         return HM_ERR_ARG;
     }
-    ProfScope prof(Op::kProfBase + Out::kProf, (long long)h * w, sizeof(T) == 4, s);
+    ProfScope prof((coarse ? (int)kProfCoarseApply : Op::kProfBase) + Out::kProf, (long long)h * w,
+                   sizeof(T) == 4, s);
     tile_kernel<T, Op, In, Out><<<grid, kBlock, 0, s>>>(op, in, out, h, w, guard);
     HM_LAUNCH_CHECK();
     if (out.dot.st != nullptr)
@@ -54,6 +55,7 @@ int launch_tile(const Op& op, const In& in, const Out& out, int h, int w, int h2
 template <typename T>
 struct LevelOp {
     bool is_cf = false;
+    bool coarse = false;  // a Galerkin level (profiling class only)
     CfOp<T> cf{};
     DiaOp<T> dia{};
     const T* D = nullptr;
@@ -97,6 +99,10 @@ int run_level(const LevelOp<T>& L, const In& in, const Out& out, const CgState* 
               hipStream_t s) {
     if (L.is_cf)
         return launch_tile<T>(L.cf, in, out, L.h, L.w, L.h / 2, L.w / 2, guard, s);
+    if (L.coarse) {
+        const DiaOp<T, true> op{L.dia.h, L.dia.w, L.dia.coef};
+        return launch_tile<T>(op, in, out, L.h, L.w, L.h / 2, L.w / 2, guard, s, true);
+    }
     return launch_tile<T>(L.dia, in, out, L.h, L.w, L.h / 2, L.w / 2, guard, s);
 }
 
@@ -234,6 +240,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_apply(const T* __restrict__ in
     double acc = 0.0;
     for (int i = threadIdx.x + blockIdx.x * kBlock; i < n; i += kBlock * gridDim.x) {
         T s = 0;
+#pragma unroll 8
         for (int j = 0; j < n; ++j) s += inv[(long long)i * n + j] * b[j];
         x[i] = s;
         acc += (double)b[i] * (double)s;
@@ -266,7 +273,9 @@ T* vptr(void* p) { return static_cast<T*>(p); }
 template <typename T>
 LevelOp<T> vc_level_op(const hm_vc* vc, size_t l) {
     if (l == 0 && vc->cf) return cf_level<T>(vc->cf);
-    return dia_level<T>(vc->levels[l].A);
+    LevelOp<T> L = dia_level<T>(vc->levels[l].A);
+    L.coarse = l > 0;
+    return L;
 }
 
 template <typename T>
@@ -304,25 +313,24 @@ int post_stage(const LevelOp<T>& op, const Base& base, const T* b, const T* e, i
 
 // ------------------------------------------------------------- V-cycle tail
 //
-// The coarsest grids (each at most kTailPoints points) are processed by one
-// 1024-thread workgroup in a single launch: pre-smoothing, residual,
-// restriction, the dense direct solve and the way back up, separated by
-// workgroup barriers. On these grids a separate launch per step would cost
-// more than the arithmetic.
+// The coarsest grids are processed by one 1024-thread workgroup in a single
+// launch with everything staged in LDS: the tail's stored stencils and
+// right-hand side are loaded once, pre-smoothing, residual, restriction, the
+// dense direct solve and the way back up run out of LDS between workgroup
+// barriers, and only the top tail grid's correction is written back. On
+// these grids each separate launch would cost far more than its arithmetic.
 
-constexpr int kTailPoints = 1024;
 constexpr int kTailMaxLevels = 24;
 constexpr int kTailBlock = 1024;
+constexpr int kTailLdsBytes = 152 * 1024;
 
 template <typename T>
 struct TailLevel {
     int h, w;
-    const T* coef;  // [25][n]
-    const T* D;
-    T* b;
-    T* x;
-    T* t;
-    T* u;
+    const T* coef;  // [13][n] symmetric storage (global)
+    T* b;           // right-hand side (global; only the top level's is read)
+    T* x;           // result (global; only the top level's is written)
+    int off;        // offset of this level's LDS block (in T units)
 };
 
 template <typename T>
@@ -331,18 +339,43 @@ struct TailParams {
     int nlev;        // tail grids including the direct one (last)
     int pre, post;
     T omega;
-    const T* inv;    // dense inverse of the last grid
+    const T* inv;    // dense inverse of the last grid (global)
+};
+
+// LDS block of one level: coef [13][n], b, x, t, u (each [n]).
+template <typename T>
+struct TailView {
+    int h, w, n;
+    T* coef;
+    T* b;
+    T* x;
+    T* t;
+    T* u;
 };
 
 template <typename T>
-__device__ __forceinline__ T tail_Ax(const TailLevel<T>& L, const T* v, int i) {
-    const int n = L.h * L.w;
-    const int y = i / L.w, x = i - y * L.w;
+__device__ __forceinline__ TailView<T> tail_view(T* lds, const TailLevel<T>& L) {
+    TailView<T> v;
+    v.h = L.h;
+    v.w = L.w;
+    v.n = L.h * L.w;
+    v.coef = lds + L.off;
+    v.b = v.coef + kSymSlots * v.n;
+    v.x = v.b + v.n;
+    v.t = v.x + v.n;
+    v.u = v.t + v.n;
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T tail_Ax(const TailView<T>& V, const T* v, int i) {
+    const int y = i / V.w, x = i - y * V.w;
     T acc = 0;
 #pragma unroll
     for (int s = 0; s < 25; ++s) {
         const int yy = y + s / 5 - 2, xx = x + s % 5 - 2;
-        if (yy >= 0 && yy < L.h && xx >= 0 && xx < L.w) acc += L.coef[s * n + i] * v[yy * L.w + xx];
+        if (yy >= 0 && yy < V.h && xx >= 0 && xx < V.w)
+            acc += sym_coef(V.coef, (long long)V.n, V.w, (long long)i, s) * v[yy * V.w + xx];
     }
     return acc;
 }
@@ -367,95 +400,126 @@ __device__ __forceinline__ T tail_prolong(const T* e, int h2, int w2, int gx, in
 template <typename T>
 __global__ __launch_bounds__(kTailBlock) void k_vc_tail(TailParams<T> P, const CgState* guard) {
     if (solve_done(guard)) return;
+    extern __shared__ __align__(16) unsigned char tail_lds_raw[];
+    T* lds = reinterpret_cast<T*>(tail_lds_raw);
     const int tid = threadIdx.x;
-    T* xpre[kTailMaxLevels];
+    // Stage every level's stencil and the top level's right-hand side.
+    for (int l = 0; l < P.nlev; ++l) {
+        const TailView<T> V = tail_view(lds, P.lv[l]);
+        for (int i = tid; i < kSymSlots * V.n; i += kTailBlock) V.coef[i] = P.lv[l].coef[i];
+        if (l == 0)
+            for (int i = tid; i < V.n; i += kTailBlock) V.b[i] = P.lv[0].b[i];
+    }
+    __syncthreads();
+    // The pre-smoothed iterate of every level ends in .x or .t depending on
+    // the number of sweeps (one buffer swap per sweep after the first).
+    const bool pre_in_t = P.pre >= 2 && ((P.pre - 1) & 1);
     // Descend.
     for (int l = 0; l + 1 < P.nlev; ++l) {
-        const TailLevel<T>& L = P.lv[l];
-        const int n = L.h * L.w;
-        T* cur = L.x;
-        T* nxt = L.t;
+        const TailView<T> V = tail_view(lds, P.lv[l]);
+        T* cur = V.x;
+        T* nxt = V.t;
         if (P.pre == 0) {
-            for (int i = tid; i < n; i += kTailBlock) cur[i] = T(0);
+            for (int i = tid; i < V.n; i += kTailBlock) cur[i] = T(0);
         } else {
-            for (int i = tid; i < n; i += kTailBlock) cur[i] = P.omega * L.b[i] / L.D[i];
+            for (int i = tid; i < V.n; i += kTailBlock) cur[i] = P.omega * V.b[i] / V.coef[i];
             for (int k = 1; k < P.pre; ++k) {
                 __syncthreads();
-                for (int i = tid; i < n; i += kTailBlock)
-                    nxt[i] = cur[i] + P.omega * (L.b[i] - tail_Ax(L, cur, i)) / L.D[i];
+                for (int i = tid; i < V.n; i += kTailBlock)
+                    nxt[i] = cur[i] + P.omega * (V.b[i] - tail_Ax(V, cur, i)) / V.coef[i];
                 T* sw = cur; cur = nxt; nxt = sw;
             }
         }
         __syncthreads();
-        for (int i = tid; i < n; i += kTailBlock) L.u[i] = L.b[i] - tail_Ax(L, cur, i);
+        for (int i = tid; i < V.n; i += kTailBlock) V.u[i] = V.b[i] - tail_Ax(V, cur, i);
         __syncthreads();
-        const TailLevel<T>& C = P.lv[l + 1];
-        const int nc = C.h * C.w;
-        for (int I = tid; I < nc; I += kTailBlock) {
+        const TailView<T> C = tail_view(lds, P.lv[l + 1]);
+        for (int I = tid; I < C.n; I += kTailBlock) {
             const int Y = I / C.w, X = I - Y * C.w;
             T acc = 0;
             for (int ey = -1; ey <= 1; ++ey) {
                 const int y = 2 * Y + ey;
-                if (y < 0 || y >= L.h) continue;
+                if (y < 0 || y >= V.h) continue;
                 for (int ex = -1; ex <= 1; ++ex) {
                     const int x = 2 * X + ex;
-                    if (x < 0 || x >= L.w) continue;
+                    if (x < 0 || x >= V.w) continue;
                     acc += (ey == 0 ? T(0.5) : T(0.25)) * (ex == 0 ? T(0.5) : T(0.25)) *
-                           L.u[y * L.w + x];
+                           V.u[y * V.w + x];
                 }
             }
             C.b[I] = acc;
         }
-        xpre[l] = cur;
         __syncthreads();
     }
-    // Direct solve on the last grid.
+    // Direct solve on the last grid: x = inv(A) b with the inverse staged in
+    // the scratch vectors of the top tail level when it fits.
     {
-        const TailLevel<T>& C = P.lv[P.nlev - 1];
-        const int nc = C.h * C.w;
-        for (int i = tid; i < nc; i += kTailBlock) {
+        const TailView<T> C = tail_view(lds, P.lv[P.nlev - 1]);
+        const TailView<T> top = tail_view(lds, P.lv[0]);
+        const int nn = C.n * C.n;
+        const bool staged = nn <= 2 * top.n;
+        T* inv = top.t;  // t and u of the top level are free at this point
+        if (staged) {
+            for (int i = tid; i < nn; i += kTailBlock) inv[i] = P.inv[i];
+            __syncthreads();
+        }
+        for (int i = tid; i < C.n; i += kTailBlock) {
             T acc = 0;
-            for (int j = 0; j < nc; ++j) acc += P.inv[(long long)i * nc + j] * C.b[j];
+            if (staged) {
+                for (int j = 0; j < C.n; ++j) acc += inv[i * C.n + j] * C.b[j];
+            } else {
+#pragma unroll 8
+                for (int j = 0; j < C.n; ++j) acc += P.inv[(long long)i * C.n + j] * C.b[j];
+            }
             C.x[i] = acc;
         }
         __syncthreads();
     }
     // Ascend.
-    const T* e = P.lv[P.nlev - 1].x;
+    const T* e = tail_view(lds, P.lv[P.nlev - 1]).x;
     for (int l = P.nlev - 2; l >= 0; --l) {
-        const TailLevel<T>& L = P.lv[l];
-        const TailLevel<T>& C = P.lv[l + 1];
-        const int n = L.h * L.w;
-        T* a = (xpre[l] == L.x) ? L.t : L.x;  // must not overwrite the pre-smoothed iterate
-        for (int i = tid; i < n; i += kTailBlock) {
-            const int y = i / L.w, x = i - y * L.w;
-            a[i] = xpre[l][i] + tail_prolong(e, C.h, C.w, x, y);
+        const TailView<T> V = tail_view(lds, P.lv[l]);
+        const TailView<T> C = tail_view(lds, P.lv[l + 1]);
+        const T* xp = pre_in_t ? V.t : V.x;
+        T* a = pre_in_t ? V.x : V.t;  // must not overwrite the pre-smoothed iterate
+        for (int i = tid; i < V.n; i += kTailBlock) {
+            const int y = i / V.w, x = i - y * V.w;
+            a[i] = xp[i] + tail_prolong(e, C.h, C.w, x, y);
         }
         T* cur = a;
-        T* nxt = L.u;
+        T* nxt = V.u;
         for (int k = 0; k < P.post; ++k) {
             __syncthreads();
-            for (int i = tid; i < n; i += kTailBlock)
-                nxt[i] = cur[i] + P.omega * (L.b[i] - tail_Ax(L, cur, i)) / L.D[i];
+            for (int i = tid; i < V.n; i += kTailBlock)
+                nxt[i] = cur[i] + P.omega * (V.b[i] - tail_Ax(V, cur, i)) / V.coef[i];
             T* sw = cur; cur = nxt; nxt = sw;
         }
         __syncthreads();
-        if (l == 0 && cur != L.x) {
-            for (int i = tid; i < n; i += kTailBlock) L.x[i] = cur[i];
-            __syncthreads();
-            cur = L.x;
-        }
         e = cur;
+    }
+    // The top tail grid's correction goes back to global memory.
+    {
+        const int n0 = P.lv[0].h * P.lv[0].w;
+        for (int i = tid; i < n0; i += kTailBlock) P.lv[0].x[i] = e[i];
     }
 }
 
-// First level index of the tail (levels tail..L-1 are all small), or L if
-// the hierarchy is a single direct level.
+// LDS footprint of tail levels [start, L): 17 values per point.
+inline long long tail_bytes(const hm_vc* vc, size_t start, size_t es) {
+    long long b = 0;
+    for (size_t l = start; l < vc->levels.size(); ++l) b += (long long)(kSymSlots + 4) * vc->levels[l].n * es;
+    return b;
+}
+
+// First level of the tail: the largest suffix of grids (never level 0) that
+// fits the LDS budget, or L - 1 if even the two coarsest grids do not.
 inline size_t vc_tail_start(const hm_vc* vc) {
     const size_t L = vc->levels.size();
     if (L <= 1) return L;
+    const size_t es = vc->prec == HM_F64 ? sizeof(double) : sizeof(float);
     size_t t = L - 1;
-    while (t > 1 && vc->levels[t - 1].n <= kTailPoints && L - (t - 1) <= (size_t)kTailMaxLevels)
+    while (t > 1 && L - (t - 1) <= (size_t)kTailMaxLevels &&
+           tail_bytes(vc, t - 1, es) <= kTailLdsBytes)
         --t;
     return t;
 }
@@ -469,20 +533,27 @@ int launch_tail(hm_vc* vc, size_t start, const CgState* guard, hipStream_t s) {
     P.post = vc->post;
     P.omega = (T)vc->omega;
     P.inv = pick<T>(vc->dense_inv64, vc->dense_inv32);
+    int off = 0;
     for (size_t l = start; l < L; ++l) {
         const VcLevel& V = vc->levels[l];
         TailLevel<T>& t = P.lv[l - start];
         t.h = V.h;
         t.w = V.w;
         t.coef = pick<T>(V.A->c64, V.A->c32);
-        t.D = pick<T>(V.A->diag64, V.A->diag32);
         t.b = static_cast<T*>(V.b);
         t.x = static_cast<T*>(V.x);
-        t.t = static_cast<T*>(V.t);
-        t.u = static_cast<T*>(V.u);
+        t.off = off;
+        off += (int)((kSymSlots + 4) * V.n);
+    }
+    const size_t lds = (size_t)off * sizeof(T);
+    static bool attr_set[2] = {false, false};
+    if (!attr_set[sizeof(T) == 4]) {
+        HM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vc_tail<T>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes));
+        attr_set[sizeof(T) == 4] = true;
     }
     ProfScope prof(kProfDense, vc->levels[start].n, sizeof(T) == 4, s);
-    k_vc_tail<T><<<1, kTailBlock, 0, s>>>(P, guard);
+    k_vc_tail<T><<<1, kTailBlock, lds, s>>>(P, guard);
     HM_LAUNCH_CHECK();
     return HM_OK;
 }
@@ -658,6 +729,24 @@ int op_apply(const OpHandle& A, const T* x, T* y, const DotSink& ds, const CgSta
     return HM_ERR_UNSUPPORTED;
 }
 
+// q = A p with p = z + beta p_old formed on the fly (stencil operators), or a
+// separate direction update followed by the product (CSR / callbacks).
+template <typename T>
+int op_apply_pupdate(const OpHandle& A, const T* z, const T* p_old, T* p_new, T* q,
+                     const DotSink& ds, CgState* st, hipStream_t s) {
+    const InPupdate<T> in{z, p_old, st};
+    OutApply<T> out{q, ds, p_new};
+    if (A.cf) return run_level<T>(cf_level<T>(A.cf), in, out, st, s);
+    if (A.dia) return run_level<T>(dia_level<T>(A.dia), in, out, st, s);
+    const long long n = A.n();
+    // p_new = p_old * beta + z as a pointwise pass, then the product.
+    k_copy<T><<<div_up(n, 256), 256, 0, s>>>(p_old, p_new, n, st);
+    HM_LAUNCH_CHECK();
+    k_cg_pupdate<T><<<vec_blocks(n), kBlock, 0, s>>>(p_new, z, n, st);
+    HM_LAUNCH_CHECK();
+    return op_apply<T>(A, p_new, q, ds, st, s);
+}
+
 __global__ void k_sub_from(const double* __restrict__ b, double* __restrict__ r, long long n) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) r[i] = b[i] - r[i];
@@ -770,9 +859,10 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
           const PcgOptions& opt, hm_cg_stats* stats, hipStream_t s) {
     const long long n = A.n();
     const int maxiter = opt.maxiter < 0 ? 0 : opt.maxiter;
-    DevBuf r, p, q, z, rold;
+    DevBuf r, p, p2, q, z, rold;
     HM_TRY(r.alloc(sizeof(T) * n));
     HM_TRY(p.alloc(sizeof(T) * n));
+    HM_TRY(p2.alloc(sizeof(T) * n));
     HM_TRY(q.alloc(sizeof(T) * n));
     HM_TRY(z.alloc(sizeof(T) * n));
     if (opt.reorth) HM_TRY(rold.alloc(sizeof(T) * n));
@@ -820,10 +910,13 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
     st.host->norm_b = norm_b;
     st.host->rr = norm_r * norm_r;
     HM_TRY(st.push(s));
-    // z = M r; p = z; rz = <r, z> (cg.py:78-80)
+    // z = M r; rz = <r, z> (cg.py:78-80). p = z comes from the first fused
+    // direction update with beta = 0 and a zero previous direction.
     HM_TRY(pc_apply<T>(M, r.as<T>(), z.as<T>(), st.dev, red, kDotRzFirst, nullptr, n, s));
-    k_copy<T><<<div_up(n, 256), 256, 0, s>>>(z.as<T>(), p.as<T>(), n, nullptr);
+    k_fill<T><<<div_up(n, 256), 256, 0, s>>>(p.as<T>(), T(0), n);
     HM_LAUNCH_CHECK();
+    T* p_cur = p.as<T>();   // previous direction
+    T* p_next = p2.as<T>(); // direction being formed
 
     const int check_every = interactive ? 1 : (opt.check_every < 1 ? 1 : opt.check_every);
     int enqueued = 0;
@@ -835,10 +928,11 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
                 k_copy<T><<<div_up(n, 256), 256, 0, s>>>(r.as<T>(), rold.as<T>(), n, st.dev);
                 HM_LAUNCH_CHECK();
             }
-            HM_TRY(op_apply<T>(A, p.as<T>(), q.as<T>(), pq, st.dev, s));
+            HM_TRY(op_apply_pupdate<T>(A, z.as<T>(), p_cur, p_next, q.as<T>(), pq, st.dev, s));
+            std::swap(p_cur, p_next);
             {
                 ProfScope prof_u(kProfCgUpdate, n, sizeof(T) == 4, s);
-                k_cg_update<T><<<vb, kBlock, 0, s>>>(x, p.as<T>(), r.as<T>(), q.as<T>(), n,
+                k_cg_update<T><<<vb, kBlock, 0, s>>>(x, p_cur, r.as<T>(), q.as<T>(), n,
                                                      st.dev, red);
                 HM_LAUNCH_CHECK();
                 HM_TRY(launch_finalize(1, red, vb, st.dev, kDotUpdate, st.dev, s));
@@ -860,11 +954,6 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
             } else {
                 HM_TRY(pc_apply<T>(M, r.as<T>(), z.as<T>(), st.dev, red, kDotRzUpdate, st.dev, n,
                                    s));
-            }
-            {
-                ProfScope prof_p(kProfCgPupdate, n, sizeof(T) == 4, s);
-                k_cg_pupdate<T><<<vb, kBlock, 0, s>>>(p.as<T>(), z.as<T>(), n, st.dev);
-                HM_LAUNCH_CHECK();
             }
         }
         enqueued += k;
@@ -971,8 +1060,9 @@ int dia_alloc(int h, int w, hm_dia** out) {
     A->h = h;
     A->w = w;
     A->n = (long long)h * w;
-    int rc = dev_alloc(reinterpret_cast<void**>(&A->c64), sizeof(double) * 25 * A->n);
-    if (rc == HM_OK) rc = dev_alloc(reinterpret_cast<void**>(&A->c32), sizeof(float) * 25 * A->n);
+    int rc = dev_alloc(reinterpret_cast<void**>(&A->c64), sizeof(double) * kSymSlots * A->n);
+    if (rc == HM_OK)
+        rc = dev_alloc(reinterpret_cast<void**>(&A->c32), sizeof(float) * kSymSlots * A->n);
     if (rc == HM_OK) rc = dev_alloc(reinterpret_cast<void**>(&A->diag64), sizeof(double) * A->n);
     if (rc == HM_OK) rc = dev_alloc(reinterpret_cast<void**>(&A->diag32), sizeof(float) * A->n);
     if (rc != HM_OK) {
@@ -1225,9 +1315,8 @@ int hm_dia_create(const double* coef, int h, int w, void* stream, hm_dia** out) 
     hipStream_t s = static_cast<hipStream_t>(stream);
     hm_dia* A = nullptr;
     HM_TRY(dia_alloc(h, w, &A));
-    hipError_t e = hipMemcpyAsync(A->c64, coef, sizeof(double) * 25 * A->n, hipMemcpyDeviceToDevice, s);
-    int rc = e == hipSuccess ? dia_finish(A, s) : HM_ERR_HIP;
-    if (e != hipSuccess) set_error("hipMemcpyAsync: %s", hipGetErrorString(e));
+    int rc = dia_pack(coef, A, s);
+    if (rc == HM_OK) rc = dia_finish(A, s);
     if (rc != HM_OK) {
         hm_dia_destroy(A);
         return rc;
@@ -1240,7 +1329,12 @@ int hm_dia_from_cf(hm_cf* op, void* stream, hm_dia** out) {
     hipStream_t s = static_cast<hipStream_t>(stream);
     hm_dia* A = nullptr;
     HM_TRY(dia_alloc(op->h, op->w, &A));
-    int rc = cf_assemble(op, A->c64, 1, A->n, nullptr, nullptr, op->mode, s);
+    void* full = nullptr;
+    int rc = dev_alloc(&full, sizeof(double) * 25 * A->n);
+    if (rc == HM_OK)
+        rc = cf_assemble(op, static_cast<double*>(full), 1, A->n, nullptr, nullptr, op->mode, s);
+    if (rc == HM_OK) rc = dia_pack(static_cast<double*>(full), A, s);
+    dev_free(full);
     if (rc == HM_OK) rc = dia_finish(A, s);
     if (rc != HM_OK) {
         hm_dia_destroy(A);
@@ -1277,9 +1371,11 @@ int hm_dia_apply(hm_dia* A, int prec, const void* x, void* y, void* stream) {
 }
 
 int hm_dia_coefficients(hm_dia* A, double* coef, void* stream) {
-    HM_HIP(hipMemcpyAsync(coef, A->c64, sizeof(double) * 25 * A->n, hipMemcpyDeviceToDevice,
-                          static_cast<hipStream_t>(stream)));
-    return HM_OK;
+    if (!A || !coef) {
+        set_error("hm_dia_coefficients: null argument");
+        return HM_ERR_ARG;
+    }
+    return dia_unpack(A, coef, static_cast<hipStream_t>(stream));
 }
 
 int hm_csr_create(int64_t n, int64_t nnz, const int64_t* indptr, const int64_t* indices,

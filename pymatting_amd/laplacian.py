@@ -183,6 +183,10 @@ class DiaOperator:
             raise ValueError("matrix is not a 5x5 grid stencil on shape %s" % (shape,))
         coef = np.zeros((25, n))
         np.add.at(coef, ((dy + 2) * 5 + (dx + 2), A.row), A.data)
+        S = scipy.sparse.csr_matrix(A)
+        asym = abs(S - S.T).max() if S.nnz else 0.0
+        if asym > 1e-10 * (abs(S).max() if S.nnz else 1.0):
+            raise ValueError("the V-cycle needs a symmetric matrix (max |A - A^T| = %g)" % asym)
         return cls(coef, h, w)
 
     def __del__(self):
