@@ -64,8 +64,8 @@ struct CgState {
 // finalize kernel sums the partials in a fixed order.
 struct Reducer {
     double* partials;   // [N][max_blocks]
-    unsigned* counter;  // one counter per reducer
     int max_blocks;
+    unsigned upd_blocks;  // partials of a deferred CG update (slot 2)
 };
 
 template <int N>

@@ -380,7 +380,11 @@ struct DotSink {
 };
 
 enum DotKind {
-    kDotPQ = 0, kDotRzFirst = 1, kDotRzUpdate = 2, kDotAux = 3, kDotUpdate = 4, kDotReorth = 5
+    kDotPQ = 0, kDotRzFirst = 1, kDotRzUpdate = 2, kDotAux = 3, kDotUpdate = 4, kDotReorth = 5,
+    // <r, z> whose finalisation also completes the preceding CG update (its
+    // ||r||^2 partials in slot 2 of the same reducer): one finalize launch
+    // per iteration instead of two.
+    kDotUpdateRz = 6
 };
 
 // Applies a finished grid-wide sum to the solve state (cg.py:90-110).
@@ -416,21 +420,34 @@ __device__ __forceinline__ void dot_finalize(CgState* st, int kind, double s, do
     }
 }
 
+// Tile shapes. S = 0 (large grids): 64 x 8 points, two per thread, which
+// keeps the halo overhead low. S = 1 (coarse grids, latency-bound): one point
+// per thread, so a thread's coefficient loads (25) stay within the loads a
+// wave can keep in flight, and a small grid spreads over more workgroups.
+template <int S>
+struct TileDims { static constexpr int X = 64, Y = 8, CX = 32, CY = 4; };
+template <>
+struct TileDims<1> { static constexpr int X = 32, Y = 8, CX = 16, CY = 3; };
+
 // Plain output tiles of OX x OY points starting at (bx*OX, by*OY).
-#define HM_TILE_GEOM(OX_, OY_)                                                  \
-    static constexpr int OX = OX_, OY = OY_;                                    \
+#define HM_TILE_GEOM(S_)                                                        \
+    static constexpr int OX = TileDims<S_>::X, OY = TileDims<S_>::Y;            \
     __device__ __forceinline__ int ox0(int bx) const { return bx * OX; }        \
     __device__ __forceinline__ int oy0(int by) const { return by * OY; }
 
 // y = A x, optionally with <x, A x> (the CG SpMV, cg.py:85-90).
-template <typename T>
+template <typename T, int S = 0>
 struct OutApply {
-    HM_TILE_GEOM(64, 8)
+    HM_TILE_GEOM(S)
     static constexpr int kProf = 0;
     T* y;
     DotSink dot;
     T* xout = nullptr;  // optional copy of the input at the tile's points
-    __device__ __forceinline__ void point(long long g, int, int, T Ax, T xi, double& acc) const {
+    OutApply<T, 1> small() const { return {y, dot, xout}; }
+    static constexpr int kPre = 0;
+    __device__ __forceinline__ T preload(int, long long) const { return T(0); }
+    __device__ __forceinline__ void point(long long g, int, int, T Ax, T xi, double& acc,
+                                          const T*) const {
         y[g] = Ax;
         if (xout) xout[g] = xi;
         acc += (double)xi * (double)Ax;
@@ -439,32 +456,40 @@ struct OutApply {
 };
 
 // r = b - A x.
-template <typename T>
+template <typename T, int S = 0>
 struct OutResidual {
-    HM_TILE_GEOM(64, 8)
+    HM_TILE_GEOM(S)
     static constexpr int kProf = 3;
     const T* b;
     T* r;
     DotSink dot;
-    __device__ __forceinline__ void point(long long g, int, int, T Ax, T, double&) const {
-        r[g] = b[g] - Ax;
+    OutResidual<T, 1> small() const { return {b, r, dot}; }
+    static constexpr int kPre = 1;
+    __device__ __forceinline__ T preload(int, long long g) const { return b[g]; }
+    __device__ __forceinline__ void point(long long g, int, int, T Ax, T, double&,
+                                          const T* pv) const {
+        r[g] = pv[0] - Ax;
     }
     __device__ __forceinline__ void outside(int, int) const {}
 };
 
 // z = x + omega (b - A x) / D (vcycle.py:42), optionally with <b, z>.
-template <typename T>
+template <typename T, int S = 0>
 struct OutSmooth {
-    HM_TILE_GEOM(64, 8)
+    HM_TILE_GEOM(S)
     static constexpr int kProf = 2;
     const T* b;
     const T* D;
     T omega;
     T* z;
     DotSink dot;
-    __device__ __forceinline__ void point(long long g, int, int, T Ax, T xi, double& acc) const {
-        const T bg = b[g];
-        const T zg = xi + omega * (bg - Ax) / D[g];
+    OutSmooth<T, 1> small() const { return {b, D, omega, z, dot}; }
+    static constexpr int kPre = 2;
+    __device__ __forceinline__ T preload(int j, long long g) const { return j == 0 ? b[g] : D[g]; }
+    __device__ __forceinline__ void point(long long g, int, int, T Ax, T xi, double& acc,
+                                          const T* pv) const {
+        const T bg = pv[0];
+        const T zg = xi + omega * (bg - Ax) / pv[1];
         z[g] = zg;
         acc += (double)bg * (double)zg;
     }
@@ -473,20 +498,24 @@ struct OutSmooth {
 
 // Residual b - A x on the fine points of a coarse tile, restricted with the
 // full-weighting stencil to bc (vcycle.py:77-80).
-template <typename T>
+template <typename T, int S = 0>
 struct OutRestrict {
     static constexpr int kProf = 1;
-    static constexpr int CX = 32, CY = 4;
+    static constexpr int CX = TileDims<S>::CX, CY = TileDims<S>::CY;
     static constexpr int OX = 2 * CX + 1, OY = 2 * CY + 1;
     const T* b;
     T* bc;
     int w2, h2;
     T* sres;  // set by the engine: LDS [OY][OX]
     DotSink dot;
+    OutRestrict<T, 1> small() const { return {b, bc, w2, h2, sres, dot}; }
     __device__ __forceinline__ int ox0(int bx) const { return 2 * CX * bx - 1; }
     __device__ __forceinline__ int oy0(int by) const { return 2 * CY * by - 1; }
-    __device__ __forceinline__ void point(long long g, int lx, int ly, T Ax, T, double&) const {
-        sres[ly * OX + lx] = b[g] - Ax;
+    static constexpr int kPre = 1;
+    __device__ __forceinline__ T preload(int, long long g) const { return b[g]; }
+    __device__ __forceinline__ void point(long long, int lx, int ly, T Ax, T, double&,
+                                          const T* pv) const {
+        sres[ly * OX + lx] = pv[0] - Ax;
     }
     __device__ __forceinline__ void outside(int lx, int ly) const { sres[ly * OX + lx] = T(0); }
     __device__ void restrict_tile(int bx, int by) const {
@@ -507,23 +536,41 @@ struct OutRestrict {
 
 template <class Out>
 struct IsRestrict { static constexpr bool value = false; };
-template <typename T>
-struct IsRestrict<OutRestrict<T>> { static constexpr bool value = true; };
+template <typename T, int S>
+struct IsRestrict<OutRestrict<T, S>> { static constexpr bool value = true; };
 
 // -------------------------------------------------------------------- kernel
 
 template <typename T, class Op, class In, class Out>
 __global__ __launch_bounds__(kBlock) void tile_kernel(Op op, In in, Out out, int h, int w,
                                                       const CgState* guard) {
-    if (solve_done(guard)) return;
+    // The guard flag is read first and tested after the tile's loads have been
+    // issued, so its latency overlaps theirs instead of adding to it.
+    const bool done = solve_done(guard);
     constexpr int OX = Out::OX, OY = Out::OY;
     constexpr int RX = OX + 4, RY = OY + 4;
+    constexpr int NO = (OX * OY + kBlock - 1) / kBlock;
+    constexpr int NP = Out::kPre > 0 ? Out::kPre : 1;
     __shared__ T sx[RY][RX];
     __shared__ typename Op::template Shared<OX, OY> sh;
     __shared__ T sres[IsRestrict<Out>::value ? OX * OY : 1];
     const int ox0 = out.ox0(blockIdx.x), oy0 = out.oy0(blockIdx.y);
     typename Op::template Prefetch<OX, OY> pf;
     op.template prefetch<OX, OY>(pf, ox0, oy0);
+    // Per-point output operands (b, D), loaded before the barrier.
+    T pre[NO][NP];
+    if constexpr (Out::kPre > 0) {
+#pragma unroll
+        for (int k = 0; k < NO; ++k) {
+            const int t = threadIdx.x + k * kBlock;
+            const int ly = t / OX, lx = t - ly * OX;
+            const int gx = ox0 + lx, gy = oy0 + ly;
+            const bool in = t < OX * OY && gx >= 0 && gx < w && gy >= 0 && gy < h;
+            const long long g = in ? (long long)gy * w + gx : 0;
+#pragma unroll
+            for (int j = 0; j < Out::kPre; ++j) pre[k][j] = in ? out.preload(j, g) : T(0);
+        }
+    }
 
     constexpr int NR = (RX * RY + kBlock - 1) / kBlock;
 #pragma unroll
@@ -539,6 +586,7 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(Op op, In in, Out out, int
             op.template load_point<OX, OY>(sh, ly, lx, inb, g, xv);
         }
     }
+    if (done) return;  // uniform across the workgroup
     __syncthreads();
     op.template prepare<OX, OY>(sh, pf);
     __syncthreads();
@@ -546,7 +594,6 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(Op op, In in, Out out, int
     Out o = out;
     if constexpr (IsRestrict<Out>::value) o.sres = sres;
     double acc = 0.0;
-    constexpr int NO = (OX * OY + kBlock - 1) / kBlock;
 #pragma unroll
     for (int k = 0; k < NO; ++k) {
         const int t = threadIdx.x + k * kBlock;
@@ -560,7 +607,7 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(Op op, In in, Out out, int
         const long long g = (long long)gy * w + gx;
         const T xi = sx[ly + 2][lx + 2];
         const T Ax = op.template apply<OX, OY>(sh, pf, sx, k, lx, ly, xi, g);
-        o.point(g, lx, ly, Ax, xi, acc);
+        o.point(g, lx, ly, Ax, xi, acc, pre[k]);
     }
     if constexpr (IsRestrict<Out>::value) {
         __syncthreads();
@@ -572,34 +619,69 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(Op op, In in, Out out, int
     }
 }
 
-// Stage two of every reduction: sums nblocks partials per value in a fixed
-// order and applies the result to the solve state.
+// Stage two of every reduction: one 1024-thread workgroup sums nblocks
+// partials per value in a fixed order and applies the result to the solve
+// state. All loads of a thread are issued before the first add.
+constexpr int kFinBlock = 1024;
+
 template <int N>
-__global__ __launch_bounds__(kBlock) void k_finalize(Reducer red, unsigned nblocks, CgState* st,
-                                                     int kind, const CgState* guard) {
-    if (solve_done(guard)) return;
-    // Eight independent partial sums per thread keep eight loads in flight.
-    constexpr int U = 8;
+__global__ __launch_bounds__(kFinBlock) void k_finalize(Reducer red, unsigned nblocks,
+                                                        CgState* st, int kind,
+                                                        const CgState* guard) {
+    constexpr int U = 4;
+    const bool done = solve_done(guard);
+    const double* src[N];
+    unsigned cnt[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        src[k] = red.partials + (size_t)k * red.max_blocks;
+        cnt[k] = nblocks;
+    }
+    if (kind == kDotUpdateRz) {
+        // The residual norm of the CG update (slot 2) and <r, z> of the
+        // preconditioner that follows it (slot 0), finalised together: the
+        // convergence test first, then beta only for an unconverged solve.
+        src[0] = red.partials + 2 * (size_t)red.max_blocks;
+        cnt[0] = red.upd_blocks;
+        if (N > 1) {
+            src[N - 1] = red.partials;
+            cnt[N - 1] = nblocks;
+        }
+    }
+    unsigned most = 0;
+#pragma unroll
+    for (int k = 0; k < N; ++k) most = cnt[k] > most ? cnt[k] : most;
+    double part[N][U];
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+#pragma unroll
+        for (int u = 0; u < U; ++u) part[k][u] = 0.0;
+    for (unsigned i0 = 0; i0 < most; i0 += kFinBlock * U) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const unsigned i = i0 + u * kFinBlock + threadIdx.x;
+#pragma unroll
+            for (int k = 0; k < N; ++k)
+                if (i < cnt[k]) part[k][u] += src[k][i];
+        }
+    }
     double acc[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        double part[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) part[u] = 0.0;
-        for (unsigned i0 = 0; i0 < nblocks; i0 += kBlock * U) {
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const unsigned i = i0 + u * kBlock + threadIdx.x;
-                if (i < nblocks) part[u] += red.partials[k * red.max_blocks + i];
-            }
-        }
         double t = 0.0;
 #pragma unroll
-        for (int u = 0; u < U; ++u) t += part[u];
+        for (int u = 0; u < U; ++u) t += part[k][u];
         acc[k] = t;
     }
-    block_sum<kBlock, N>(acc);
-    if (threadIdx.x == 0) dot_finalize(st, kind, acc[0], N > 1 ? acc[N > 1 ? 1 : 0] : 0.0);
+    if (done) return;  // uniform across the workgroup
+    block_sum<kFinBlock, N>(acc);
+    if (threadIdx.x != 0) return;
+    if (kind == kDotUpdateRz) {
+        dot_finalize(st, kDotUpdate, acc[0]);
+        if (!st->done) dot_finalize(st, kDotRzUpdate, acc[N - 1]);
+    } else {
+        dot_finalize(st, kind, acc[0], N > 1 ? acc[N - 1] : 0.0);
+    }
 }
 
 template <class Out>

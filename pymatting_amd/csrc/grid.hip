@@ -94,17 +94,15 @@ void pool_trim() {
 
 int make_reducer(Reducer* r, int max_blocks, int nvals) {
     r->max_blocks = max_blocks;
+    r->upd_blocks = 0;
     HM_TRY(dev_alloc(reinterpret_cast<void**>(&r->partials),
                      sizeof(double) * (size_t)max_blocks * nvals));
-    r->counter = nullptr;
     return HM_OK;
 }
 
 void free_reducer(Reducer* r) {
     dev_free(r->partials);
-    dev_free(r->counter);
     r->partials = nullptr;
-    r->counter = nullptr;
 }
 
 // ------------------------------------------------------------ Galerkin R A P
@@ -417,10 +415,14 @@ int launch_finalize(int nvals, const Reducer& red, unsigned nblocks, CgState* st
         set_error("reduction buffer too small for %u blocks", nblocks);
         return HM_ERR_ARG;
     }
-    if (nvals == 1)
-        k_finalize<1><<<1, kBlock, 0, s>>>(red, nblocks, st, kind, guard);
+    if (kind == kDotUpdateRz && (int)red.upd_blocks > red.max_blocks) {
+        set_error("reduction buffer too small for %u blocks", red.upd_blocks);
+        return HM_ERR_ARG;
+    }
+    if (nvals == 1 && kind != kDotUpdateRz)
+        k_finalize<1><<<1, kFinBlock, 0, s>>>(red, nblocks, st, kind, guard);
     else
-        k_finalize<2><<<1, kBlock, 0, s>>>(red, nblocks, st, kind, guard);
+        k_finalize<2><<<1, kFinBlock, 0, s>>>(red, nblocks, st, kind, guard);
     HM_LAUNCH_CHECK();
     return HM_OK;
 }

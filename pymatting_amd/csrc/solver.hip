@@ -99,11 +99,15 @@ int run_level(const LevelOp<T>& L, const In& in, const Out& out, const CgState* 
               hipStream_t s) {
     if (L.is_cf)
         return launch_tile<T>(L.cf, in, out, L.h, L.w, L.h / 2, L.w / 2, guard, s);
-    if (L.coarse) {
+    // Coarse grids below kSmallGrid points are latency-bound: one point per
+    // thread with its coefficients loaded before the tile barrier. Larger
+    // grids keep the bandwidth-oriented shape of the finest level.
+    constexpr long long kSmallGrid = 1 << 18;
+    if (L.coarse && (long long)L.h * L.w < kSmallGrid) {
         const DiaOp<T, true> op{L.dia.h, L.dia.w, L.dia.coef};
-        return launch_tile<T>(op, in, out, L.h, L.w, L.h / 2, L.w / 2, guard, s, true);
+        return launch_tile<T>(op, in, out.small(), L.h, L.w, L.h / 2, L.w / 2, guard, s, true);
     }
-    return launch_tile<T>(L.dia, in, out, L.h, L.w, L.h / 2, L.w / 2, guard, s);
+    return launch_tile<T>(L.dia, in, out, L.h, L.w, L.h / 2, L.w / 2, guard, s, L.coarse);
 }
 
 // ------------------------------------------------------------- vector kernels
@@ -234,8 +238,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dense_apply(const T* __restrict__ inv,
                                                         const T* __restrict__ b,
                                                         T* __restrict__ x, int n, CgState* st,
-                                                        Reducer red, int kind,
-                                                        const CgState* guard) {
+                                                        Reducer red, const CgState* guard) {
     if (solve_done(guard)) return;
     double acc = 0.0;
     for (int i = threadIdx.x + blockIdx.x * kBlock; i < n; i += kBlock * gridDim.x) {
@@ -245,10 +248,9 @@ __global__ __launch_bounds__(kBlock) void k_dense_apply(const T* __restrict__ in
         x[i] = s;
         acc += (double)b[i] * (double)s;
     }
-    if (st != nullptr) {  // a single workgroup: its block sum is the total
+    if (st != nullptr) {  // a single workgroup: one partial, finalised by the host's next launch
         double v[1] = {acc};
-        block_sum<kBlock, 1>(v);
-        if (threadIdx.x == 0) dot_finalize(st, kind, v[0]);
+        block_partials<kBlock, 1>(v, red);
     }
 }
 
@@ -403,12 +405,30 @@ __global__ __launch_bounds__(kTailBlock) void k_vc_tail(TailParams<T> P, const C
     extern __shared__ __align__(16) unsigned char tail_lds_raw[];
     T* lds = reinterpret_cast<T*>(tail_lds_raw);
     const int tid = threadIdx.x;
-    // Stage every level's stencil and the top level's right-hand side.
-    for (int l = 0; l < P.nlev; ++l) {
-        const TailView<T> V = tail_view(lds, P.lv[l]);
-        for (int i = tid; i < kSymSlots * V.n; i += kTailBlock) V.coef[i] = P.lv[l].coef[i];
-        if (l == 0)
-            for (int i = tid; i < V.n; i += kTailBlock) V.b[i] = P.lv[0].b[i];
+    // Stage every level's stencil and the top level's right-hand side. Each
+    // thread issues kStage loads before its first LDS store, so the copy costs
+    // a few memory latencies rather than one per loop trip.
+    {
+        constexpr int kStage = 8;
+        for (int l = 0; l <= P.nlev; ++l) {
+            const TailView<T> V = tail_view(lds, P.lv[l < P.nlev ? l : 0]);
+            const T* src = l < P.nlev ? P.lv[l].coef : P.lv[0].b;
+            T* dst = l < P.nlev ? V.coef : V.b;
+            const int cnt = l < P.nlev ? kSymSlots * V.n : V.n;
+            for (int i0 = 0; i0 < cnt; i0 += kTailBlock * kStage) {
+                T v[kStage];
+#pragma unroll
+                for (int u = 0; u < kStage; ++u) {
+                    const int i = i0 + u * kTailBlock + tid;
+                    v[u] = i < cnt ? src[i] : T(0);
+                }
+#pragma unroll
+                for (int u = 0; u < kStage; ++u) {
+                    const int i = i0 + u * kTailBlock + tid;
+                    if (i < cnt) dst[i] = v[u];
+                }
+            }
+        }
     }
     __syncthreads();
     // The pre-smoothed iterate of every level ends in .x or .t depending on
@@ -460,7 +480,20 @@ __global__ __launch_bounds__(kTailBlock) void k_vc_tail(TailParams<T> P, const C
         const bool staged = nn <= 2 * top.n;
         T* inv = top.t;  // t and u of the top level are free at this point
         if (staged) {
-            for (int i = tid; i < nn; i += kTailBlock) inv[i] = P.inv[i];
+            constexpr int kStage = 4;
+            for (int i0 = 0; i0 < nn; i0 += kTailBlock * kStage) {
+                T v[kStage];
+#pragma unroll
+                for (int u = 0; u < kStage; ++u) {
+                    const int i = i0 + u * kTailBlock + tid;
+                    v[u] = i < nn ? P.inv[i] : T(0);
+                }
+#pragma unroll
+                for (int u = 0; u < kStage; ++u) {
+                    const int i = i0 + u * kTailBlock + tid;
+                    if (i < nn) inv[i] = v[u];
+                }
+            }
             __syncthreads();
         }
         for (int i = tid; i < C.n; i += kTailBlock) {
@@ -559,18 +592,20 @@ int launch_tail(hm_vc* vc, size_t start, const CgState* guard, hipStream_t s) {
 }
 
 // z = V-cycle(r) (vcycle.py:46-100). If st is given, the final kernel also
-// forms <r, z> and finalises it as `kind`.
+// forms <r, z> and finalises it as `kind`, through `red` (the caller's
+// reducer, which a deferred CG update shares) or the V-cycle's own.
 template <typename T>
-int vcycle_apply_t(hm_vc* vc, const T* r, T* z, CgState* st, int kind, const CgState* guard,
-                   hipStream_t s) {
+int vcycle_apply_t(hm_vc* vc, const T* r, T* z, CgState* st, const Reducer* red, int kind,
+                   const CgState* guard, hipStream_t s) {
     const size_t L = vc->levels.size();
     const T omega = (T)vc->omega;
+    const Reducer& R = red ? *red : vc->red;
     const DotSink nodot{nullptr, Reducer{}, 0};
-    const DotSink dot{st, vc->red, kind};
+    const DotSink dot{st, R, kind};
     if (L == 1) {
-        k_dense_apply<T><<<1, kBlock, 0, s>>>(vc_inv<T>(vc), r, z, vc->n_direct, st, vc->red,
-                                              kind, guard);
+        k_dense_apply<T><<<1, kBlock, 0, s>>>(vc_inv<T>(vc), r, z, vc->n_direct, st, R, guard);
         HM_LAUNCH_CHECK();
+        if (st) return launch_finalize(1, R, 1, st, kind, guard, s);
         return HM_OK;
     }
     std::vector<T*> result(L, nullptr), pre_x(L, nullptr);
@@ -607,7 +642,7 @@ int vcycle_apply_t(hm_vc* vc, const T* r, T* z, CgState* st, int kind, const CgS
     } else {
         const VcLevel& cl = vc->levels[L - 1];
         k_dense_apply<T><<<1, kBlock, 0, s>>>(vc_inv<T>(vc), vptr<T>(cl.b), vptr<T>(cl.x),
-                                              vc->n_direct, nullptr, Reducer{}, 0, guard);
+                                              vc->n_direct, nullptr, Reducer{}, guard);
         HM_LAUNCH_CHECK();
     }
     result[tail] = vptr<T>(vc->levels[tail].x);
@@ -637,26 +672,26 @@ int vcycle_apply_t(hm_vc* vc, const T* r, T* z, CgState* st, int kind, const CgS
         int rc;
         if (vc->pre == 0)
             rc = post_stage<T>(op, InZero<T>{}, b, e, cl.h, cl.w, vc->post, omega, target, other,
-                               fin, r, guard, s, vc->red);
+                               fin, r, guard, s, R);
         else if (vc->pre == 1)
             rc = post_stage<T>(op, InJac0<T>{b, op.D, omega}, b, e, cl.h, cl.w, vc->post, omega,
-                               target, other, fin, r, guard, s, vc->red);
+                               target, other, fin, r, guard, s, R);
         else
             rc = post_stage<T>(op, InVec<T>{pre_x[l]}, b, e, cl.h, cl.w, vc->post, omega,
-                               target, other, fin, r, guard, s, vc->red);
+                               target, other, fin, r, guard, s, R);
         HM_TRY(rc);
         result[l] = target;
     }
     return HM_OK;
 }
 
-int vcycle_apply(hm_vc* vc, const void* r, void* z, CgState* st, int kind, const CgState* guard,
-                 hipStream_t s) {
+int vcycle_apply(hm_vc* vc, const void* r, void* z, CgState* st, const Reducer* red, int kind,
+                 const CgState* guard, hipStream_t s) {
     if (vc->prec == HM_F64)
         return vcycle_apply_t<double>(vc, static_cast<const double*>(r), static_cast<double*>(z),
-                                      st, kind, guard, s);
+                                      st, red, kind, guard, s);
     return vcycle_apply_t<float>(vc, static_cast<const float*>(r), static_cast<float*>(z), st,
-                                 kind, guard, s);
+                                 red, kind, guard, s);
 }
 
 }  // namespace hm
@@ -775,7 +810,7 @@ int op_residual(const OpHandle& A, const T* b, const T* x, T* r, hipStream_t s) 
 template <typename T>
 int pc_apply(const PcHandle& M, const T* r, T* z, CgState* st, const Reducer& red, int kind,
              const CgState* guard, long long n, hipStream_t s) {
-    if (M.kind == HM_PC_VCYCLE) return vcycle_apply(M.vc, r, z, st, kind, guard, s);
+    if (M.kind == HM_PC_VCYCLE) return vcycle_apply(M.vc, r, z, st, &red, kind, guard, s);
     if (M.kind == HM_PC_CSR || M.kind == HM_PC_HOST) {
         if constexpr (std::is_same<T, double>::value) {
             if (M.kind == HM_PC_CSR)
@@ -867,14 +902,17 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
     HM_TRY(z.alloc(sizeof(T) * n));
     if (opt.reorth) HM_TRY(rold.alloc(sizeof(T) * n));
     Reducer red{};
-    HM_TRY(make_reducer(&red, reducer_blocks(n, A.h(), A.w()), 2));
+    const unsigned vb = vec_blocks(n);
+    HM_TRY(make_reducer(&red, std::max<int>(reducer_blocks(n, A.h(), A.w()), (int)vb), 3));
     struct RedGuard { Reducer* r; ~RedGuard() { free_reducer(r); } } rg{&red};
+    red.upd_blocks = vb;
+    Reducer red_upd = red;  // slot 2: ||r||^2 partials of the CG update
+    red_upd.partials += 2 * (size_t)red.max_blocks;
     DeviceState st;
     HM_TRY(st.init());
     st.host->atol = opt.atol;
     st.host->rtol = opt.rtol;
     HM_TRY(st.push(s));
-    const unsigned vb = vec_blocks(n);
     const bool interactive = A.cb || M.kind == HM_PC_HOST || opt.callback;
 
     // norm_b = ||b|| (cg.py:66)
@@ -919,6 +957,11 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
     T* p_next = p2.as<T>(); // direction being formed
 
     const int check_every = interactive ? 1 : (opt.check_every < 1 ? 1 : opt.check_every);
+    // Without a host round trip per iteration, the convergence test of the
+    // update is finalised together with the next <r, z> (one launch fewer per
+    // iteration); the preconditioner then runs once more after convergence,
+    // which changes neither x nor the iteration count.
+    const bool defer = !interactive && !opt.reorth;
     int enqueued = 0;
     const DotSink pq{st.dev, red, kDotPQ};
     while (enqueued < maxiter) {
@@ -933,9 +976,9 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
             {
                 ProfScope prof_u(kProfCgUpdate, n, sizeof(T) == 4, s);
                 k_cg_update<T><<<vb, kBlock, 0, s>>>(x, p_cur, r.as<T>(), q.as<T>(), n,
-                                                     st.dev, red);
+                                                     st.dev, red_upd);
                 HM_LAUNCH_CHECK();
-                HM_TRY(launch_finalize(1, red, vb, st.dev, kDotUpdate, st.dev, s));
+                if (!defer) HM_TRY(launch_finalize(1, red_upd, vb, st.dev, kDotUpdate, st.dev, s));
             }
             if (interactive) {
                 HM_TRY(st.pull(s));
@@ -952,8 +995,8 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
                 HM_TRY(pc_apply<T>(M, r.as<T>(), z.as<T>(), nullptr, red, 0, st.dev, n, s));
                 HM_TRY(launch_dot<T>(r.as<T>(), z.as<T>(), rold.as<T>(), n, st.dev, red, kDotRzUpdate, s));
             } else {
-                HM_TRY(pc_apply<T>(M, r.as<T>(), z.as<T>(), st.dev, red, kDotRzUpdate, st.dev, n,
-                                   s));
+                HM_TRY(pc_apply<T>(M, r.as<T>(), z.as<T>(), st.dev, red,
+                                   defer ? kDotUpdateRz : kDotRzUpdate, st.dev, n, s));
             }
         }
         enqueued += k;
@@ -1468,7 +1511,7 @@ int hm_vcycle_apply(hm_vc* vc, const void* r, void* z, void* stream) {
         set_error("hm_vcycle_apply: null argument");
         return HM_ERR_ARG;
     }
-    return vcycle_apply(vc, r, z, nullptr, 0, nullptr, static_cast<hipStream_t>(stream));
+    return vcycle_apply(vc, r, z, nullptr, nullptr, 0, nullptr, static_cast<hipStream_t>(stream));
 }
 
 int hm_copy(void* dst, const void* src, int64_t bytes, int kind, void* stream) {
