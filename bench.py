@@ -15,14 +15,19 @@ this workload is ~3e-3 away from the converged one. The converged solution
 (float64, rtol 1e-11) is computed outside the timed regions and every timed
 alpha is checked against it.
 
-The roofline object covers the kernel class that takes most time in a
-mixed step, from HIP events recorded around each of its launches on its own
-stream during one extra step after the timed ones (events between kernels
-would perturb the timed steps): achieved = algorithmic bytes per launch /
-mean launch duration. The CPU baseline times the oracle's restatement of the reference
+The roofline object covers the stored-stencil SpMV on the finest grid, the
+pass that every PCG iteration and every fine-level smoothing sweep makes
+(y = A x over the 13 symmetric coefficient planes, float32): HIP events on
+the library's stream bracket 50 back-to-back launches on the solve's own
+operator and vectors, and achieved = algorithmic bytes per launch / mean
+launch duration. A per-class breakdown of one extra mixed step (events
+around every launch, which adds a few microseconds per launch) is reported
+beside it. The CPU baseline times the oracle's restatement of the reference
 (scipy CSR, vcycle PCG, float64) single-threaded on the 320x240 workload.
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+       python bench.py --pmc-probe   (SpMV launches and a 256 MiB copy only,
+                                      for hardware-counter runs)
 """
 import argparse
 import ctypes
@@ -36,6 +41,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the float32 stored-stencil SpMV at 1920x1080, from
+# rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by the same counters on a copy
+# kernel of known size (python bench.py --pmc-probe); None until measured.
+TRAFFIC_SPMV_F32 = None
 
 # Algorithmic bytes per grid point and launch, by kernel class, for vector
 # element size s (8 float64, 4 float32). Matrix-free passes read the image
@@ -68,6 +77,56 @@ KINDS = ["cf_apply", "cf_restrict", "cf_smooth", "cf_residual", "dia_apply", "di
          "coarse_apply", "coarse_restrict", "coarse_smooth", "coarse_residual"]
 
 
+# Algorithmic bytes per grid point of the stored-stencil SpMV y = A x:
+# 13 coefficient planes, x and y.
+def spmv_bytes_per_point(s):
+    return 13 * s + 2 * s
+
+
+def time_spmv(L, handle, prec, x, y, reps=50):
+    """Mean duration of `reps` back-to-back hm_dia_apply launches (HIP events
+    on the library's stream, which is torch's current stream)."""
+    import torch
+    from pymatting_amd import _ffi
+    stream = _ffi.stream()
+    for _ in range(3):
+        _ffi.check(L.hm_dia_apply(handle, prec, _ffi.ptr(x), _ffi.ptr(y), stream))
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        _ffi.check(L.hm_dia_apply(handle, prec, _ffi.ptr(x), _ffi.ptr(y), stream))
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps * 1e-3  # seconds
+
+
+def pmc_probe():
+    """20 float32 SpMV launches on the 1080p stored operator and 5 copies of
+    256 MiB (the calibration kernel for FETCH_SIZE/WRITE_SIZE), nothing else."""
+    import torch
+    import pymatting_amd as pm
+    from pymatting_amd import _ffi, synthetic
+    image_h, trimap_h, _ = synthetic.config2()
+    solver = pm.LambdaSolver(torch.from_numpy(image_h).cuda(), torch.from_numpy(trimap_h).cuda(),
+                             precision="mixed", level0="stencil")
+    n = solver.op.n
+    x = torch.rand(n, device="cuda", dtype=torch.float32)
+    y = torch.empty_like(x)
+    L = _ffi.lib()
+    for _ in range(20):
+        _ffi.check(L.hm_dia_apply(solver.op.handle, _ffi.HM_F32, _ffi.ptr(x), _ffi.ptr(y),
+                                  _ffi.stream()))
+    a = torch.empty(64 << 20, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": "spmv_f32_1080p", "n": n, "spmv_launches": 20,
+                      "copy_bytes": a.numel() * 4, "copy_launches": 5}))
+
+
 def cpu_baseline():
     """Oracle (port of the reference) on the 320x240 workload, one thread."""
     code = (
@@ -98,11 +157,15 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the workload is a single-image solve with no "
               "multi-GPU path (see the sharding design notes); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.pmc_probe:
+        pmc_probe()
+        return
 
     import numpy as np
     import torch
@@ -123,11 +186,14 @@ def main():
     FP64_REF_RTOL = dict(precision="f64", level0="stencil", rtol=1e-7)
     MIXED_MF = dict(MIXED, level0="matrix_free")
 
+    last = {}
+
     def step(cfg):
         kw = dict(cfg)
         solver = pm.LambdaSolver(image, trimap, precision=kw.pop("precision"),
                                  level0=kw.pop("level0"))
         alpha = solver.solve(**kw)
+        last["solver"] = solver
         return alpha, solver.stats
 
     def timed(cfg, steps, prof=False):
@@ -147,6 +213,14 @@ def main():
     torch.cuda.synchronize()
 
     alpha_m, st_m, t_mixed = timed(MIXED, args.steps)
+    # Roofline kernel: the finest-grid stored-stencil SpMV on this solve's operator.
+    op = last["solver"].op
+    xs = torch.rand(n, device="cuda", dtype=torch.float32)
+    ys = torch.empty_like(xs)
+    t_spmv32 = time_spmv(L, op.handle, _ffi.HM_F32, xs, ys)
+    xd = xs.double()
+    yd = torch.empty_like(xd)
+    t_spmv64 = time_spmv(L, op.handle, _ffi.HM_F64, xd, yd)
     # Separate pass with per-launch events (they would perturb the timing).
     timed(MIXED, 1, prof=True)
     prof = {}
@@ -171,15 +245,18 @@ def main():
 
     e_m, e_d, e_r, e_f = err(alpha_m), err(alpha_d), err(alpha_r), err(alpha_f)
 
-    (kind, prec), (ms, cnt, pts, s) = max(
-        ((k, v) for k, v in prof.items() if bytes_per_point(k[0], 8) is not None),
-        key=lambda kv: kv[1][0])
-    avg_s = ms / cnt / 1e3
-    bytes_launch = bytes_per_point(kind, s) * pts / cnt
-    achieved = bytes_launch / avg_s / 1e9
-    breakdown = {"%s_%s" % (k, "f32" if p else "f64"): {"ms_total": round(v[0], 3),
-                                                         "launches": v[1]}
-                 for (k, p), v in sorted(prof.items(), key=lambda kv: -kv[1][0])}
+    bytes_launch = spmv_bytes_per_point(4) * n
+    achieved = bytes_launch / t_spmv32 / 1e9
+    achieved64 = spmv_bytes_per_point(8) * n / t_spmv64 / 1e9
+    breakdown = {}
+    for (k, p), v in sorted(prof.items(), key=lambda kv: -kv[1][0]):
+        ms, cnt, pts, s = v
+        entry = {"ms_total": round(ms, 3), "launches": cnt,
+                 "avg_us_with_events": round(ms / cnt * 1e3, 2)}
+        bpp = bytes_per_point(k, s)
+        if bpp is not None:
+            entry["GBps_with_events"] = round(bpp * pts / (ms * 1e-3) / 1e9, 1)
+        breakdown["%s_%s" % (k, "f32" if p else "f64")] = entry
 
     cpu = None if args.no_cpu else cpu_baseline()
 
@@ -211,11 +288,16 @@ def main():
                            "rms_dalpha": e_r[1]},
         "mixed_matrix_free": {"value": round(n / t_mf / 1e6, 3), "unit": "Mpix/s",
                               "iterations": st_f["iterations"], "max_abs_dalpha": e_f[0]},
-        "roofline": {"bound": "hbm", "kernel": "%s_%s" % (kind, "f32" if prec else "f64"),
+        "roofline": {"bound": "hbm",
+                     "kernel": "tile_kernel<float, DiaOp<float, false>, InVec<float>, "
+                               "OutApply<float, 0>> (stored-stencil SpMV, 1920x1080)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_SPMV_F32,
                      "bytes_per_launch": int(bytes_launch),
-                     "avg_launch_us": round(avg_s * 1e6, 2)},
+                     "avg_launch_us": round(t_spmv32 * 1e6, 2),
+                     "f64": {"achieved": round(achieved64, 1),
+                             "frac": round(achieved64 / HBM_PEAK_GBS, 4),
+                             "avg_launch_us": round(t_spmv64 * 1e6, 2)}},
         "kernel_breakdown_timed_mixed": breakdown,
         "cpu_baseline": cpu,
     }

@@ -541,9 +541,27 @@ struct IsRestrict<OutRestrict<T, S>> { static constexpr bool value = true; };
 
 // -------------------------------------------------------------------- kernel
 
+// Workgroups are dispatched round-robin over the XCDs (workgroup b runs on
+// XCD b % kXcd), and each XCD has its own L2. Tile t (row-major over the
+// grid's tiles) is given to the workgroup that makes XCD j process the j-th
+// contiguous band of tiles in raster order, so the halo rows and columns a
+// tile shares with its neighbours are read while they are in the same L2.
+constexpr unsigned kXcd = 8;
+
+struct TileMap {
+    unsigned tiles_x, ntiles;
+    __device__ __forceinline__ void tile(unsigned b, int& bx, int& by) const {
+        const unsigned q = ntiles / kXcd, r = ntiles % kXcd;
+        const unsigned j = b % kXcd, k = b / kXcd;
+        const unsigned t = j * q + (j < r ? j : r) + k;
+        by = (int)(t / tiles_x);
+        bx = (int)(t - (unsigned)by * tiles_x);
+    }
+};
+
 template <typename T, class Op, class In, class Out>
 __global__ __launch_bounds__(kBlock) void tile_kernel(Op op, In in, Out out, int h, int w,
-                                                      const CgState* guard) {
+                                                      const CgState* guard, TileMap map) {
     // The guard flag is read first and tested after the tile's loads have been
     // issued, so its latency overlaps theirs instead of adding to it.
     const bool done = solve_done(guard);
@@ -554,7 +572,9 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(Op op, In in, Out out, int
     __shared__ T sx[RY][RX];
     __shared__ typename Op::template Shared<OX, OY> sh;
     __shared__ T sres[IsRestrict<Out>::value ? OX * OY : 1];
-    const int ox0 = out.ox0(blockIdx.x), oy0 = out.oy0(blockIdx.y);
+    int bx, by;
+    map.tile(blockIdx.x, bx, by);
+    const int ox0 = out.ox0(bx), oy0 = out.oy0(by);
     typename Op::template Prefetch<OX, OY> pf;
     op.template prefetch<OX, OY>(pf, ox0, oy0);
     // Per-point output operands (b, D), loaded before the barrier.
@@ -611,7 +631,7 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(Op op, In in, Out out, int
     }
     if constexpr (IsRestrict<Out>::value) {
         __syncthreads();
-        o.restrict_tile(blockIdx.x, blockIdx.y);
+        o.restrict_tile(bx, by);
     }
     if (o.dot.st != nullptr) {
         double v[1] = {acc};
@@ -685,11 +705,14 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(Reducer red, unsigned nb
 }
 
 template <class Out>
-inline dim3 tile_grid(const Out&, int h, int w, int h2, int w2) {
-    if constexpr (IsRestrict<Out>::value)
-        return dim3(div_up(w2, Out::CX), div_up(h2, Out::CY));
-    else
-        return dim3(div_up(w, Out::OX), div_up(h, Out::OY));
+inline TileMap tile_map(const Out&, int h, int w, int h2, int w2) {
+    if constexpr (IsRestrict<Out>::value) {
+        const unsigned tx = div_up(w2, Out::CX), ty = div_up(h2, Out::CY);
+        return TileMap{tx, tx * ty};
+    } else {
+        const unsigned tx = div_up(w, Out::OX), ty = div_up(h, Out::OY);
+        return TileMap{tx, tx * ty};
+    }
 }
 
 }  // namespace hm

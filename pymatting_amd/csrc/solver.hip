@@ -38,17 +38,17 @@ void set_error(const char* fmt, ...) {
 template <typename T, class Op, class In, class Out>
 int launch_tile(const Op& op, const In& in, const Out& out, int h, int w, int h2, int w2,
                 const CgState* guard, hipStream_t s, bool coarse = false) {
-    const dim3 grid = tile_grid(out, h, w, h2, w2);
-    if (out.dot.st != nullptr && (long long)grid.x * grid.y > out.dot.red.max_blocks) {
-        set_error("reduction buffer too small for %u x %u blocks", grid.x, grid.y);
+    const TileMap map = tile_map(out, h, w, h2, w2);
+    if (out.dot.st != nullptr && (long long)map.ntiles > out.dot.red.max_blocks) {
+        set_error("reduction buffer too small for %u blocks", map.ntiles);
         return HM_ERR_ARG;
     }
     ProfScope prof((coarse ? (int)kProfCoarseApply : Op::kProfBase) + Out::kProf, (long long)h * w,
                    sizeof(T) == 4, s);
-    tile_kernel<T, Op, In, Out><<<grid, kBlock, 0, s>>>(op, in, out, h, w, guard);
+    tile_kernel<T, Op, In, Out><<<map.ntiles, kBlock, 0, s>>>(op, in, out, h, w, guard, map);
     HM_LAUNCH_CHECK();
     if (out.dot.st != nullptr)
-        return launch_finalize(1, out.dot.red, grid.x * grid.y, out.dot.st, out.dot.kind, guard, s);
+        return launch_finalize(1, out.dot.red, map.ntiles, out.dot.st, out.dot.kind, guard, s);
     return HM_OK;
 }
 
