@@ -23,7 +23,8 @@ operator and vectors, and achieved = algorithmic bytes per launch / mean
 launch duration. A per-class breakdown of one extra mixed step (events
 around every launch, which adds a few microseconds per launch) is reported
 beside it. The CPU baseline times the oracle's restatement of the reference
-(scipy CSR, vcycle PCG, float64) single-threaded on the 320x240 workload.
+(scipy CSR, vcycle PCG at rtol 1e-7, float64) single-threaded on the same
+1920x1080 workload, one solve.
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
        python bench.py --pmc-probe   (SpMV launches and a 256 MiB copy only,
@@ -42,9 +43,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the float32 stored-stencil SpMV at 1920x1080, from
-# rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by the same counters on a copy
-# kernel of known size (python bench.py --pmc-probe); None until measured.
-TRAFFIC_SPMV_F32 = None
+# two rocprofv3 --pmc passes over `python bench.py --pmc-probe` (FETCH_SIZE,
+# then WRITE_SIZE, KiB per dispatch), each scaled by what the same counter
+# reads for a 256 MiB copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0):
+# 57585.8 KiB x 2.0 + 8100.0 KiB = 126.2 MB (algorithmic: 124.4 MB).
+TRAFFIC_SPMV_F32 = 126218326
 
 # Algorithmic bytes per grid point and launch, by kernel class, for vector
 # element size s (8 float64, 4 float32). Matrix-free passes read the image
@@ -128,27 +131,27 @@ def pmc_probe():
 
 
 def cpu_baseline():
-    """Oracle (port of the reference) on the 320x240 workload, one thread."""
+    """The oracle (a CPU restatement of the reference: numpy/scipy CSR assembly,
+    vcycle-preconditioned CG at the reference's default rtol 1e-7, float64) on
+    the same 1920x1080 workload, once, in one thread."""
     code = (
         "import sys, time, json; sys.path.insert(0, %r)\n"
         "from oracle.cf import solve_alpha_lambda\n"
         "from pymatting_amd import synthetic\n"
-        "image, trimap, _ = synthetic.config1()\n"
-        "ts = []\n"
-        "for _ in range(3):\n"
-        "    t = time.perf_counter(); solve_alpha_lambda(image, trimap); ts.append(time.perf_counter() - t)\n"
-        "print(json.dumps({'t': min(ts), 'n': image.shape[0] * image.shape[1]}))\n" % ROOT)
+        "image, trimap, _ = synthetic.config2()\n"
+        "t = time.perf_counter(); solve_alpha_lambda(image, trimap); t = time.perf_counter() - t\n"
+        "print(json.dumps({'t': t, 'n': image.shape[0] * image.shape[1]}))\n" % ROOT)
     env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1",
                HIP_VISIBLE_DEVICES="", CUDA_VISIBLE_DEVICES="")
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env,
-                         timeout=600)
+                         timeout=900)
     if out.returncode != 0:
         return None
     r = json.loads(out.stdout.strip().splitlines()[-1])
     return {"value": round(r["n"] / r["t"] / 1e6, 4), "unit": "Mpix/s", "cores": 1,
             "kind": "port",
-            "sample": "320x240 lambda-system solve (cf_laplacian, make_linear_system, vcycle-PCG "
-                      "rtol 1e-7, float64; scipy CSR), best of 3, %.2f s each" % r["t"]}
+            "sample": "one 1920x1080 lambda-system solve (cf_laplacian, make_linear_system, "
+                      "vcycle-PCG rtol 1e-7, float64; numpy/scipy CSR), %.1f s" % r["t"]}
 
 
 def main():
