@@ -120,6 +120,15 @@ int hm_csr_apply(hm_csr* A, const double* x, double* y, void* stream);
 int hm_vcycle_create(hm_cf* cf, hm_dia* dia, int h, int w, int num_pre_iter,
                      int num_post_iter, double omega, int direct_solve_size, int prec,
                      void* stream, hm_vc** out);
+/* As hm_vcycle_create, with two extensions the reference does not have:
+ * max_levels > 0 caps the number of grids (the coarsest is then the
+ * max_levels-th, (h >> (max_levels-1), w >> (max_levels-1))); if that grid
+ * has more than 2048 points it is solved approximately by coarse_sweeps
+ * damped Jacobi sweeps from zero (coarse_sweeps = 0 then fails). With
+ * max_levels = 0 and coarse_sweeps = 0 it is hm_vcycle_create. */
+int hm_vcycle_create_ex(hm_cf* cf, hm_dia* dia, int h, int w, int num_pre_iter,
+                        int num_post_iter, double omega, int direct_solve_size, int prec,
+                        int max_levels, int coarse_sweeps, void* stream, hm_vc** out);
 /* Direct-solve-only "V-cycle" for a dense n x n device matrix (n <= 2048):
  * the case h*w <= direct_solve_size of vcycle.py:59-60 for operators that are
  * not grid stencils. */

@@ -83,6 +83,8 @@ _SIGS = {
     "hm_csr_destroy": (i32, [P]),
     "hm_csr_apply": (i32, [P, P, P, P]),
     "hm_vcycle_create": (i32, [P, P, i32, i32, i32, i32, f64, i32, i32, P, ctypes.POINTER(P)]),
+    "hm_vcycle_create_ex": (i32, [P, P, i32, i32, i32, i32, f64, i32, i32, i32, i32, P,
+                                  ctypes.POINTER(P)]),
     "hm_vcycle_create_dense": (i32, [P, i32, i32, P, ctypes.POINTER(P)]),
     "hm_vcycle_destroy": (i32, [P]),
     "hm_vcycle_levels": (i32, [P]),

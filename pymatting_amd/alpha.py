@@ -71,7 +71,7 @@ class LambdaSolver:
 
     def __init__(self, image, trimap, epsilon=1e-7, lambda_value=100.0, precision="f64",
                  num_pre_iter=1, num_post_iter=1, omega=0.8, direct_solve_size=64,
-                 level0="matrix_free"):
+                 level0="matrix_free", max_levels=0, coarse_sweeps=0):
         self.h, self.w = int(trimap.shape[0]), int(trimap.shape[1])
         self.precision = precision
         self.classes, self.tdev = device_classes(trimap)
@@ -82,7 +82,8 @@ class LambdaSolver:
         # and streamed (the stored ELL/DIA form).
         self.op = DiaOperator.from_cf(self.cf) if level0 == "stencil" else self.cf
         self.M = VCycle(self.op, (self.h, self.w), num_pre_iter, num_post_iter, omega,
-                        direct_solve_size, "f64" if precision == "f64" else "f32")
+                        direct_solve_size, "f64" if precision == "f64" else "f32",
+                        max_levels, coarse_sweeps)
         self.stats = {}
 
     def solve(self, rtol=1e-7, atol=0.0, maxiter=10000, inner_rtol=1e-4, max_outer=20,

@@ -69,6 +69,10 @@ struct hm_vc {
     double* dense_inv64 = nullptr;    // inverse of the coarsest matrix (n_c x n_c)
     float* dense_inv32 = nullptr;
     int n_direct = 0;
+    // > 0: the coarsest grid is solved approximately by this many damped
+    // Jacobi sweeps from zero (a fixed polynomial in D^-1 A, so the cycle
+    // stays a symmetric preconditioner) instead of the dense inverse.
+    int coarse_sweeps = 0;
     hm::Reducer red{};
 };
 

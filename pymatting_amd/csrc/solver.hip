@@ -254,6 +254,15 @@ __global__ __launch_bounds__(kBlock) void k_dense_apply(const T* __restrict__ in
     }
 }
 
+// First damped-Jacobi iterate from zero: x = omega b / D (vcycle.py:35-38).
+template <typename T>
+__global__ void k_jac0(const T* __restrict__ b, const T* __restrict__ D, T omega,
+                       T* __restrict__ x, long long n, const CgState* guard) {
+    if (solve_done(guard)) return;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = omega * b[i] / D[i];
+}
+
 // Mixed-precision helpers.
 __global__ void k_scale_to_f32(const double* __restrict__ a, double scale, float* __restrict__ b,
                                long long n) {
@@ -549,6 +558,7 @@ inline long long tail_bytes(const hm_vc* vc, size_t start, size_t es) {
 inline size_t vc_tail_start(const hm_vc* vc) {
     const size_t L = vc->levels.size();
     if (L <= 1) return L;
+    if (vc->coarse_sweeps > 0) return L - 1;  // no direct solve to end a tail with
     const size_t es = vc->prec == HM_F64 ? sizeof(double) : sizeof(float);
     size_t t = L - 1;
     while (t > 1 && L - (t - 1) <= (size_t)kTailMaxLevels &&
@@ -637,7 +647,30 @@ int vcycle_apply_t(hm_vc* vc, const T* r, T* z, CgState* st, const Reducer* red,
     }
     // Coarse tail in one launch (its top grid's correction lands in .x), or
     // the direct solve alone (vcycle.py:59-60).
-    if (tail < L - 1) {
+    if (vc->coarse_sweeps > 0) {
+        // Approximate coarsest solve: x = omega b / D, then damped Jacobi
+        // sweeps, ping-ponging so that the last one lands in .x.
+        const VcLevel& cl = vc->levels[L - 1];
+        const LevelOp<T> op = vc_level_op<T>(vc, L - 1);
+        const T* cb = vptr<T>(cl.b);
+        T* xs = vptr<T>(cl.x);
+        T* ts = vptr<T>(cl.t);
+        const int m = vc->coarse_sweeps - 1;  // sweeps after the first iterate
+        if (m == 0) {
+            k_jac0<T><<<div_up(cl.n, 256), 256, 0, s>>>(cb, op.D, omega, xs, cl.n, guard);
+            HM_LAUNCH_CHECK();
+        } else {
+            T* cur = (m % 2 == 1) ? xs : ts;
+            T* oth = (m % 2 == 1) ? ts : xs;
+            HM_TRY(run_level<T>(op, InJac0<T>{cb, op.D, omega},
+                                OutSmooth<T>{cb, op.D, omega, cur, nodot}, guard, s));
+            for (int k = 1; k < m; ++k) {
+                HM_TRY(run_level<T>(op, InVec<T>{cur}, OutSmooth<T>{cb, op.D, omega, oth, nodot},
+                                    guard, s));
+                std::swap(cur, oth);
+            }
+        }
+    } else if (tail < L - 1) {
         HM_TRY(launch_tail<T>(vc, tail, guard, s));
     } else {
         const VcLevel& cl = vc->levels[L - 1];
@@ -1117,7 +1150,7 @@ int dia_alloc(int h, int w, hm_dia** out) {
 }
 
 int vc_create(hm_cf* cf, hm_dia* dia, int h, int w, int pre, int post, double omega, int direct,
-              int prec, hipStream_t s, hm_vc** out) {
+              int prec, int max_levels, int coarse_sweeps, hipStream_t s, hm_vc** out) {
     if ((cf == nullptr) == (dia == nullptr)) {
         set_error("vcycle needs exactly one operator");
         return HM_ERR_ARG;
@@ -1132,6 +1165,10 @@ int vc_create(hm_cf* cf, hm_dia* dia, int h, int w, int pre, int post, double om
     }
     if (pre < 0 || post < 0) {
         set_error("num_pre_iter and num_post_iter must be non-negative");
+        return HM_ERR_ARG;
+    }
+    if (max_levels < 0 || coarse_sweeps < 0) {
+        set_error("max_levels and coarse_sweeps must be non-negative");
         return HM_ERR_ARG;
     }
     hm_vc* vc = new hm_vc();
@@ -1200,6 +1237,7 @@ int vc_create(hm_cf* cf, hm_dia* dia, int h, int w, int pre, int post, double om
             if (rc != HM_OK) break;
             fine = Lc.A;
             if (Lc.n <= direct) break;
+            if (max_levels > 0 && (int)vc->levels.size() >= max_levels) break;
         }
         if (tmp0) {
             HM_HIP(hipStreamSynchronize(s));
@@ -1209,6 +1247,21 @@ int vc_create(hm_cf* cf, hm_dia* dia, int h, int w, int pre, int post, double om
         if (rc != HM_OK) { vc_free(vc); return rc; }
     }
     const hm_dia* last = vc->levels.back().A;
+    if (vc->levels.size() > 1 && last->n > kMaxDirect) {
+        // Level cap reached above the dense-solve limit: approximate coarse solve.
+        if (coarse_sweeps == 0) {
+            set_error("the coarsest grid (%d x %d) exceeds the %d-point direct solve; give "
+                      "coarse_sweeps > 0 or allow more levels", last->h, last->w, kMaxDirect);
+            vc_free(vc);
+            return HM_ERR_ARG;
+        }
+        vc->coarse_sweeps = coarse_sweeps;
+        rc = make_reducer(&vc->red, reducer_blocks(n0, h, w), 2);
+        if (rc != HM_OK) { vc_free(vc); return rc; }
+        HM_HIP(hipStreamSynchronize(s));
+        *out = vc;
+        return HM_OK;
+    }
     vc->n_direct = (int)last->n;
     rc = dev_alloc(reinterpret_cast<void**>(&vc->dense_inv64), sizeof(double) * last->n * last->n);
     if (rc == HM_OK)
@@ -1469,7 +1522,18 @@ int hm_vcycle_create(hm_cf* cf, hm_dia* dia, int h, int w, int num_pre_iter, int
         return HM_ERR_ARG;
     }
     return vc_create(cf, dia, h, w, num_pre_iter, num_post_iter, omega, direct_solve_size, prec,
-                     static_cast<hipStream_t>(stream), out);
+                     0, 0, static_cast<hipStream_t>(stream), out);
+}
+
+int hm_vcycle_create_ex(hm_cf* cf, hm_dia* dia, int h, int w, int num_pre_iter,
+                        int num_post_iter, double omega, int direct_solve_size, int prec,
+                        int max_levels, int coarse_sweeps, void* stream, hm_vc** out) {
+    if (!out) {
+        set_error("hm_vcycle_create_ex: null output");
+        return HM_ERR_ARG;
+    }
+    return vc_create(cf, dia, h, w, num_pre_iter, num_post_iter, omega, direct_solve_size, prec,
+                     max_levels, coarse_sweeps, static_cast<hipStream_t>(stream), out);
 }
 
 int hm_vcycle_create_dense(const double* A_dense, int n, int prec, void* stream, hm_vc** out) {

@@ -54,10 +54,14 @@ class VCycle:
     Levels: (h, w) -> (h//2, w//2) -> ... until h*w <= direct_solve_size,
     Galerkin coarse operators R A P, damped Jacobi smoothing, and a dense
     direct solve on the coarsest grid.
+
+    Extensions beyond the reference: max_levels > 0 caps the number of grids;
+    a capped coarsest grid above 2048 points is then solved approximately by
+    coarse_sweeps damped Jacobi sweeps from zero.
     """
 
     def __init__(self, A, shape, num_pre_iter=1, num_post_iter=1, omega=0.8,
-                 direct_solve_size=64, precision="f64"):
+                 direct_solve_size=64, precision="f64", max_levels=0, coarse_sweeps=0):
         self.shape = (int(shape[0]), int(shape[1]))
         h, w = self.shape
         self.precision = precision
@@ -85,9 +89,11 @@ class VCycle:
             self._keep.append(op)
             cf = op.handle if isinstance(op, CfOperator) else None
             dia = op.handle if isinstance(op, DiaOperator) else None
-            _ffi.check(L.hm_vcycle_create(cf, dia, h, w, int(num_pre_iter), int(num_post_iter),
-                                          float(omega), int(direct_solve_size), prec,
-                                          _ffi.stream(), ctypes.byref(handle)))
+            _ffi.check(L.hm_vcycle_create_ex(cf, dia, h, w, int(num_pre_iter),
+                                             int(num_post_iter), float(omega),
+                                             int(direct_solve_size), prec, int(max_levels),
+                                             int(coarse_sweeps), _ffi.stream(),
+                                             ctypes.byref(handle)))
         self.handle = handle
         self.levels = L.hm_vcycle_levels(handle)
 

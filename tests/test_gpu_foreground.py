@@ -43,6 +43,20 @@ def test_float64(shape):
     assert np.abs(F - Fr).max() < 1e-12
 
 
+def test_float32_within_1e4_of_float64_at_1080p():
+    """The float32 form (the reference's typing) stays within 1e-4 of the
+    float64 form on a 1920x1080 synthetic composite (F, B in [0, 1])."""
+    import torch
+    import pymatting_amd as pm
+    from pymatting_amd import synthetic
+    image, _, alpha = synthetic.make_image(1080, 1920, 2000, strands=40, erode=10)
+    img, a = torch.from_numpy(image).cuda(), torch.from_numpy(alpha).cuda()
+    F32, B32 = pm.estimate_foreground_ml(img.float(), a.float(), return_background=True)
+    F64, B64 = pm.estimate_foreground_ml(img, a, return_background=True, precision="f64")
+    assert float((F32.double() - F64).abs().max()) <= 1e-4
+    assert float((B32.double() - B64).abs().max()) <= 1e-4
+
+
 def test_lemur_known_answer_and_options():
     import pymatting_amd as pm
     d = load_golden("lemur_fg_0p1")

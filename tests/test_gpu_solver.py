@@ -75,6 +75,35 @@ def test_vcycle_matches_oracle():
     assert np.abs(Md(r) - zr).max() <= 1e-8 * np.abs(zr).max()
 
 
+def test_vcycle_level_cap_and_coarse_sweeps():
+    """Extensions with no reference counterpart: max_levels and an approximate
+    coarsest solve. A cap at or above the natural depth changes nothing; a cap
+    that leaves the coarsest grid above the dense-solve limit needs
+    coarse_sweeps, and then gives a symmetric, linear preconditioner with
+    which PCG reaches the exact solution of the system."""
+    import scipy.sparse.linalg
+    import pymatting_amd as pm
+    from pymatting_amd import synthetic
+    from pymatting_amd.preconditioner import VCycle
+    image, trimap, _ = synthetic.make_image(128, 160, 7, strands=6, erode=6)
+    A, b = pm.make_linear_system(pm.cf_laplacian(image), trimap)
+    op = A.operator
+    rng = np.random.default_rng(3)
+    r = rng.standard_normal(b.size)
+    assert np.array_equal(VCycle(op, trimap.shape)(r), VCycle(op, trimap.shape, max_levels=32)(r))
+    with pytest.raises(ValueError, match="coarse_sweeps"):  # coarsest 64 x 80 = 5120 points
+        VCycle(op, trimap.shape, max_levels=2)
+    Ms = VCycle(op, trimap.shape, num_pre_iter=2, num_post_iter=2, max_levels=2, coarse_sweeps=3)
+    u, v = rng.standard_normal(b.size), rng.standard_normal(b.size)
+    Mu, Mv = Ms(u), Ms(v)
+    assert abs(u @ Mv - v @ Mu) <= 1e-9 * np.linalg.norm(u) * np.linalg.norm(Mv)
+    assert np.abs(Ms(2 * u + 3 * v) - (2 * Mu + 3 * Mv)).max() <= 1e-10 * np.abs(Mu).max()
+    x = pm.cg(A, b, M=Ms, rtol=1e-11)
+    Ar, br = ocf.make_linear_system(ocf.cf_laplacian(image), trimap)
+    xr = scipy.sparse.linalg.spsolve(Ar.tocsc(), br)
+    assert np.abs(x - xr).max() < 1e-6
+
+
 def test_iteration_ceilings_and_oracle_counts():
     import pymatting_amd as pm
     image, trimap = _lemur_system()
