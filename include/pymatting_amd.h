@@ -195,6 +195,15 @@ int hm_diag_apply(const double* inv_diag, const double* r, double* z, int64_t n,
 int hm_alpha_finalize(const double* x, const double* trimap, const uint8_t* cls, double* alpha,
                       int64_t n, void* stream);
 
+/* ---- adjoint of the alpha solve (no reference counterpart: pymatting is
+ * not differentiable; this is the image-gradient step of the adjoint method
+ * for the closed-form system of cf_laplacian.py:5-129 / laplacian.py:46-55) ----
+ * For x = A(I)^-1 b with A = L(I) + lambda diag(known) and lam = A^-1 g,
+ * writes dl/dI = -lam^T (dA/dI) x as a device float64 (h, w, 3) array (added
+ * to grad_hwc when accumulate != 0). lam and x are device float64 [n]. */
+int hm_cf_adjoint_image(hm_cf* op, const double* lam, const double* x, double* grad_hwc,
+                        int accumulate, void* stream);
+
 /* ---- estimate_foreground_ml: pymatting/foreground/estimate_foreground_ml.py:61-253 ----
  * image (h0,w0,3) and alpha (h0,w0) device float32 (HM_F32) or float64
  * (HM_F64); F, B outputs of the same type. level_h/level_w: n_levels+1 host

@@ -6,12 +6,14 @@ Drop-ins (same names and arguments as pymatting 1.1.15):
   jacobi, vcycle, estimate_alpha_cf (estimate_alpha), estimate_foreground_ml,
   trimap_split, sanity_check_image.
 Device-resident API: CfOperator, DiaOperator, LambdaSolver, solve_alpha_lambda.
+Differentiable solve (torch.autograd, hand-written adjoint): solve_alpha.
 
 All numerical work runs in hand-written HIP kernels of libpymatting_amd.so
 (C ABI: include/pymatting_amd.h). There is no CPU fallback.
 """
 from pymatting_amd.alpha import (LambdaSolver, estimate_alpha, estimate_alpha_cf,
                                  solve_alpha_lambda)
+from pymatting_amd.autograd import AlphaSolve, solve_alpha
 from pymatting_amd.foreground import estimate_foreground_ml
 from pymatting_amd.laplacian import (CfLaplacianMatrix, CfOperator, DiaOperator, cf_laplacian,
                                      make_linear_system)
@@ -23,5 +25,5 @@ __all__ = [
     "cf_laplacian", "make_linear_system", "cg", "CounterCallback", "ProgressCallback", "jacobi",
     "vcycle", "ichol", "estimate_alpha_cf", "estimate_alpha", "estimate_foreground_ml",
     "trimap_split", "sanity_check_image", "CfOperator", "DiaOperator", "CfLaplacianMatrix",
-    "LambdaSolver", "solve_alpha_lambda",
+    "LambdaSolver", "solve_alpha_lambda", "AlphaSolve", "solve_alpha",
 ]

@@ -74,6 +74,7 @@ _SIGS = {
     "hm_cf_apply": (i32, [P, i32, P, P, P]),
     "hm_cf_diagonal": (i32, [P, P, P]),
     "hm_cf_rhs": (i32, [P, P, P]),
+    "hm_cf_adjoint_image": (i32, [P, P, P, P, i32, P]),
     "hm_dia_create": (i32, [P, i32, i32, P, ctypes.POINTER(P)]),
     "hm_dia_from_cf": (i32, [P, P, ctypes.POINTER(P)]),
     "hm_dia_destroy": (i32, [P]),

@@ -86,12 +86,18 @@ class LambdaSolver:
                         max_levels, coarse_sweeps)
         self.stats = {}
 
-    def solve(self, rtol=1e-7, atol=0.0, maxiter=10000, inner_rtol=1e-4, max_outer=20,
-              check_every=8):
+    def solve_system(self, b, rtol=1e-7, atol=0.0, maxiter=10000, inner_rtol=1e-4,
+                     max_outer=20, check_every=8):
+        """x = A^-1 b for a device float64 right-hand side b (length h*w), with
+        this solver's operator, preconditioner and precision."""
         import ctypes
         n = self.h * self.w
+        b = b.reshape(-1)
+        if b.numel() != n or b.dtype != torch.float64 or not b.is_cuda:
+            raise ValueError("b must be a device float64 vector of length %d" % n)
+        b = b.contiguous()
         if self.precision == "f64":
-            x, st = cg(self.op, self.b, atol=atol, rtol=rtol, maxiter=maxiter, M=self.M,
+            x, st = cg(self.op, b, atol=atol, rtol=rtol, maxiter=maxiter, M=self.M,
                        check_every=check_every, return_stats=True)
             self.stats = st
         else:
@@ -100,13 +106,18 @@ class LambdaSolver:
             cf = self.op.handle if isinstance(self.op, CfOperator) else None
             dia = self.op.handle if isinstance(self.op, DiaOperator) else None
             _ffi.check(_ffi.lib().hm_cg_solve_mixed(
-                cf, dia, self.M.handle, _ffi.ptr(self.b), _ffi.ptr(x), float(rtol),
+                cf, dia, self.M.handle, _ffi.ptr(b), _ffi.ptr(x), float(rtol),
                 float(inner_rtol), int(max_outer), int(maxiter), ctypes.byref(stats),
                 _ffi.stream()))
             self.stats = {"iterations": stats.iterations, "norm_b": stats.norm_b,
                           "norm_r": stats.norm_r}
-        self.x = x
-        return _finalize(x, self.tdev, None, (self.h, self.w))
+        return x
+
+    def solve(self, rtol=1e-7, atol=0.0, maxiter=10000, inner_rtol=1e-4, max_outer=20,
+              check_every=8):
+        self.x = self.solve_system(self.b, rtol, atol, maxiter, inner_rtol, max_outer,
+                                   check_every)
+        return _finalize(self.x, self.tdev, None, (self.h, self.w))
 
 
 def solve_alpha_lambda(image, trimap, epsilon=1e-7, lambda_value=100.0, rtol=1e-7,

@@ -24,6 +24,7 @@ SOURCES = {
     "solver.hip": [],
     "fg.hip": ["-ffp-contract=off"],
     "scan.hip": [],
+    "adjoint.hip": [],
 }
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
