@@ -195,6 +195,28 @@ int hm_diag_apply(const double* inv_diag, const double* r, double* z, int64_t n,
 int hm_alpha_finalize(const double* x, const double* trimap, const uint8_t* cls, double* alpha,
                       int64_t n, void* stream);
 
+/* ---- batched solve (no reference counterpart: pymatting solves one image
+ * per call; this is cg.py:4-117 with the V-cycle of vcycle.py:46-100 run on
+ * many independent systems at once) ----
+ * ops[i]: the stored level-0 operator of image i; vcs[i]: its V-cycle, built
+ * on ops[i] (float64 for precision HM_F64; float32 for the mixed form,
+ * precision HM_F32), all with the same sweeps and omega and a direct
+ * coarsest solve. level0_shape / coarse_shape: tile shape of the finest and
+ * of the Galerkin grids (0: 64x8 points, two per thread; 1: 32x8, one per
+ * thread). The batch keeps its own stream; hm_batch_solve orders it after
+ * and before `stream`. Each image iterates until its own residual test
+ * passes (per-image convergence masks); graph_iters > 0 records that many
+ * iterations (rounded down to even) into a HIP graph that is replayed;
+ * 0 launches them directly. rhs/x: host arrays of count device float64
+ * vectors; stats: count entries. */
+typedef struct hm_batch hm_batch;
+int hm_batch_create(int count, hm_dia* const* ops, hm_vc* const* vcs, int precision,
+                    int level0_shape, int coarse_shape, hm_batch** out);
+int hm_batch_solve(hm_batch* b, const double* const* rhs, double* const* x, double rtol,
+                   double inner_rtol, int max_outer, int maxiter, int graph_iters,
+                   hm_cg_stats* stats, void* stream);
+int hm_batch_destroy(hm_batch* b);
+
 /* ---- adjoint of the alpha solve (no reference counterpart: pymatting is
  * not differentiable; this is the image-gradient step of the adjoint method
  * for the closed-form system of cf_laplacian.py:5-129 / laplacian.py:46-55) ----

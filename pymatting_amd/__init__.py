@@ -7,6 +7,7 @@ Drop-ins (same names and arguments as pymatting 1.1.15):
   trimap_split, sanity_check_image.
 Device-resident API: CfOperator, DiaOperator, LambdaSolver, solve_alpha_lambda.
 Differentiable solve (torch.autograd, hand-written adjoint): solve_alpha.
+Batched solves of many images: BatchSolver.
 
 All numerical work runs in hand-written HIP kernels of libpymatting_amd.so
 (C ABI: include/pymatting_amd.h). There is no CPU fallback.
@@ -14,6 +15,7 @@ All numerical work runs in hand-written HIP kernels of libpymatting_amd.so
 from pymatting_amd.alpha import (LambdaSolver, estimate_alpha, estimate_alpha_cf,
                                  solve_alpha_lambda)
 from pymatting_amd.autograd import AlphaSolve, solve_alpha
+from pymatting_amd.batch import BatchSolver
 from pymatting_amd.foreground import estimate_foreground_ml
 from pymatting_amd.laplacian import (CfLaplacianMatrix, CfOperator, DiaOperator, cf_laplacian,
                                      make_linear_system)
@@ -25,5 +27,5 @@ __all__ = [
     "cf_laplacian", "make_linear_system", "cg", "CounterCallback", "ProgressCallback", "jacobi",
     "vcycle", "ichol", "estimate_alpha_cf", "estimate_alpha", "estimate_foreground_ml",
     "trimap_split", "sanity_check_image", "CfOperator", "DiaOperator", "CfLaplacianMatrix",
-    "LambdaSolver", "solve_alpha_lambda", "AlphaSolve", "solve_alpha",
+    "LambdaSolver", "solve_alpha_lambda", "AlphaSolve", "solve_alpha", "BatchSolver",
 ]

@@ -25,6 +25,7 @@ SOURCES = {
     "fg.hip": ["-ffp-contract=off"],
     "scan.hip": [],
     "adjoint.hip": [],
+    "batch.hip": [],
 }
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

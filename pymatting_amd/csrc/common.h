@@ -109,13 +109,18 @@ __device__ __forceinline__ void block_sum(double (&v)[N]) {
 // visible through the kernel boundary without a device-scope fence per block
 // (on gfx950 such a fence writes back the XCD's L2).
 template <int BS, int N>
-__device__ __forceinline__ void block_partials(double (&v)[N], const Reducer& red) {
-    const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+__device__ __forceinline__ void block_partials_at(double (&v)[N], const Reducer& red,
+                                                  unsigned slot) {
     block_sum<BS, N>(v);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < N; ++k) red.partials[k * red.max_blocks + bid] = v[k];
+        for (int k = 0; k < N; ++k) red.partials[k * red.max_blocks + slot] = v[k];
     }
+}
+
+template <int BS, int N>
+__device__ __forceinline__ void block_partials(double (&v)[N], const Reducer& red) {
+    block_partials_at<BS, N>(v, red, blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
 }
 
 __device__ __forceinline__ bool solve_done(const CgState* st) {

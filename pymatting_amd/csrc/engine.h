@@ -559,12 +559,21 @@ struct TileMap {
     }
 };
 
+// XCD-banded tile of workgroup b in a launch of ntiles tiles (see above).
+__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned ntiles) {
+    const unsigned q = ntiles / kXcd, r = ntiles % kXcd;
+    const unsigned j = b % kXcd, k = b / kXcd;
+    return j * q + (j < r ? j : r) + k;
+}
+
+// One workgroup's tile (bx, by) of an (h, w) grid: stage the input tile and
+// its 2-point halo in LDS, apply the operator at the tile's points and hand
+// each result to the output; a reduction writes its partial at `slot`.
+// `done` (the solve's freeze flag) is tested after the loads are issued, so
+// its latency overlaps theirs instead of adding to it.
 template <typename T, class Op, class In, class Out>
-__global__ __launch_bounds__(kBlock) void tile_kernel(Op op, In in, Out out, int h, int w,
-                                                      const CgState* guard, TileMap map) {
-    // The guard flag is read first and tested after the tile's loads have been
-    // issued, so its latency overlaps theirs instead of adding to it.
-    const bool done = solve_done(guard);
+__device__ __forceinline__ void tile_body(const Op& op, const In& in, const Out& out, int h,
+                                          int w, bool done, int bx, int by, unsigned slot) {
     constexpr int OX = Out::OX, OY = Out::OY;
     constexpr int RX = OX + 4, RY = OY + 4;
     constexpr int NO = (OX * OY + kBlock - 1) / kBlock;
@@ -572,8 +581,6 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(Op op, In in, Out out, int
     __shared__ T sx[RY][RX];
     __shared__ typename Op::template Shared<OX, OY> sh;
     __shared__ T sres[IsRestrict<Out>::value ? OX * OY : 1];
-    int bx, by;
-    map.tile(blockIdx.x, bx, by);
     const int ox0 = out.ox0(bx), oy0 = out.oy0(by);
     typename Op::template Prefetch<OX, OY> pf;
     op.template prefetch<OX, OY>(pf, ox0, oy0);
@@ -635,8 +642,17 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(Op op, In in, Out out, int
     }
     if (o.dot.st != nullptr) {
         double v[1] = {acc};
-        block_partials<kBlock, 1>(v, o.dot.red);
+        block_partials_at<kBlock, 1>(v, o.dot.red, slot);
     }
+}
+
+template <typename T, class Op, class In, class Out>
+__global__ __launch_bounds__(kBlock) void tile_kernel(Op op, In in, Out out, int h, int w,
+                                                      const CgState* guard, TileMap map) {
+    const bool done = solve_done(guard);
+    int bx, by;
+    map.tile(blockIdx.x, bx, by);
+    tile_body<T, Op, In, Out>(op, in, out, h, w, done, bx, by, blockIdx.x);
 }
 
 // Stage two of every reduction: one 1024-thread workgroup sums nblocks
@@ -644,30 +660,11 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(Op op, In in, Out out, int
 // state. All loads of a thread are issued before the first add.
 constexpr int kFinBlock = 1024;
 
+// Fixed-order sums of cnt[k] partials at src[k] for k < N, valid in thread 0.
 template <int N>
-__global__ __launch_bounds__(kFinBlock) void k_finalize(Reducer red, unsigned nblocks,
-                                                        CgState* st, int kind,
-                                                        const CgState* guard) {
+__device__ __forceinline__ void finalize_sums(const double* const (&src)[N],
+                                              const unsigned (&cnt)[N], double (&acc)[N]) {
     constexpr int U = 4;
-    const bool done = solve_done(guard);
-    const double* src[N];
-    unsigned cnt[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-        src[k] = red.partials + (size_t)k * red.max_blocks;
-        cnt[k] = nblocks;
-    }
-    if (kind == kDotUpdateRz) {
-        // The residual norm of the CG update (slot 2) and <r, z> of the
-        // preconditioner that follows it (slot 0), finalised together: the
-        // convergence test first, then beta only for an unconverged solve.
-        src[0] = red.partials + 2 * (size_t)red.max_blocks;
-        cnt[0] = red.upd_blocks;
-        if (N > 1) {
-            src[N - 1] = red.partials;
-            cnt[N - 1] = nblocks;
-        }
-    }
     unsigned most = 0;
 #pragma unroll
     for (int k = 0; k < N; ++k) most = cnt[k] > most ? cnt[k] : most;
@@ -685,7 +682,6 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(Reducer red, unsigned nb
                 if (i < cnt[k]) part[k][u] += src[k][i];
         }
     }
-    double acc[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) {
         double t = 0.0;
@@ -693,15 +689,47 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(Reducer red, unsigned nb
         for (int u = 0; u < U; ++u) t += part[k][u];
         acc[k] = t;
     }
-    if (done) return;  // uniform across the workgroup
     block_sum<kFinBlock, N>(acc);
-    if (threadIdx.x != 0) return;
+}
+
+// Applies finished sums to the solve state (thread 0 only). kDotUpdateRz: the
+// residual norm of the CG update (acc[0]) and <r, z> of the preconditioner
+// that follows it (acc[N-1]): the convergence test first, then beta only for
+// an unconverged solve.
+template <int N>
+__device__ __forceinline__ void finalize_apply(CgState* st, int kind, const double (&acc)[N]) {
     if (kind == kDotUpdateRz) {
         dot_finalize(st, kDotUpdate, acc[0]);
         if (!st->done) dot_finalize(st, kDotRzUpdate, acc[N - 1]);
     } else {
         dot_finalize(st, kind, acc[0], N > 1 ? acc[N - 1] : 0.0);
     }
+}
+
+template <int N>
+__global__ __launch_bounds__(kFinBlock) void k_finalize(Reducer red, unsigned nblocks,
+                                                        CgState* st, int kind,
+                                                        const CgState* guard) {
+    const bool done = solve_done(guard);
+    const double* src[N];
+    unsigned cnt[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        src[k] = red.partials + (size_t)k * red.max_blocks;
+        cnt[k] = nblocks;
+    }
+    if (kind == kDotUpdateRz) {  // slot 2: the update's partials; slot 0: <r, z>
+        src[0] = red.partials + 2 * (size_t)red.max_blocks;
+        cnt[0] = red.upd_blocks;
+        if (N > 1) {
+            src[N - 1] = red.partials;
+            cnt[N - 1] = nblocks;
+        }
+    }
+    if (done) return;  // uniform across the workgroup
+    double acc[N];
+    finalize_sums<N>(src, cnt, acc);
+    if (threadIdx.x == 0) finalize_apply<N>(st, kind, acc);
 }
 
 template <class Out>

@@ -17,6 +17,7 @@
 #include "common.h"
 #include "engine.h"
 #include "internal.h"
+#include "vcycle.h"
 
 namespace {
 thread_local std::string g_last_error;
@@ -33,236 +34,6 @@ void set_error(const char* fmt, ...) {
     g_last_error = buf;
 }
 
-// ------------------------------------------------------------- launch helpers
-
-template <typename T, class Op, class In, class Out>
-int launch_tile(const Op& op, const In& in, const Out& out, int h, int w, int h2, int w2,
-                const CgState* guard, hipStream_t s, bool coarse = false) {
-    const TileMap map = tile_map(out, h, w, h2, w2);
-    if (out.dot.st != nullptr && (long long)map.ntiles > out.dot.red.max_blocks) {
-        set_error("reduction buffer too small for %u blocks", map.ntiles);
-        return HM_ERR_ARG;
-    }
-    ProfScope prof((coarse ? (int)kProfCoarseApply : Op::kProfBase) + Out::kProf, (long long)h * w,
-                   sizeof(T) == 4, s);
-    tile_kernel<T, Op, In, Out><<<map.ntiles, kBlock, 0, s>>>(op, in, out, h, w, guard, map);
-    HM_LAUNCH_CHECK();
-    if (out.dot.st != nullptr)
-        return launch_finalize(1, out.dot.red, map.ntiles, out.dot.st, out.dot.kind, guard, s);
-    return HM_OK;
-}
-
-template <typename T>
-struct LevelOp {
-    bool is_cf = false;
-    bool coarse = false;  // a Galerkin level (profiling class only)
-    CfOp<T> cf{};
-    DiaOp<T> dia{};
-    const T* D = nullptr;
-    int h = 0, w = 0;
-};
-
-template <typename T>
-const T* pick(const double* a, const float* b);
-template <>
-const double* pick<double>(const double* a, const float*) { return a; }
-template <>
-const float* pick<float>(const double*, const float* b) { return b; }
-
-template <typename T>
-LevelOp<T> cf_level(const hm_cf* op, int force_mode = -1) {
-    LevelOp<T> L;
-    L.is_cf = true;
-    L.cf.d = CfDev{op->h, op->w, op->mode, op->skip_known, op->eps, op->lambda, op->cls};
-    L.cf.I = pick<T>(op->I64, op->I32);
-    L.cf.W = pick<T>(op->wc, op->wc32);
-    L.cf.force_mode = force_mode;
-    L.D = pick<T>(op->diag64, op->diag32);
-    L.h = op->h;
-    L.w = op->w;
-    return L;
-}
-
-template <typename T>
-LevelOp<T> dia_level(const hm_dia* A) {
-    LevelOp<T> L;
-    L.is_cf = false;
-    L.dia = DiaOp<T>{A->h, A->w, pick<T>(A->c64, A->c32)};
-    L.D = pick<T>(A->diag64, A->diag32);
-    L.h = A->h;
-    L.w = A->w;
-    return L;
-}
-
-template <typename T, class In, class Out>
-int run_level(const LevelOp<T>& L, const In& in, const Out& out, const CgState* guard,
-              hipStream_t s) {
-    if (L.is_cf)
-        return launch_tile<T>(L.cf, in, out, L.h, L.w, L.h / 2, L.w / 2, guard, s);
-    // Coarse grids below kSmallGrid points are latency-bound: one point per
-    // thread with its coefficients loaded before the tile barrier. Larger
-    // grids keep the bandwidth-oriented shape of the finest level.
-    constexpr long long kSmallGrid = 1 << 18;
-    if (L.coarse && (long long)L.h * L.w < kSmallGrid) {
-        const DiaOp<T, true> op{L.dia.h, L.dia.w, L.dia.coef};
-        return launch_tile<T>(op, in, out.small(), L.h, L.w, L.h / 2, L.w / 2, guard, s, true);
-    }
-    return launch_tile<T>(L.dia, in, out, L.h, L.w, L.h / 2, L.w / 2, guard, s, L.coarse);
-}
-
-// ------------------------------------------------------------- vector kernels
-
-constexpr int kVecPerThread = 4;
-
-inline unsigned vec_blocks(long long n) { return div_up(n, (long long)kBlock * kVecPerThread); }
-
-// x += alpha p; r -= alpha q; ||r||^2 and the convergence test (cg.py:90-100).
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_update(T* __restrict__ x, const T* __restrict__ p,
-                                                      T* __restrict__ r, const T* __restrict__ q,
-                                                      long long n, CgState* st, Reducer red) {
-    if (solve_done(st)) return;
-    const T alpha = (T)st->alpha;
-    double acc = 0.0;
-    const long long base = (long long)blockIdx.x * kBlock * kVecPerThread + threadIdx.x;
-#pragma unroll
-    for (int k = 0; k < kVecPerThread; ++k) {
-        const long long i = base + (long long)k * kBlock;
-        if (i < n) {
-            x[i] += alpha * p[i];
-            const T ri = r[i] - alpha * q[i];
-            r[i] = ri;
-            acc += (double)ri * (double)ri;
-        }
-    }
-    double v[1] = {acc};
-    block_partials<kBlock, 1>(v, red);
-}
-
-// p = z + beta p (cg.py:112-113).
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_pupdate(T* __restrict__ p, const T* __restrict__ z,
-                                                       long long n, const CgState* st) {
-    if (solve_done(st)) return;
-    const T beta = (T)st->beta;
-    const long long base = (long long)blockIdx.x * kBlock * kVecPerThread + threadIdx.x;
-#pragma unroll
-    for (int k = 0; k < kVecPerThread; ++k) {
-        const long long i = base + (long long)k * kBlock;
-        if (i < n) p[i] = p[i] * beta + z[i];
-    }
-}
-
-// z = r * inv_diag (jacobi.py:24-31), or z = r without a preconditioner; <r, z>.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_diag_pc(const T* __restrict__ r,
-                                                    const double* __restrict__ inv_diag,
-                                                    T* __restrict__ z, long long n, CgState* st,
-                                                    Reducer red, int kind) {
-    if (solve_done(st)) return;
-    double acc = 0.0;
-    const long long base = (long long)blockIdx.x * kBlock * kVecPerThread + threadIdx.x;
-#pragma unroll
-    for (int k = 0; k < kVecPerThread; ++k) {
-        const long long i = base + (long long)k * kBlock;
-        if (i < n) {
-            const T zi = inv_diag ? (T)((double)r[i] * inv_diag[i]) : r[i];
-            z[i] = zi;
-            acc += (double)r[i] * (double)zi;
-        }
-    }
-    if (st != nullptr) {
-        double v[1] = {acc};
-        block_partials<kBlock, 1>(v, red);
-    }
-}
-
-// <a, b> (and <a - c, b> when c is given, for the Polak-Ribiere beta of
-// cg.py:104-106) into the state.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_dot(const T* __restrict__ a, const T* __restrict__ b,
-                                                const T* __restrict__ c, long long n, CgState* st,
-                                                Reducer red, int kind) {
-    if (kind != kDotAux && solve_done(st)) return;
-    double acc[2] = {0.0, 0.0};
-    const long long base = (long long)blockIdx.x * kBlock * kVecPerThread + threadIdx.x;
-#pragma unroll
-    for (int k = 0; k < kVecPerThread; ++k) {
-        const long long i = base + (long long)k * kBlock;
-        if (i < n) {
-            acc[0] += (double)a[i] * (double)b[i];
-            if (c) acc[1] += ((double)a[i] - (double)c[i]) * (double)b[i];
-        }
-    }
-    block_partials<kBlock, 2>(acc, red);
-}
-
-// <a, b> (or the Polak-Ribiere pair when c is given) into the solve state.
-template <typename T>
-int launch_dot(const T* a, const T* b, const T* c, long long n, CgState* st, const Reducer& red,
-               int kind, hipStream_t s) {
-    const unsigned g = vec_blocks(n);
-    k_dot<T><<<g, kBlock, 0, s>>>(a, b, c, n, st, red, kind);
-    HM_LAUNCH_CHECK();
-    return launch_finalize(c ? 2 : 1, red, g, st, c ? kDotReorth : kind,
-                           kind == kDotAux ? nullptr : st, s);
-}
-
-template <typename T>
-__global__ void k_copy(const T* __restrict__ a, T* __restrict__ b, long long n,
-                       const CgState* st) {
-    if (solve_done(st)) return;
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) b[i] = a[i];
-}
-
-template <typename T>
-__global__ void k_fill(T* __restrict__ a, T v, long long n) {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = v;
-}
-
-// out = base + P e with base = omega b / D (pre = 1), a stored vector, or 0.
-template <typename T, class Base>
-__global__ void k_prolong_add(Base base, InProlong<T, Base> pro, T* __restrict__ out, int h,
-                              int w, const CgState* st) {
-    if (solve_done(st)) return;
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long long)h * w) return;
-    const int y = (int)(i / w), x = (int)(i - (long long)y * w);
-    out[i] = pro.load(x, y, i);
-}
-
-// x = inv(A_c) b on the coarsest grid (n <= 1024), optional <b, x>.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_dense_apply(const T* __restrict__ inv,
-                                                        const T* __restrict__ b,
-                                                        T* __restrict__ x, int n, CgState* st,
-                                                        Reducer red, const CgState* guard) {
-    if (solve_done(guard)) return;
-    double acc = 0.0;
-    for (int i = threadIdx.x + blockIdx.x * kBlock; i < n; i += kBlock * gridDim.x) {
-        T s = 0;
-#pragma unroll 8
-        for (int j = 0; j < n; ++j) s += inv[(long long)i * n + j] * b[j];
-        x[i] = s;
-        acc += (double)b[i] * (double)s;
-    }
-    if (st != nullptr) {  // a single workgroup: one partial, finalised by the host's next launch
-        double v[1] = {acc};
-        block_partials<kBlock, 1>(v, red);
-    }
-}
-
-// First damped-Jacobi iterate from zero: x = omega b / D (vcycle.py:35-38).
-template <typename T>
-__global__ void k_jac0(const T* __restrict__ b, const T* __restrict__ D, T omega,
-                       T* __restrict__ x, long long n, const CgState* guard) {
-    if (solve_done(guard)) return;
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = omega * b[i] / D[i];
-}
-
 // Mixed-precision helpers.
 __global__ void k_scale_to_f32(const double* __restrict__ a, double scale, float* __restrict__ b,
                                long long n) {
@@ -274,331 +45,6 @@ __global__ void k_axpy_f32_to_f64(double* __restrict__ x, double scale,
                                   const float* __restrict__ e, long long n) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) x[i] += scale * (double)e[i];
-}
-
-// ------------------------------------------------------------------ V-cycle
-
-template <typename T>
-T* vptr(void* p) { return static_cast<T*>(p); }
-
-template <typename T>
-LevelOp<T> vc_level_op(const hm_vc* vc, size_t l) {
-    if (l == 0 && vc->cf) return cf_level<T>(vc->cf);
-    LevelOp<T> L = dia_level<T>(vc->levels[l].A);
-    L.coarse = l > 0;
-    return L;
-}
-
-template <typename T>
-const T* vc_inv(const hm_vc* vc) { return pick<T>(vc->dense_inv64, vc->dense_inv32); }
-
-template <typename T, class Base>
-int post_stage(const LevelOp<T>& op, const Base& base, const T* b, const T* e, int h2, int w2,
-               int post, T omega, T* target, T* other, DotSink fin, const T* r_top,
-               const CgState* guard, hipStream_t s, const Reducer& red) {
-    DotSink nodot{nullptr, Reducer{}, 0};
-    InProlong<T, Base> pro{base, e, w2, h2};
-    if (post == 0) {
-        k_prolong_add<T, Base><<<div_up((long long)op.h * op.w, 256), 256, 0, s>>>(
-            base, pro, target, op.h, op.w, guard);
-        HM_LAUNCH_CHECK();
-        if (fin.st != nullptr) {
-            const long long n = (long long)op.h * op.w;
-            HM_TRY(launch_dot<T>(r_top, target, nullptr, n, fin.st, red, fin.kind, s));
-        }
-        return HM_OK;
-    }
-    const T* prev = nullptr;
-    for (int k = 0; k < post; ++k) {
-        T* outp = ((post - 1 - k) % 2 == 0) ? target : other;
-        const DotSink ds = (k == post - 1) ? fin : nodot;
-        OutSmooth<T> sm{b, op.D, omega, outp, ds};
-        if (k == 0)
-            HM_TRY(run_level<T>(op, pro, sm, guard, s));
-        else
-            HM_TRY(run_level<T>(op, InVec<T>{prev}, sm, guard, s));
-        prev = outp;
-    }
-    return HM_OK;
-}
-
-// ------------------------------------------------------------- V-cycle tail
-//
-// The coarsest grids are processed by one 1024-thread workgroup in a single
-// launch with everything staged in LDS: the tail's stored stencils and
-// right-hand side are loaded once, pre-smoothing, residual, restriction, the
-// dense direct solve and the way back up run out of LDS between workgroup
-// barriers, and only the top tail grid's correction is written back. On
-// these grids each separate launch would cost far more than its arithmetic.
-
-constexpr int kTailMaxLevels = 24;
-constexpr int kTailBlock = 1024;
-constexpr int kTailLdsBytes = 152 * 1024;
-
-template <typename T>
-struct TailLevel {
-    int h, w;
-    const T* coef;  // [13][n] symmetric storage (global)
-    T* b;           // right-hand side (global; only the top level's is read)
-    T* x;           // result (global; only the top level's is written)
-    int off;        // offset of this level's LDS block (in T units)
-};
-
-template <typename T>
-struct TailParams {
-    TailLevel<T> lv[kTailMaxLevels];
-    int nlev;        // tail grids including the direct one (last)
-    int pre, post;
-    T omega;
-    const T* inv;    // dense inverse of the last grid (global)
-};
-
-// LDS block of one level: coef [13][n], b, x, t, u (each [n]).
-template <typename T>
-struct TailView {
-    int h, w, n;
-    T* coef;
-    T* b;
-    T* x;
-    T* t;
-    T* u;
-};
-
-template <typename T>
-__device__ __forceinline__ TailView<T> tail_view(T* lds, const TailLevel<T>& L) {
-    TailView<T> v;
-    v.h = L.h;
-    v.w = L.w;
-    v.n = L.h * L.w;
-    v.coef = lds + L.off;
-    v.b = v.coef + kSymSlots * v.n;
-    v.x = v.b + v.n;
-    v.t = v.x + v.n;
-    v.u = v.t + v.n;
-    return v;
-}
-
-template <typename T>
-__device__ __forceinline__ T tail_Ax(const TailView<T>& V, const T* v, int i) {
-    const int y = i / V.w, x = i - y * V.w;
-    T acc = 0;
-#pragma unroll
-    for (int s = 0; s < 25; ++s) {
-        const int yy = y + s / 5 - 2, xx = x + s % 5 - 2;
-        if (yy >= 0 && yy < V.h && xx >= 0 && xx < V.w)
-            acc += sym_coef(V.coef, (long long)V.n, V.w, (long long)i, s) * v[yy * V.w + xx];
-    }
-    return acc;
-}
-
-template <typename T>
-__device__ __forceinline__ T tail_prolong(const T* e, int h2, int w2, int gx, int gy) {
-    T pe = 0;
-    for (int ey = -1; ey <= 1; ++ey) {
-        if (((gy - ey) & 1) != 0) continue;
-        const int Y = (gy - ey) >> 1;
-        if (Y < 0 || Y >= h2) continue;
-        for (int ex = -1; ex <= 1; ++ex) {
-            if (((gx - ex) & 1) != 0) continue;
-            const int X = (gx - ex) >> 1;
-            if (X < 0 || X >= w2) continue;
-            pe += (ey == 0 ? T(0.5) : T(0.25)) * (ex == 0 ? T(0.5) : T(0.25)) * e[Y * w2 + X];
-        }
-    }
-    return pe;
-}
-
-template <typename T>
-__global__ __launch_bounds__(kTailBlock) void k_vc_tail(TailParams<T> P, const CgState* guard) {
-    if (solve_done(guard)) return;
-    extern __shared__ __align__(16) unsigned char tail_lds_raw[];
-    T* lds = reinterpret_cast<T*>(tail_lds_raw);
-    const int tid = threadIdx.x;
-    // Stage every level's stencil and the top level's right-hand side. Each
-    // thread issues kStage loads before its first LDS store, so the copy costs
-    // a few memory latencies rather than one per loop trip.
-    {
-        constexpr int kStage = 8;
-        for (int l = 0; l <= P.nlev; ++l) {
-            const TailView<T> V = tail_view(lds, P.lv[l < P.nlev ? l : 0]);
-            const T* src = l < P.nlev ? P.lv[l].coef : P.lv[0].b;
-            T* dst = l < P.nlev ? V.coef : V.b;
-            const int cnt = l < P.nlev ? kSymSlots * V.n : V.n;
-            for (int i0 = 0; i0 < cnt; i0 += kTailBlock * kStage) {
-                T v[kStage];
-#pragma unroll
-                for (int u = 0; u < kStage; ++u) {
-                    const int i = i0 + u * kTailBlock + tid;
-                    v[u] = i < cnt ? src[i] : T(0);
-                }
-#pragma unroll
-                for (int u = 0; u < kStage; ++u) {
-                    const int i = i0 + u * kTailBlock + tid;
-                    if (i < cnt) dst[i] = v[u];
-                }
-            }
-        }
-    }
-    __syncthreads();
-    // The pre-smoothed iterate of every level ends in .x or .t depending on
-    // the number of sweeps (one buffer swap per sweep after the first).
-    const bool pre_in_t = P.pre >= 2 && ((P.pre - 1) & 1);
-    // Descend.
-    for (int l = 0; l + 1 < P.nlev; ++l) {
-        const TailView<T> V = tail_view(lds, P.lv[l]);
-        T* cur = V.x;
-        T* nxt = V.t;
-        if (P.pre == 0) {
-            for (int i = tid; i < V.n; i += kTailBlock) cur[i] = T(0);
-        } else {
-            for (int i = tid; i < V.n; i += kTailBlock) cur[i] = P.omega * V.b[i] / V.coef[i];
-            for (int k = 1; k < P.pre; ++k) {
-                __syncthreads();
-                for (int i = tid; i < V.n; i += kTailBlock)
-                    nxt[i] = cur[i] + P.omega * (V.b[i] - tail_Ax(V, cur, i)) / V.coef[i];
-                T* sw = cur; cur = nxt; nxt = sw;
-            }
-        }
-        __syncthreads();
-        for (int i = tid; i < V.n; i += kTailBlock) V.u[i] = V.b[i] - tail_Ax(V, cur, i);
-        __syncthreads();
-        const TailView<T> C = tail_view(lds, P.lv[l + 1]);
-        for (int I = tid; I < C.n; I += kTailBlock) {
-            const int Y = I / C.w, X = I - Y * C.w;
-            T acc = 0;
-            for (int ey = -1; ey <= 1; ++ey) {
-                const int y = 2 * Y + ey;
-                if (y < 0 || y >= V.h) continue;
-                for (int ex = -1; ex <= 1; ++ex) {
-                    const int x = 2 * X + ex;
-                    if (x < 0 || x >= V.w) continue;
-                    acc += (ey == 0 ? T(0.5) : T(0.25)) * (ex == 0 ? T(0.5) : T(0.25)) *
-                           V.u[y * V.w + x];
-                }
-            }
-            C.b[I] = acc;
-        }
-        __syncthreads();
-    }
-    // Direct solve on the last grid: x = inv(A) b with the inverse staged in
-    // the scratch vectors of the top tail level when it fits.
-    {
-        const TailView<T> C = tail_view(lds, P.lv[P.nlev - 1]);
-        const TailView<T> top = tail_view(lds, P.lv[0]);
-        const int nn = C.n * C.n;
-        const bool staged = nn <= 2 * top.n;
-        T* inv = top.t;  // t and u of the top level are free at this point
-        if (staged) {
-            constexpr int kStage = 4;
-            for (int i0 = 0; i0 < nn; i0 += kTailBlock * kStage) {
-                T v[kStage];
-#pragma unroll
-                for (int u = 0; u < kStage; ++u) {
-                    const int i = i0 + u * kTailBlock + tid;
-                    v[u] = i < nn ? P.inv[i] : T(0);
-                }
-#pragma unroll
-                for (int u = 0; u < kStage; ++u) {
-                    const int i = i0 + u * kTailBlock + tid;
-                    if (i < nn) inv[i] = v[u];
-                }
-            }
-            __syncthreads();
-        }
-        for (int i = tid; i < C.n; i += kTailBlock) {
-            T acc = 0;
-            if (staged) {
-                for (int j = 0; j < C.n; ++j) acc += inv[i * C.n + j] * C.b[j];
-            } else {
-#pragma unroll 8
-                for (int j = 0; j < C.n; ++j) acc += P.inv[(long long)i * C.n + j] * C.b[j];
-            }
-            C.x[i] = acc;
-        }
-        __syncthreads();
-    }
-    // Ascend.
-    const T* e = tail_view(lds, P.lv[P.nlev - 1]).x;
-    for (int l = P.nlev - 2; l >= 0; --l) {
-        const TailView<T> V = tail_view(lds, P.lv[l]);
-        const TailView<T> C = tail_view(lds, P.lv[l + 1]);
-        const T* xp = pre_in_t ? V.t : V.x;
-        T* a = pre_in_t ? V.x : V.t;  // must not overwrite the pre-smoothed iterate
-        for (int i = tid; i < V.n; i += kTailBlock) {
-            const int y = i / V.w, x = i - y * V.w;
-            a[i] = xp[i] + tail_prolong(e, C.h, C.w, x, y);
-        }
-        T* cur = a;
-        T* nxt = V.u;
-        for (int k = 0; k < P.post; ++k) {
-            __syncthreads();
-            for (int i = tid; i < V.n; i += kTailBlock)
-                nxt[i] = cur[i] + P.omega * (V.b[i] - tail_Ax(V, cur, i)) / V.coef[i];
-            T* sw = cur; cur = nxt; nxt = sw;
-        }
-        __syncthreads();
-        e = cur;
-    }
-    // The top tail grid's correction goes back to global memory.
-    {
-        const int n0 = P.lv[0].h * P.lv[0].w;
-        for (int i = tid; i < n0; i += kTailBlock) P.lv[0].x[i] = e[i];
-    }
-}
-
-// LDS footprint of tail levels [start, L): 17 values per point.
-inline long long tail_bytes(const hm_vc* vc, size_t start, size_t es) {
-    long long b = 0;
-    for (size_t l = start; l < vc->levels.size(); ++l) b += (long long)(kSymSlots + 4) * vc->levels[l].n * es;
-    return b;
-}
-
-// First level of the tail: the largest suffix of grids (never level 0) that
-// fits the LDS budget, or L - 1 if even the two coarsest grids do not.
-inline size_t vc_tail_start(const hm_vc* vc) {
-    const size_t L = vc->levels.size();
-    if (L <= 1) return L;
-    if (vc->coarse_sweeps > 0) return L - 1;  // no direct solve to end a tail with
-    const size_t es = vc->prec == HM_F64 ? sizeof(double) : sizeof(float);
-    size_t t = L - 1;
-    while (t > 1 && L - (t - 1) <= (size_t)kTailMaxLevels &&
-           tail_bytes(vc, t - 1, es) <= kTailLdsBytes)
-        --t;
-    return t;
-}
-
-template <typename T>
-int launch_tail(hm_vc* vc, size_t start, const CgState* guard, hipStream_t s) {
-    TailParams<T> P{};
-    const size_t L = vc->levels.size();
-    P.nlev = (int)(L - start);
-    P.pre = vc->pre;
-    P.post = vc->post;
-    P.omega = (T)vc->omega;
-    P.inv = pick<T>(vc->dense_inv64, vc->dense_inv32);
-    int off = 0;
-    for (size_t l = start; l < L; ++l) {
-        const VcLevel& V = vc->levels[l];
-        TailLevel<T>& t = P.lv[l - start];
-        t.h = V.h;
-        t.w = V.w;
-        t.coef = pick<T>(V.A->c64, V.A->c32);
-        t.b = static_cast<T*>(V.b);
-        t.x = static_cast<T*>(V.x);
-        t.off = off;
-        off += (int)((kSymSlots + 4) * V.n);
-    }
-    const size_t lds = (size_t)off * sizeof(T);
-    static bool attr_set[2] = {false, false};
-    if (!attr_set[sizeof(T) == 4]) {
-        HM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vc_tail<T>),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes));
-        attr_set[sizeof(T) == 4] = true;
-    }
-    ProfScope prof(kProfDense, vc->levels[start].n, sizeof(T) == 4, s);
-    k_vc_tail<T><<<1, kTailBlock, lds, s>>>(P, guard);
-    HM_LAUNCH_CHECK();
-    return HM_OK;
 }
 
 // z = V-cycle(r) (vcycle.py:46-100). If st is given, the final kernel also
