@@ -154,6 +154,92 @@ def cpu_baseline():
                       "vcycle-PCG rtol 1e-7, float64; numpy/scipy CSR), %.1f s" % r["t"]}
 
 
+def bench_c3(args):
+    """64 images, heights and widths uniform in 256..1024 (step 16), unknown
+    bands 5-40 px: all alpha solves (set-up + PCG + clip, rtol 1e-7 float64 and
+    the mixed form at rtol 1e-8) batched, against a loop of one-image solves
+    with the same settings. Tile shapes and iterations per graph come from
+    BatchSolver.autotune (run once, outside the timed steps)."""
+    import torch
+    import pymatting_amd as pm
+    from pymatting_amd import synthetic
+
+    data = synthetic.config3()
+    images = [torch.from_numpy(d[0]).cuda() for d in data]
+    trimaps = [torch.from_numpy(d[1]).cuda() for d in data]
+    npix = sum(int(t.numel()) for t in trimaps)
+    forms = {"f64": dict(precision="f64", rtol=1e-7),
+             "mixed": dict(precision="mixed", rtol=1e-8, inner_rtol=1e-3)}
+
+    def batched(form, tuned=None):
+        cfg = dict(forms[form])
+        bs = pm.BatchSolver(images, trimaps, precision=cfg.pop("precision"))
+        if tuned is not None:
+            bs.level0_shape, bs.coarse_shape, bs.graph_iters = tuned
+        alphas = bs.solve(**cfg)
+        return alphas, bs
+
+    def sequential(form):
+        cfg = dict(forms[form])
+        prec = cfg.pop("precision")
+        out, its = [], []
+        for im, tr in zip(images, trimaps):
+            s = pm.LambdaSolver(im, tr, precision=prec, level0="stencil")
+            out.append(s.solve(**cfg))
+            its.append(s.stats["iterations"])
+        return out, its
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            r = fn()
+        torch.cuda.synchronize()
+        return r, (time.perf_counter() - t0) / args.steps
+
+    ref = [pm.LambdaSolver(im, tr, precision="f64", level0="stencil").solve(rtol=1e-11,
+                                                                        maxiter=100000)
+           for im, tr in zip(images, trimaps)]
+    res = {}
+    for form in ("f64", "mixed"):
+        tuner = pm.BatchSolver(images, trimaps, precision=forms[form]["precision"])
+        cfg = {k: v for k, v in forms[form].items() if k != "precision"}
+        choice = tuner.autotune(**cfg)
+        tuning = getattr(tuner, "tuning", None)
+        del tuner
+        (alphas, bs), tb = timed(lambda: batched(form, choice))
+        (alphas_s, its_s), ts = timed(lambda: sequential(form))
+        err = max(float((a - r).abs().max()) for a, r in zip(alphas, ref))
+        err_s = max(float((a - r).abs().max()) for a, r in zip(alphas_s, ref))
+        its = [st["iterations"] for st in bs.stats]
+        res[form] = {"batched_Mpix_per_s": round(npix / tb / 1e6, 3),
+                     "batched_ms": round(tb * 1e3, 3),
+                     "sequential_Mpix_per_s": round(npix / ts / 1e6, 3),
+                     "sequential_ms": round(ts * 1e3, 3),
+                     "speedup": round(ts / tb, 3),
+                     "iterations_min_max": [min(its), max(its)],
+                     "sequential_iterations_min_max": [min(its_s), max(its_s)],
+                     "max_abs_dalpha_vs_converged": err,
+                     "sequential_max_abs_dalpha": err_s,
+                     "autotuned": {"level0_shape": choice[0], "coarse_shape": choice[1],
+                                   "graph_iters": choice[2], "timings_s": tuning}}
+    line = {
+        "metric": "batched alpha Mpix/s, 64 images 256..1024 px (mixed form)",
+        "value": res["mixed"]["batched_Mpix_per_s"], "unit": "Mpix/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["mixed"]["batched_ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "64 lambda-system alpha solves (eps 1e-7, lambda 100, V(1,1)) "
+                               "of images with h, w uniform in 256..1024 step 16, bands 5-40 "
+                               "px; set-up + batched PCG + clip", "images": len(images),
+                   "total_pixels": npix},
+        "f64": res["f64"], "mixed": res["mixed"],
+    }
+    print(json.dumps(line))
+
+
 def bench_c4(args):
     """3840x2160 (unknown band 40 px): mixed form with a 5-level V-cycle, two
     pre- and two post-smoothing float32 Jacobi sweeps and an approximate
@@ -323,9 +409,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pmc-probe", action="store_true")
-    ap.add_argument("--workload", choices=["c2", "c4", "c5"], default="c2",
-                    help="c2: 1920x1080 solve (the metric); c4: 3840x2160 5-level V(2,2) "
-                         "solve and float32 foreground; c5: gradient of a 16x512x512 batch")
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c2",
+                    help="c2: 1920x1080 solve (the metric); c3: 64 images of 256..1024 px, "
+                         "batched vs one at a time; c4: 3840x2160 5-level V(2,2) solve and "
+                         "float32 foreground; c5: gradient of a 16x512x512 batch")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: sharding over several GPUs is a design, not "
@@ -333,6 +420,9 @@ def main():
         sys.exit(2)
     if args.pmc_probe:
         pmc_probe()
+        return
+    if args.workload == "c3":
+        bench_c3(args)
         return
     if args.workload == "c4":
         bench_c4(args)
