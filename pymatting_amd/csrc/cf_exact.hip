@@ -173,22 +173,23 @@ __device__ __forceinline__ double entry_exact(const double* __restrict__ I,
     return value;
 }
 
-// Assembles 25 slots per pixel. Output element (i, s) lives at
-// values[i * stride_i + s * stride_s]. mode: HM_OP_* adjustments of the
+// Assembles slots s_lo..24 per pixel. Output element (i, s) lives at
+// values[i * stride_i + (s - s_lo) * stride_s] (s_lo = 12: the 13 symmetric
+// slots of the stored stencil form). mode: HM_OP_* adjustments of the
 // stored operator (lambda on the diagonal of known rows, or the
 // hard-constraint masking). indices/indptr (nullable) follow the reference's
 // CSR layout.
 __global__ void k_assemble_exact(const double* __restrict__ I, const double* __restrict__ wc,
                                  int h, int w, const uint8_t* __restrict__ cls, int mode,
                                  double lambda, double* __restrict__ values,
-                                 long long stride_i, long long stride_s,
+                                 long long stride_i, long long stride_s, int s_lo,
                                  int64_t* __restrict__ indices, int64_t* __restrict__ indptr) {
     const long long n = (long long)h * w;
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int yi = (int)(i / w), xi = (int)(i - (long long)yi * w);
     const bool known_i = cls && cls[i] != 0;
-    for (int s = 0; s < 25; ++s) {
+    for (int s = s_lo; s < 25; ++s) {
         const int yj = yi + s / 5 - 2, xj = xi + s % 5 - 2;
         double v = 0.0;
         int64_t col = 0;
@@ -204,7 +205,7 @@ __global__ void k_assemble_exact(const double* __restrict__ I, const double* __r
                 if (mode == HM_OP_LAMBDA && s == 12 && known_i) v = v + lambda;
             }
         }
-        values[i * stride_i + s * stride_s] = v;
+        values[i * stride_i + (s - s_lo) * stride_s] = v;
         if (indices) indices[i * 25 + s] = col;
     }
     if (indptr) {
@@ -348,11 +349,15 @@ int cf_upload_image(const double* hwc, hm_cf* op, hipStream_t s) {
     return HM_OK;
 }
 
-int cf_assemble(hm_cf* op, double* values, long long stride_i, long long stride_s,
+int cf_assemble(hm_cf* op, double* values, long long stride_i, long long stride_s, int s_lo,
                 int64_t* indices, int64_t* indptr, int mode, hipStream_t s) {
+    if (s_lo != 0 && (indices || indptr)) {
+        set_error("internal: CSR indices need all 25 slots");
+        return HM_ERR_ARG;
+    }
     k_assemble_exact<<<div_up(op->n, 256), 256, 0, s>>>(op->I64, op->wc, op->h, op->w, op->cls,
                                                          mode, op->lambda, values, stride_i,
-                                                         stride_s, indices, indptr);
+                                                         stride_s, s_lo, indices, indptr);
     HM_LAUNCH_CHECK();
     return HM_OK;
 }

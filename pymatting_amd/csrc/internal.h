@@ -81,7 +81,7 @@ namespace hm {
 // cf_exact.hip
 int cf_upload_image(const double* hwc, hm_cf* op, hipStream_t s);
 int cf_exact_setup(hm_cf* op, hipStream_t s);
-int cf_assemble(hm_cf* op, double* values, long long stride_i, long long stride_s,
+int cf_assemble(hm_cf* op, double* values, long long stride_i, long long stride_s, int s_lo,
                 int64_t* indices, int64_t* indptr, int mode, hipStream_t s);
 int cf_rhs(hm_cf* op, double* b, hipStream_t s);
 int cf_system_counts(hm_cf* op, int64_t* counts, hipStream_t s);

@@ -811,7 +811,7 @@ int hm_cf_assemble_csr(hm_cf* op, double* values, int64_t* indices, int64_t* ind
         set_error("hm_cf_assemble_csr: null argument");
         return HM_ERR_ARG;
     }
-    return cf_assemble(op, values, 25, 1, indices, indptr, op->mode,
+    return cf_assemble(op, values, 25, 1, 0, indices, indptr, op->mode,
                        static_cast<hipStream_t>(stream));
 }
 
@@ -871,12 +871,8 @@ int hm_dia_from_cf(hm_cf* op, void* stream, hm_dia** out) {
     hipStream_t s = static_cast<hipStream_t>(stream);
     hm_dia* A = nullptr;
     HM_TRY(dia_alloc(op->h, op->w, &A));
-    void* full = nullptr;
-    int rc = dev_alloc(&full, sizeof(double) * 25 * A->n);
-    if (rc == HM_OK)
-        rc = cf_assemble(op, static_cast<double*>(full), 1, A->n, nullptr, nullptr, op->mode, s);
-    if (rc == HM_OK) rc = dia_pack(static_cast<double*>(full), A, s);
-    dev_free(full);
+    // The 13 stored slots (dy, dx) >= (0, 0), assembled exactly in place.
+    int rc = cf_assemble(op, A->c64, 1, A->n, 12, nullptr, nullptr, op->mode, s);
     if (rc == HM_OK) rc = dia_finish(A, s);
     if (rc != HM_OK) {
         hm_dia_destroy(A);

@@ -290,15 +290,20 @@ int post_stage(const LevelOp<T>& op, const Base& base, const T* b, const T* e, i
 // ------------------------------------------------------------- V-cycle tail
 //
 // The coarsest grids are processed by one 1024-thread workgroup in a single
-// launch with everything staged in LDS: the tail's stored stencils and
-// right-hand side are loaded once, pre-smoothing, residual, restriction, the
-// dense direct solve and the way back up run out of LDS between workgroup
-// barriers, and only the top tail grid's correction is written back. On
-// these grids each separate launch would cost far more than its arithmetic.
+// launch. Every vector of every tail grid lives in LDS with a zero halo of two
+// points, so the 25-point stencil, the restriction and the prolongation run
+// without bounds tests (out-of-grid neighbours read zeros, which is the
+// reference's dropped taps). The stencils of all but possibly the top tail
+// grid are staged in LDS too; the top grid's may stay in global memory (L2)
+// when they do not fit, which lets a larger grid join the tail. Only the top
+// grid's right-hand side is read from and its correction written to global
+// memory. On these grids each separate launch would cost far more than its
+// arithmetic.
 
 constexpr int kTailMaxLevels = 24;
 constexpr int kTailBlock = 1024;
 constexpr int kTailLdsBytes = 152 * 1024;
+constexpr int kTailMaxTop = 4096;  // largest top grid of a tail (points)
 
 template <typename T>
 struct TailLevel {
@@ -307,6 +312,7 @@ struct TailLevel {
     T* b;           // right-hand side (global; only the top level's is read)
     T* x;           // result (global; only the top level's is written)
     int off;        // offset of this level's LDS block (in T units)
+    int coef_lds;   // 1: coefficients staged in LDS after the vectors
 };
 
 template <typename T>
@@ -316,18 +322,24 @@ struct TailParams {
     int pre, post;
     T omega;
     const T* inv;    // dense inverse of the last grid (global)
+    int inv_off;     // LDS offset of the staged inverse, or -1 (read from global)
 };
 
-// LDS block of one level: coef [13][n], b, x, t, u (each [n]).
+// LDS block of one level: padded vectors b, x, t, u of (h + 4) x (w + 4),
+// element (y, x) at [(y + 2) * pitch + x + 2], then [13][n] coefficients if
+// staged.
 template <typename T>
 struct TailView {
-    int h, w, n;
-    T* coef;
+    int h, w, n, pitch, psize;
+    const T* coef;
     T* b;
     T* x;
     T* t;
     T* u;
+    __device__ __forceinline__ int at(int y, int x_) const { return (y + 2) * pitch + x_ + 2; }
 };
+
+__host__ __device__ inline int tail_psize(int h, int w) { return (h + 4) * (w + 4); }
 
 template <typename T>
 __device__ __forceinline__ TailView<T> tail_view(T* lds, const TailLevel<T>& L) {
@@ -335,42 +347,39 @@ __device__ __forceinline__ TailView<T> tail_view(T* lds, const TailLevel<T>& L) 
     v.h = L.h;
     v.w = L.w;
     v.n = L.h * L.w;
-    v.coef = lds + L.off;
-    v.b = v.coef + kSymSlots * v.n;
-    v.x = v.b + v.n;
-    v.t = v.x + v.n;
-    v.u = v.t + v.n;
+    v.pitch = L.w + 4;
+    v.psize = (L.h + 4) * v.pitch;
+    v.b = lds + L.off;
+    v.x = v.b + v.psize;
+    v.t = v.x + v.psize;
+    v.u = v.t + v.psize;
+    v.coef = L.coef_lds ? static_cast<const T*>(v.u + v.psize) : L.coef;
     return v;
 }
 
+// (A v) at (y, x): 25 taps, no bounds tests (zero halo).
 template <typename T>
-__device__ __forceinline__ T tail_Ax(const TailView<T>& V, const T* v, int i) {
-    const int y = i / V.w, x = i - y * V.w;
+__device__ __forceinline__ T tail_Ax(const TailView<T>& V, const T* v, int y, int x) {
+    const long long i = (long long)y * V.w + x;
+    const T* c = v + y * V.pitch + x;  // top-left tap of the 5 x 5 neighbourhood
     T acc = 0;
 #pragma unroll
-    for (int s = 0; s < 25; ++s) {
-        const int yy = y + s / 5 - 2, xx = x + s % 5 - 2;
-        if (yy >= 0 && yy < V.h && xx >= 0 && xx < V.w)
-            acc += sym_coef(V.coef, (long long)V.n, V.w, (long long)i, s) * v[yy * V.w + xx];
-    }
+    for (int s = 0; s < 25; ++s)
+        acc += sym_coef(V.coef, (long long)V.n, V.w, i, s) * c[(s / 5) * V.pitch + s % 5];
     return acc;
 }
 
+// P e at fine point (y, x) from the padded coarse vector e (vcycle.py:6-29):
+// weights 1/2 at coincident rows/columns, 1/4 at the two neighbours.
 template <typename T>
-__device__ __forceinline__ T tail_prolong(const T* e, int h2, int w2, int gx, int gy) {
-    T pe = 0;
-    for (int ey = -1; ey <= 1; ++ey) {
-        if (((gy - ey) & 1) != 0) continue;
-        const int Y = (gy - ey) >> 1;
-        if (Y < 0 || Y >= h2) continue;
-        for (int ex = -1; ex <= 1; ++ex) {
-            if (((gx - ex) & 1) != 0) continue;
-            const int X = (gx - ex) >> 1;
-            if (X < 0 || X >= w2) continue;
-            pe += (ey == 0 ? T(0.5) : T(0.25)) * (ex == 0 ? T(0.5) : T(0.25)) * e[Y * w2 + X];
-        }
-    }
-    return pe;
+__device__ __forceinline__ T tail_prolong(const TailView<T>& C, const T* e, int y, int x) {
+    const int y0 = y >> 1, x0 = x >> 1;
+    const bool oy = y & 1, ox = x & 1;
+    const T* r0 = e + C.at(y0, x0);
+    if (!oy && !ox) return T(0.25) * r0[0];
+    if (!oy) return T(0.125) * (r0[0] + r0[1]);
+    if (!ox) return T(0.125) * (r0[0] + r0[C.pitch]);
+    return T(0.0625) * ((r0[0] + r0[1]) + (r0[C.pitch] + r0[C.pitch + 1]));
 }
 
 template <typename T>
@@ -378,16 +387,27 @@ __device__ __forceinline__ void vc_tail_body(const TailParams<T>& P) {
     extern __shared__ __align__(16) unsigned char tail_lds_raw[];
     T* lds = reinterpret_cast<T*>(tail_lds_raw);
     const int tid = threadIdx.x;
-    // Stage every level's stencil and the top level's right-hand side. Each
-    // thread issues kStage loads before its first LDS store, so the copy costs
-    // a few memory latencies rather than one per loop trip.
+    const int nlev = P.nlev;
+    // Zero every vector (the halos must read as zero), then stage the top
+    // right-hand side, the LDS stencils and the inverse. kStage loads per
+    // thread are issued before the stores.
+    {
+        const TailLevel<T>& last = P.lv[nlev - 1];
+        const int end = last.off + 4 * tail_psize(last.h, last.w) +
+                        (last.coef_lds ? kSymSlots * last.h * last.w : 0);
+        for (int i = tid; i < end; i += kTailBlock) lds[i] = T(0);
+    }
+    __syncthreads();
     {
         constexpr int kStage = 8;
-        for (int l = 0; l <= P.nlev; ++l) {
-            const TailView<T> V = tail_view(lds, P.lv[l < P.nlev ? l : 0]);
-            const T* src = l < P.nlev ? P.lv[l].coef : P.lv[0].b;
-            T* dst = l < P.nlev ? V.coef : V.b;
-            const int cnt = l < P.nlev ? kSymSlots * V.n : V.n;
+        for (int l = 0; l <= nlev; ++l) {
+            const bool rhs = l == nlev;
+            const TailLevel<T>& L = P.lv[rhs ? 0 : l];
+            if (!rhs && !L.coef_lds) continue;
+            const TailView<T> V = tail_view(lds, L);
+            const T* src = rhs ? L.b : L.coef;
+            const int cnt = rhs ? V.n : kSymSlots * V.n;
+            T* dst = rhs ? V.b : const_cast<T*>(V.coef);
             for (int i0 = 0; i0 < cnt; i0 += kTailBlock * kStage) {
                 T v[kStage];
 #pragma unroll
@@ -398,115 +418,121 @@ __device__ __forceinline__ void vc_tail_body(const TailParams<T>& P) {
 #pragma unroll
                 for (int u = 0; u < kStage; ++u) {
                     const int i = i0 + u * kTailBlock + tid;
-                    if (i < cnt) dst[i] = v[u];
+                    if (i >= cnt) continue;
+                    if (rhs) {
+                        const int y = i / V.w;
+                        dst[V.at(y, i - y * V.w)] = v[u];
+                    } else {
+                        dst[i] = v[u];
+                    }
                 }
             }
+        }
+        if (P.inv_off >= 0) {
+            const TailLevel<T>& L = P.lv[nlev - 1];
+            const int nn = L.h * L.w * L.h * L.w;
+            for (int i = tid; i < nn; i += kTailBlock) lds[P.inv_off + i] = P.inv[i];
         }
     }
     __syncthreads();
     // The pre-smoothed iterate of every level ends in .x or .t depending on
     // the number of sweeps (one buffer swap per sweep after the first).
     const bool pre_in_t = P.pre >= 2 && ((P.pre - 1) & 1);
-    // Descend.
-    for (int l = 0; l + 1 < P.nlev; ++l) {
+    const T omega = P.omega;
+    // Descend: pre-smooth, residual, restrict.
+    for (int l = 0; l + 1 < nlev; ++l) {
         const TailView<T> V = tail_view(lds, P.lv[l]);
         T* cur = V.x;
         T* nxt = V.t;
-        if (P.pre == 0) {
-            for (int i = tid; i < V.n; i += kTailBlock) cur[i] = T(0);
-        } else {
-            for (int i = tid; i < V.n; i += kTailBlock) cur[i] = P.omega * V.b[i] / V.coef[i];
+        if (P.pre > 0) {
+            for (int i = tid; i < V.n; i += kTailBlock) {
+                const int y = i / V.w, x = i - y * V.w;
+                cur[V.at(y, x)] = omega * V.b[V.at(y, x)] / V.coef[i];
+            }
             for (int k = 1; k < P.pre; ++k) {
                 __syncthreads();
-                for (int i = tid; i < V.n; i += kTailBlock)
-                    nxt[i] = cur[i] + P.omega * (V.b[i] - tail_Ax(V, cur, i)) / V.coef[i];
-                T* sw = cur; cur = nxt; nxt = sw;
+                for (int i = tid; i < V.n; i += kTailBlock) {
+                    const int y = i / V.w, x = i - y * V.w;
+                    const int p = V.at(y, x);
+                    nxt[p] = cur[p] + omega * (V.b[p] - tail_Ax(V, cur, y, x)) / V.coef[i];
+                }
+                T* sw = cur;
+                cur = nxt;
+                nxt = sw;
             }
         }
         __syncthreads();
-        for (int i = tid; i < V.n; i += kTailBlock) V.u[i] = V.b[i] - tail_Ax(V, cur, i);
+        for (int i = tid; i < V.n; i += kTailBlock) {
+            const int y = i / V.w, x = i - y * V.w;
+            const int p = V.at(y, x);
+            V.u[p] = V.b[p] - tail_Ax(V, cur, y, x);
+        }
         __syncthreads();
         const TailView<T> C = tail_view(lds, P.lv[l + 1]);
         for (int I = tid; I < C.n; I += kTailBlock) {
             const int Y = I / C.w, X = I - Y * C.w;
-            T acc = 0;
-            for (int ey = -1; ey <= 1; ++ey) {
-                const int y = 2 * Y + ey;
-                if (y < 0 || y >= V.h) continue;
-                for (int ex = -1; ex <= 1; ++ex) {
-                    const int x = 2 * X + ex;
-                    if (x < 0 || x >= V.w) continue;
-                    acc += (ey == 0 ? T(0.5) : T(0.25)) * (ex == 0 ? T(0.5) : T(0.25)) *
-                           V.u[y * V.w + x];
-                }
-            }
-            C.b[I] = acc;
+            const T* r0 = V.u + V.at(2 * Y - 1, 2 * X - 1);
+            const T* r1 = r0 + V.pitch;
+            const T* r2 = r1 + V.pitch;
+            const T s = (r0[0] + T(2) * r0[1] + r0[2]) + T(2) * (r1[0] + T(2) * r1[1] + r1[2]) +
+                        (r2[0] + T(2) * r2[1] + r2[2]);
+            C.b[C.at(Y, X)] = s * T(0.0625);
         }
         __syncthreads();
     }
-    // Direct solve on the last grid: x = inv(A) b with the inverse staged in
-    // the scratch vectors of the top tail level when it fits.
+    // Direct solve on the last grid: x = inv(A) b.
     {
-        const TailView<T> C = tail_view(lds, P.lv[P.nlev - 1]);
-        const TailView<T> top = tail_view(lds, P.lv[0]);
-        const int nn = C.n * C.n;
-        const bool staged = nn <= 2 * top.n;
-        T* inv = top.t;  // t and u of the top level are free at this point
-        if (staged) {
-            constexpr int kStage = 4;
-            for (int i0 = 0; i0 < nn; i0 += kTailBlock * kStage) {
-                T v[kStage];
-#pragma unroll
-                for (int u = 0; u < kStage; ++u) {
-                    const int i = i0 + u * kTailBlock + tid;
-                    v[u] = i < nn ? P.inv[i] : T(0);
-                }
-#pragma unroll
-                for (int u = 0; u < kStage; ++u) {
-                    const int i = i0 + u * kTailBlock + tid;
-                    if (i < nn) inv[i] = v[u];
-                }
-            }
-            __syncthreads();
+        const TailView<T> C = tail_view(lds, P.lv[nlev - 1]);
+        T* bc = C.u;  // contiguous copy of the right-hand side
+        for (int j = tid; j < C.n; j += kTailBlock) {
+            const int y = j / C.w;
+            bc[j] = C.b[C.at(y, j - y * C.w)];
         }
+        __syncthreads();
+        const T* inv = P.inv_off >= 0 ? lds + P.inv_off : P.inv;
         for (int i = tid; i < C.n; i += kTailBlock) {
             T acc = 0;
-            if (staged) {
-                for (int j = 0; j < C.n; ++j) acc += inv[i * C.n + j] * C.b[j];
-            } else {
-#pragma unroll 8
-                for (int j = 0; j < C.n; ++j) acc += P.inv[(long long)i * C.n + j] * C.b[j];
-            }
-            C.x[i] = acc;
+            for (int j = 0; j < C.n; ++j) acc += inv[(long long)i * C.n + j] * bc[j];
+            const int y = i / C.w;
+            C.x[C.at(y, i - y * C.w)] = acc;
         }
         __syncthreads();
     }
-    // Ascend.
-    const T* e = tail_view(lds, P.lv[P.nlev - 1]).x;
-    for (int l = P.nlev - 2; l >= 0; --l) {
+    // Ascend: prolongate the correction and post-smooth.
+    const T* e = tail_view(lds, P.lv[nlev - 1]).x;
+    for (int l = nlev - 2; l >= 0; --l) {
         const TailView<T> V = tail_view(lds, P.lv[l]);
         const TailView<T> C = tail_view(lds, P.lv[l + 1]);
         const T* xp = pre_in_t ? V.t : V.x;
         T* a = pre_in_t ? V.x : V.t;  // must not overwrite the pre-smoothed iterate
         for (int i = tid; i < V.n; i += kTailBlock) {
             const int y = i / V.w, x = i - y * V.w;
-            a[i] = xp[i] + tail_prolong(e, C.h, C.w, x, y);
+            const int p = V.at(y, x);
+            a[p] = xp[p] + tail_prolong(C, e, y, x);
         }
         T* cur = a;
         T* nxt = V.u;
         for (int k = 0; k < P.post; ++k) {
             __syncthreads();
-            for (int i = tid; i < V.n; i += kTailBlock)
-                nxt[i] = cur[i] + P.omega * (V.b[i] - tail_Ax(V, cur, i)) / V.coef[i];
-            T* sw = cur; cur = nxt; nxt = sw;
+            for (int i = tid; i < V.n; i += kTailBlock) {
+                const int y = i / V.w, x = i - y * V.w;
+                const int p = V.at(y, x);
+                nxt[p] = cur[p] + omega * (V.b[p] - tail_Ax(V, cur, y, x)) / V.coef[i];
+            }
+            T* sw = cur;
+            cur = nxt;
+            nxt = sw;
         }
         __syncthreads();
         e = cur;
     }
     // The top tail grid's correction goes back to global memory.
     {
-        const int n0 = P.lv[0].h * P.lv[0].w;
-        for (int i = tid; i < n0; i += kTailBlock) P.lv[0].x[i] = e[i];
+        const TailView<T> V = tail_view(lds, P.lv[0]);
+        for (int i = tid; i < V.n; i += kTailBlock) {
+            const int y = i / V.w;
+            P.lv[0].x[i] = e[V.at(y, i - y * V.w)];
+        }
     }
 }
 
@@ -516,11 +542,39 @@ __global__ __launch_bounds__(kTailBlock) void k_vc_tail(TailParams<T> P, const C
     vc_tail_body<T>(P);
 }
 
-// LDS footprint of tail levels [start, L): 17 values per point.
-inline long long tail_bytes(const hm_vc* vc, size_t start, size_t es) {
-    long long b = 0;
-    for (size_t l = start; l < vc->levels.size(); ++l) b += (long long)(kSymSlots + 4) * vc->levels[l].n * es;
-    return b;
+// LDS elements of a tail over levels [start, L) with the top level's stencil
+// staged or not, including the staged inverse when it fits.
+inline long long tail_elems(const hm_vc* vc, size_t start, bool top_coef, bool* inv_staged) {
+    long long e = 0;
+    const size_t L = vc->levels.size();
+    for (size_t l = start; l < L; ++l) {
+        const VcLevel& V = vc->levels[l];
+        e += 4LL * tail_psize(V.h, V.w);
+        if (l > start || top_coef) e += (long long)kSymSlots * V.n;
+    }
+    const long long nn = (long long)vc->levels[L - 1].n * vc->levels[L - 1].n;
+    const size_t es = vc->prec == HM_F64 ? sizeof(double) : sizeof(float);
+    *inv_staged = (e + nn) * (long long)es <= kTailLdsBytes;
+    if (*inv_staged) e += nn;
+    return e;
+}
+
+inline bool tail_fits(const hm_vc* vc, size_t start, bool* top_coef, bool* inv_staged) {
+    const size_t es = vc->prec == HM_F64 ? sizeof(double) : sizeof(float);
+    if (vc->levels.size() - start > (size_t)kTailMaxLevels) return false;
+    if (vc->levels[start].n > kTailMaxTop) return false;
+    bool inv = false;
+    if (tail_elems(vc, start, true, &inv) * (long long)es <= kTailLdsBytes) {
+        *top_coef = true;
+        *inv_staged = inv;
+        return true;
+    }
+    if (tail_elems(vc, start, false, &inv) * (long long)es <= kTailLdsBytes) {
+        *top_coef = false;
+        *inv_staged = inv;
+        return true;
+    }
+    return false;
 }
 
 // First level of the tail: the largest suffix of grids (never level 0) that
@@ -529,11 +583,9 @@ inline size_t vc_tail_start(const hm_vc* vc) {
     const size_t L = vc->levels.size();
     if (L <= 1) return L;
     if (vc->coarse_sweeps > 0) return L - 1;  // no direct solve to end a tail with
-    const size_t es = vc->prec == HM_F64 ? sizeof(double) : sizeof(float);
     size_t t = L - 1;
-    while (t > 1 && L - (t - 1) <= (size_t)kTailMaxLevels &&
-           tail_bytes(vc, t - 1, es) <= kTailLdsBytes)
-        --t;
+    bool tc, is;
+    while (t > 1 && tail_fits(vc, t - 1, &tc, &is)) --t;
     return t;
 }
 
@@ -547,6 +599,11 @@ size_t make_tail_params(const hm_vc* vc, size_t start, TailParams<T>& P) {
     P.post = vc->post;
     P.omega = (T)vc->omega;
     P.inv = pick<T>(vc->dense_inv64, vc->dense_inv32);
+    bool top_coef = true, inv_staged = false;
+    if (!tail_fits(vc, start, &top_coef, &inv_staged)) {
+        top_coef = true;
+        inv_staged = false;
+    }
     int off = 0;
     for (size_t l = start; l < L; ++l) {
         const VcLevel& V = vc->levels[l];
@@ -557,8 +614,11 @@ size_t make_tail_params(const hm_vc* vc, size_t start, TailParams<T>& P) {
         t.b = static_cast<T*>(V.b);
         t.x = static_cast<T*>(V.x);
         t.off = off;
-        off += (int)((kSymSlots + 4) * V.n);
+        t.coef_lds = (l > start || top_coef) ? 1 : 0;
+        off += 4 * tail_psize(V.h, V.w) + (t.coef_lds ? kSymSlots * (int)V.n : 0);
     }
+    P.inv_off = inv_staged ? off : -1;
+    if (inv_staged) off += (int)(vc->levels[L - 1].n * vc->levels[L - 1].n);
     return (size_t)off * sizeof(T);
 }
 
