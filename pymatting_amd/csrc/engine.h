@@ -342,29 +342,29 @@ struct InProlong {
     const T* e;  // coarse vector (h2 x w2)
     int w2, h2;
     __device__ __forceinline__ T load(int gx, int gy, long long g) const {
-        T v = base.load(gx, gy, g);
-        int jx[2], jy[2];
-        T wx[2], wy[2];
-        int nx = 0, ny = 0;
-        if ((gx & 1) == 0) {
-            jx[0] = gx >> 1; wx[0] = T(0.5); nx = 1;
-        } else {
-            jx[0] = (gx - 1) >> 1; wx[0] = T(0.25);
-            jx[1] = (gx + 1) >> 1; wx[1] = T(0.25); nx = 2;
-        }
-        if ((gy & 1) == 0) {
-            jy[0] = gy >> 1; wy[0] = T(0.5); ny = 1;
-        } else {
-            jy[0] = (gy - 1) >> 1; wy[0] = T(0.25);
-            jy[1] = (gy + 1) >> 1; wy[1] = T(0.25); ny = 2;
-        }
+        const T v = base.load(gx, gy, g);
+        // Fine (gx, gy) takes 1/2 of the coincident coarse point per axis when
+        // even, 1/4 of each of the two neighbours when odd (x0 = gx >> 1
+        // floors, so the odd case's taps are x0 and x0 + 1); taps outside the
+        // coarse grid are dropped.
+        const int x0 = gx >> 1, y0 = gy >> 1;
+        const bool ox = gx & 1, oy = gy & 1;
+        const T wx0 = ox ? T(0.25) : T(0.5), wy0 = oy ? T(0.25) : T(0.5);
+        const bool vx0 = x0 >= 0 && x0 < w2, vx1 = ox && x0 + 1 < w2;
+        const bool vy0 = y0 >= 0 && y0 < h2, vy1 = oy && y0 + 1 < h2;
+        const long long r0 = (long long)y0 * w2, r1 = r0 + w2;
         T pe = 0;
-        for (int a = 0; a < ny; ++a) {
-            if (jy[a] < 0 || jy[a] >= h2) continue;
-            for (int b = 0; b < nx; ++b) {
-                if (jx[b] < 0 || jx[b] >= w2) continue;
-                pe += wy[a] * wx[b] * e[(long long)jy[a] * w2 + jx[b]];
-            }
+        if (vy0) {
+            T row = 0;
+            if (vx0) row += wx0 * e[r0 + x0];
+            if (vx1) row += T(0.25) * e[r0 + x0 + 1];
+            pe += wy0 * row;
+        }
+        if (vy1) {
+            T row = 0;
+            if (vx0) row += wx0 * e[r1 + x0];
+            if (vx1) row += T(0.25) * e[r1 + x0 + 1];
+            pe += T(0.25) * row;
         }
         return v + pe;
     }
