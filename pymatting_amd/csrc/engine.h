@@ -235,6 +235,16 @@ __device__ __forceinline__ T sym_coef(const T* __restrict__ c, long long n, int 
     return c[(12 - s) * n + j];
 }
 
+// sym_coef for grids below 2^31 points with 32-bit indices: the plane base
+// is uniform across the workgroup and the neighbour index only needs the
+// lower clamp (slots s < 12 point up or left).
+template <typename T>
+__device__ __forceinline__ T sym_coef32(const T* __restrict__ c, int n, int w, int i, int s) {
+    if (s >= 12) return (c + (size_t)(s - 12) * n)[(unsigned)i];
+    const int j = i + (s / 5 - 2) * w + (s % 5 - 2);
+    return (c + (size_t)(12 - s) * n)[(unsigned)(j < 0 ? 0 : j)];
+}
+
 template <int OX, int OY>
 struct NoShared {};
 
@@ -264,7 +274,7 @@ struct DiaOp {
     template <int OX, int OY>
     __device__ __forceinline__ void prefetch(Prefetch<OX, OY>& pf, int ox0, int oy0) const {
         if constexpr (PF) {
-            const long long n = (long long)h * w;
+            const int n = h * w;
             constexpr int NO = (OX * OY + kBlock - 1) / kBlock;
 #pragma unroll
             for (int k = 0; k < NO; ++k) {
@@ -272,9 +282,9 @@ struct DiaOp {
                 const int ly = t / OX, lx = t - ly * OX;
                 const int gx = ox0 + lx, gy = oy0 + ly;
                 const bool in = t < OX * OY && gx >= 0 && gx < w && gy >= 0 && gy < h;
-                const long long g = in ? (long long)gy * w + gx : 0;
+                const int g = in ? gy * w + gx : 0;
 #pragma unroll
-                for (int s = 0; s < 25; ++s) pf.c[k][s] = in ? sym_coef(coef, n, w, g, s) : T(0);
+                for (int s = 0; s < 25; ++s) pf.c[k][s] = in ? sym_coef32(coef, n, w, g, s) : T(0);
             }
         }
     }
@@ -292,10 +302,11 @@ struct DiaOp {
 #pragma unroll
             for (int s = 0; s < 25; ++s) acc += pf.c[k][s] * sx[ly + s / 5][lx + s % 5];
         } else {
-            const long long n = (long long)h * w;
+            const int n = h * w;
+            const int i = (int)g;
 #pragma unroll
             for (int s = 0; s < 25; ++s)
-                acc += sym_coef(coef, n, w, g, s) * sx[ly + s / 5][lx + s % 5];
+                acc += sym_coef32(coef, n, w, i, s) * sx[ly + s / 5][lx + s % 5];
         }
         return acc;
     }

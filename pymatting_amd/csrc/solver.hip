@@ -578,6 +578,10 @@ void vc_free(hm_vc* vc) {
 }
 
 int dia_alloc(int h, int w, hm_dia** out) {
+    if (h < 1 || w < 1 || (long long)h * w > 0x7fffffffLL) {
+        set_error("stored stencils need 1 <= h*w < 2^31 points, got %d x %d", h, w);
+        return HM_ERR_ARG;
+    }
     hm_dia* A = new hm_dia();
     A->h = h;
     A->w = w;
