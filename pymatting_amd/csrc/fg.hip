@@ -11,10 +11,14 @@
 //     previous step, and the old value of the pixel below is the "next old"
 //     register lane y+1 loaded for its right neighbour; both move by
 //     cross-lane shuffles, never through memory.
-//   - strip s reads the new last row of strip s-1 from global memory after
-//     strip s-1 has published (device-scope release/acquire) that the column
-//     is final; strip s-1 reads the old first row of strip s before strip s
-//     may overwrite it, which the same publish order guarantees.
+//   - strip s gets the new last row of strip s-1 through a halo buffer: each
+//     value is stored with a 2-bit sweep tag in its two top bits (F and B are
+//     clamped to [0, 1], whose float and double encodings leave those bits
+//     zero), as a single relaxed device-scope atomic word, so a reader that
+//     sees the current tag has the value without any fence or flag. Strip
+//     s-1 reads the old first row of strip s before strip s may overwrite it:
+//     strip s writes a column of its first row only after it has read that
+//     column of strip s-1's last row, which strip s-1 writes after reading.
 // Waits are bounded: a strip that waits too long raises a failure flag and
 // finishes, so every wave exits.
 //
@@ -28,7 +32,6 @@ namespace hm {
 
 namespace {
 
-constexpr int kFgChunk = 8;          // steps between progress publications
 constexpr int kBlockMean = 256;
 constexpr long long kSpinLimit = 1ll << 26;
 
@@ -79,159 +82,325 @@ __device__ __forceinline__ S shfl_up1(S v) { return __shfl_up(v, 1, kWave); }
 template <typename S>
 __device__ __forceinline__ S shfl_down1(S v) { return __shfl_down(v, 1, kWave); }
 
+// Tagged halo words: the value's bit pattern with the sweep tag (1..3) in its
+// two top bits.
 template <typename S>
-__device__ __forceinline__ S load_acquire_val(const S* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+struct FgWord;
+template <>
+struct FgWord<float> {
+    using U = unsigned int;
+};
+template <>
+struct FgWord<double> {
+    using U = unsigned long long;
+};
+
+template <typename S>
+__device__ __forceinline__ typename FgWord<S>::U fg_tag(S v, unsigned tag) {
+    using U = typename FgWord<S>::U;
+    return __builtin_bit_cast(U, v) | ((U)tag << (8 * sizeof(U) - 2));
+}
+template <typename S>
+__device__ __forceinline__ unsigned fg_word_tag(typename FgWord<S>::U b) {
+    return (unsigned)(b >> (8 * sizeof(b) - 2));
+}
+template <typename S>
+__device__ __forceinline__ S fg_untag(typename FgWord<S>::U b) {
+    using U = typename FgWord<S>::U;
+    return __builtin_bit_cast(S, (U)(b & ~((U)3 << (8 * sizeof(U) - 2))));
 }
 
-// One in-place Gauss-Seidel sweep over an (h, w) level. img (h,w,3), alpha
-// (h,w), F/B (h,w,3). progress[s] holds (tag << 32) | columns finished by the
-// last row of strip s; tag identifies this sweep.
+// Per-pixel coefficients of one level's 2x2 systems. They depend only on the
+// level's alpha and colours, so they are computed once per level, in
+// parallel, and every sweep of the level streams them. Planes: S [10][n]
+// (da left/right/up/down, b0 rgb, b1 rgb), double [3][n] (b00, b01, b11).
 template <typename S>
-__global__ __launch_bounds__(64) void k_fg_sweep(const S* __restrict__ img,
-                                                 const S* __restrict__ alpha, S* F, S* B, int h,
-                                                 int w, S reg, S gw,
-                                                 unsigned long long* progress,
-                                                 unsigned long long tag, int* fail) {
-    const int s = blockIdx.x;
-    const int lane = threadIdx.x;
-    const int r = s * kWave + lane;
-    const bool row_ok = r < h;
-    const int rc = row_ok ? r : h - 1;
-    const long long row = (long long)rc * w;
-    const int last_lane = min(kWave, h - s * kWave) - 1;  // lane owning the strip's last row
-
-    S cur_old[6], next_old[6], prev_res[6];
-    // Old values at column 0 (the first step's "current" pixel).
+__global__ void k_fg_coef(const S* __restrict__ img, const S* __restrict__ alpha, int h, int w,
+                          S reg, S gw, S* __restrict__ cs, double* __restrict__ cd) {
+    const long long n = (long long)h * w;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int y = (int)(i / w), x = (int)(i - (long long)y * w);
+    const long long row = (long long)y * w;
+    S an[4];
+    an[0] = alpha[row + (x > 0 ? x - 1 : 0)];
+    an[1] = alpha[row + (x + 1 < w ? x + 1 : x)];
+    an[2] = alpha[(long long)(y > 0 ? y - 1 : 0) * w + x];
+    an[3] = alpha[(long long)(y + 1 < h ? y + 1 : y) * w + x];
+    const S I[3] = {img[i * 3], img[i * 3 + 1], img[i * 3 + 2]};
+    PixelCoef<S> pc;
+    pixel_coef<S>(alpha[i], an, I, reg, gw, pc);
+#pragma unroll
+    for (int d = 0; d < 4; ++d) cs[d * n + i] = pc.da[d];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        cur_old[c] = F[row * 3 + c];
-        cur_old[3 + c] = B[row * 3 + c];
-        prev_res[c] = cur_old[c];
-        prev_res[3 + c] = cur_old[3 + c];
+        cs[(4 + c) * n + i] = pc.b0[c];
+        cs[(7 + c) * n + i] = pc.b1[c];
     }
-    const int steps = w + kWave - 1;
-    long long waited = 0;
-    bool failed = false;
-    for (int t0 = 0; t0 < steps; t0 += kFgChunk) {
-        // Strip s needs row 64s-1 finished up to the columns lane 0 touches.
-        if (s > 0) {
-            const unsigned long long need = (unsigned long long)min(t0 + kFgChunk, w);
-            while (!failed) {
-                const unsigned long long v = __hip_atomic_load(&progress[s - 1], __ATOMIC_ACQUIRE,
-                                                               __HIP_MEMORY_SCOPE_AGENT);
-                if ((v >> 32) == tag && (v & 0xffffffffull) >= need) break;
+    cd[i] = pc.b00;
+    cd[n + i] = pc.b01;
+    cd[2 * n + i] = pc.b11;
+}
+
+// One in-place Gauss-Seidel sweep over an (h, w) level with the level's
+// coefficient planes (k_fg_coef); F/B (h,w,3). halo[s][x][6] carries strip
+// s's last row, tagged with this sweep's tag (1..3), to strip s+1.
+//
+// A workgroup of kFgThreads threads owns one strip. Wave 0 computes; the
+// other waves stage the inputs of the next chunk of C steps in LDS while wave
+// 0 computes the current one (double buffering), so wave 0 waits on LDS, not
+// on global memory. Per lane and step the stage holds the pixel's
+// coefficients and the old F/B one column to the right; wave 1 also stages
+// the new row above the strip (for lane 0, after strip s-1 has published
+// those columns) and the old row below it (for the strip's last lane).
+constexpr int kFgThreads = 512;
+constexpr int kFgLoaders = kFgThreads - kWave;
+constexpr int kFgS = 16;  // S fields: da 4, b0 3, b1 3, old F/B to the right 6
+constexpr int kFgD = 3;   // double fields: b00, b01, b11
+constexpr int kFgPad = kWave + 1;  // lane stride (conflict-free staging stores)
+
+template <typename S>
+struct FgChunk {
+    static constexpr int C = sizeof(S) == 4 ? 12 : 6;  // steps per chunk
+};
+
+template <typename S>
+struct FgStage {
+    static constexpr int C = FgChunk<S>::C;
+    S sv[2][C][kFgS][kFgPad];
+    double dv[2][C][kFgD][kFgPad];
+    S up[2][C][6];
+    S dn[2][C][6];
+};
+
+// Stages chunk [t0, t0 + C) into buffer `bf`; run by the loader threads
+// (lt = threadIdx.x - kWave). Returns false if the wait for strip s-1 timed out.
+template <typename S>
+__device__ __forceinline__ bool fg_stage(FgStage<S>& st, int bf, int t0, int s, int lt,
+                                         const S* __restrict__ cs, const double* __restrict__ cd,
+                                         const S* F, const S* B, int h, int w, int last_lane,
+                                         const typename FgWord<S>::U* halo, unsigned tag,
+                                         long long& waited) {
+    using W = typename FgWord<S>::U;
+    constexpr int C = FgChunk<S>::C;
+    constexpr int U = (kWave * C + kFgLoaders - 1) / kFgLoaders;
+    const long long n = (long long)h * w;
+    bool ok = true;
+    // Row above the strip (new values, from strip s-1's halo words, waiting
+    // for the current tag) and row below it (old values). Wave 1 only.
+    if (lt < kWave) {
+        const int r0 = s * kWave;
+        const bool is_up = lt < C, is_dn = lt >= 32 && lt < 32 + C;
+        const int j = is_up ? lt : lt - 32;
+        if (is_up && r0 > 0) {
+            const int x = t0 + j;
+            const int xc = x < 0 ? 0 : (x >= w ? w - 1 : x);
+            const W* src = halo + ((long long)(s - 1) * w + xc) * 6;
+            W v[6];
+            while (true) {
+                bool all = true;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    v[q] = __hip_atomic_load(src + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    all = all && fg_word_tag<S>(v[q]) == tag;
+                }
+                if (all) break;
                 if (++waited > kSpinLimit) {
-                    failed = true;
+                    ok = false;
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
+#pragma unroll
+            for (int q = 0; q < 6; ++q) st.up[bf][j][q] = fg_untag<S>(v[q]);
         }
-        const int t1 = min(t0 + kFgChunk, steps);
-        for (int t = t0; t < t1; ++t) {
-            const int x = t - lane;
-            const bool active = row_ok && x >= 0 && x < w;
+        if (is_dn) {
+            const int x = t0 + j - last_lane;
             const int xc = x < 0 ? 0 : (x >= w ? w - 1 : x);
-            // Old values one column to the right (kept for the next step and
-            // for lane-1's down neighbour).
-            const int xn = xc + 1 < w ? xc + 1 : xc;
-            const long long pn = (row + (x + 1 >= 0 && x + 1 < w ? x + 1 : xn)) * 3;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                next_old[c] = F[pn + c];
-                next_old[3 + c] = B[pn + c];
-            }
-            // Neighbour values.
-            S left[6], right[6], up[6], down[6];
-            S from_above[6], from_below[6];
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                from_above[k] = shfl_up1(prev_res[k]);    // lane-1's result at this column
-                from_below[k] = shfl_down1(next_old[k]);  // lane+1's old value at this column
-            }
-            const long long p = (row + xc) * 3;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                left[k] = x > 0 ? prev_res[k] : cur_old[k];
-                right[k] = x + 1 < w ? next_old[k] : cur_old[k];
-            }
-            if (r == 0) {
-#pragma unroll
-                for (int k = 0; k < 6; ++k) up[k] = cur_old[k];
-            } else if (lane > 0) {
-#pragma unroll
-                for (int k = 0; k < 6; ++k) up[k] = from_above[k];
-            } else {
-                const long long pu = ((long long)(r - 1) * w + xc) * 3;
+            const int rr = s * kWave + last_lane + 1;
+            if (rr < h) {
+                const long long q = ((long long)rr * w + xc) * 3;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    up[c] = load_acquire_val(F + pu + c);
-                    up[3 + c] = load_acquire_val(B + pu + c);
+                    st.dn[bf][j][c] = F[q + c];
+                    st.dn[bf][j][3 + c] = B[q + c];
                 }
             }
-            if (r >= h - 1) {
-#pragma unroll
-                for (int k = 0; k < 6; ++k) down[k] = cur_old[k];
-            } else if (lane < last_lane) {
-#pragma unroll
-                for (int k = 0; k < 6; ++k) down[k] = from_below[k];
-            } else {
-                const long long pd = ((long long)(r + 1) * w + xc) * 3;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    down[c] = load_acquire_val(F + pd + c);
-                    down[3 + c] = load_acquire_val(B + pd + c);
-                }
-            }
-            // The 2x2 solve.
-            const long long pa = row + xc;
-            const S a0 = alpha[pa];
-            S an[4];
-            an[0] = alpha[row + (xc > 0 ? xc - 1 : 0)];
-            an[1] = alpha[row + (xc + 1 < w ? xc + 1 : xc)];
-            an[2] = alpha[(long long)(rc > 0 ? rc - 1 : 0) * w + xc];
-            an[3] = alpha[(long long)(rc + 1 < h ? rc + 1 : rc) * w + xc];
-            S I[3] = {img[p], img[p + 1], img[p + 2]};
-            PixelCoef<S> pc;
-            pixel_coef<S>(a0, an, I, reg, gw, pc);
-            S res[6];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                S b0 = pc.b0[c], b1 = pc.b1[c];
-                b0 += pc.da[0] * left[c];
-                b1 += pc.da[0] * left[3 + c];
-                b0 += pc.da[1] * right[c];
-                b1 += pc.da[1] * right[3 + c];
-                b0 += pc.da[2] * up[c];
-                b1 += pc.da[2] * up[3 + c];
-                b0 += pc.da[3] * down[c];
-                b1 += pc.da[3] * down[3 + c];
-                res[c] = clamp01_store<S>(pc.b00 * (double)b0 + pc.b01 * (double)b1);
-                res[3 + c] = clamp01_store<S>(pc.b01 * (double)b0 + pc.b11 * (double)b1);
-            }
-            if (active) {
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    F[p + c] = res[c];
-                    B[p + c] = res[3 + c];
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                prev_res[k] = res[k];
-                cur_old[k] = next_old[k];
-            }
-        }
-        // Publish how many columns the strip's last row has finished.
-        if (lane == last_lane) {
-            const int done_cols = min(max(t1 - last_lane, 0), w);
-            __threadfence();
-            __hip_atomic_store(&progress[s], (tag << 32) | (unsigned long long)done_cols,
-                               __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
-    if (failed && lane == 0) atomicExch(fail, 1);
+    // Per-lane inputs, U (lane, step) pairs per thread with all loads issued
+    // before the stores.
+    S sval[U][kFgS];
+    double dval[U][kFgD];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int q = lt + u * kFgLoaders;
+        const bool used = q < kWave * C;
+        const int y = used ? q / C : 0, j = used ? q - y * C : 0;
+        const int r = s * kWave + y;
+        const int rc = r < h ? r : h - 1;
+        const long long row = (long long)rc * w;
+        const int x = t0 + j - y;
+        const int xc = x < 0 ? 0 : (x >= w ? w - 1 : x);
+        const int xn = xc + 1 < w ? xc + 1 : xc;
+        const long long i = row + xc;
+        const long long pn = (row + (x + 1 >= 0 && x + 1 < w ? x + 1 : xn)) * 3;
+#pragma unroll
+        for (int f = 0; f < 10; ++f) sval[u][f] = cs[f * n + i];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            sval[u][10 + c] = F[pn + c];
+            sval[u][13 + c] = B[pn + c];
+        }
+#pragma unroll
+        for (int f = 0; f < kFgD; ++f) dval[u][f] = cd[f * n + i];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int q = lt + u * kFgLoaders;
+        if (q >= kWave * C) continue;
+        const int y = q / C, j = q - y * C;
+#pragma unroll
+        for (int f = 0; f < kFgS; ++f) st.sv[bf][j][f][y] = sval[u][f];
+#pragma unroll
+        for (int f = 0; f < kFgD; ++f) st.dv[bf][j][f][y] = dval[u][f];
+    }
+    return ok;
+}
+
+template <typename S>
+__global__ __launch_bounds__(kFgThreads) void k_fg_sweep(const S* __restrict__ cs,
+                                                         const double* __restrict__ cd, S* F,
+                                                         S* B, int h, int w,
+                                                         typename FgWord<S>::U* halo,
+                                                         unsigned tag, int* fail) {
+    constexpr int C = FgChunk<S>::C;
+    __shared__ FgStage<S> st;
+    const int s = blockIdx.x;
+    const bool computer = threadIdx.x < kWave;
+    const int lane = threadIdx.x;             // compute wave
+    const int lt = threadIdx.x - kWave;       // loaders
+    const int last_lane = min(kWave, h - s * kWave) - 1;  // lane owning the strip's last row
+    const int steps = w + kWave - 1;
+    const int nchunks = (steps + C - 1) / C;
+    long long waited = 0;
+    bool ok = true;
+
+    const int r = s * kWave + lane;
+    const bool row_ok = r < h;
+    const int rc = row_ok ? r : h - 1;
+    const long long row = (long long)rc * w;
+    S cur_old[6], prev_res[6];
+    if (computer) {
+        // Old values at column 0 (the first step's "current" pixel).
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            cur_old[c] = F[row * 3 + c];
+            cur_old[3 + c] = B[row * 3 + c];
+            prev_res[c] = cur_old[c];
+            prev_res[3 + c] = cur_old[3 + c];
+        }
+    } else {
+        ok = fg_stage<S>(st, 0, 0, s, lt, cs, cd, F, B, h, w, last_lane, halo, tag, waited);
+    }
+    __syncthreads();
+    for (int k = 0; k < nchunks; ++k) {
+        const int bf = k & 1;
+        const int t0 = k * C;
+        if (computer) {
+            const int t1 = min(t0 + C, steps);
+            for (int t = t0; t < t1; ++t) {
+                const int j = t - t0;
+                const int x = t - lane;
+                const bool active = row_ok && x >= 0 && x < w;
+                const int xc = x < 0 ? 0 : (x >= w ? w - 1 : x);
+                const S* sv = &st.sv[bf][j][0][lane];
+                S next_old[6];
+#pragma unroll
+                for (int c = 0; c < 6; ++c) next_old[c] = sv[(10 + c) * kFgPad];
+                S left[6], right[6], up[6], down[6];
+                S from_above[6], from_below[6];
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    from_above[q] = shfl_up1(prev_res[q]);    // lane-1's result at this column
+                    from_below[q] = shfl_down1(next_old[q]);  // lane+1's old value at this column
+                }
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    left[q] = x > 0 ? prev_res[q] : cur_old[q];
+                    right[q] = x + 1 < w ? next_old[q] : cur_old[q];
+                }
+                if (r == 0) {
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) up[q] = cur_old[q];
+                } else if (lane > 0) {
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) up[q] = from_above[q];
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) up[q] = st.up[bf][j][q];
+                }
+                if (r >= h - 1) {
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) down[q] = cur_old[q];
+                } else if (lane < last_lane) {
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) down[q] = from_below[q];
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) down[q] = st.dn[bf][j][q];
+                }
+                // The 2x2 solve (estimate_foreground_ml.py:158-175).
+                S da[4];
+#pragma unroll
+                for (int d = 0; d < 4; ++d) da[d] = sv[d * kFgPad];
+                const double b00 = st.dv[bf][j][0][lane];
+                const double b01 = st.dv[bf][j][1][lane];
+                const double b11 = st.dv[bf][j][2][lane];
+                S res[6];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    S b0 = sv[(4 + c) * kFgPad], b1 = sv[(7 + c) * kFgPad];
+                    b0 += da[0] * left[c];
+                    b1 += da[0] * left[3 + c];
+                    b0 += da[1] * right[c];
+                    b1 += da[1] * right[3 + c];
+                    b0 += da[2] * up[c];
+                    b1 += da[2] * up[3 + c];
+                    b0 += da[3] * down[c];
+                    b1 += da[3] * down[3 + c];
+                    res[c] = clamp01_store<S>(b00 * (double)b0 + b01 * (double)b1);
+                    res[3 + c] = clamp01_store<S>(b01 * (double)b0 + b11 * (double)b1);
+                }
+                if (active) {
+                    const long long p = (row + xc) * 3;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        F[p + c] = res[c];
+                        B[p + c] = res[3 + c];
+                    }
+                    if (lane == last_lane && r + 1 < h) {
+                        // The strip's last row, tagged, for strip s+1.
+                        auto* dst = halo + ((long long)s * w + xc) * 6;
+#pragma unroll
+                        for (int q = 0; q < 6; ++q)
+                            __hip_atomic_store(dst + q, fg_tag<S>(res[q], tag), __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    prev_res[q] = res[q];
+                    cur_old[q] = next_old[q];
+                }
+            }
+        } else if (k + 1 < nchunks) {
+            ok = fg_stage<S>(st, bf ^ 1, t0 + C, s, lt, cs, cd, F, B, h, w, last_lane, halo,
+                             tag, waited) && ok;
+        }
+        __syncthreads();
+    }
+    if (!ok && lt == 0) atomicExch(fail, 1);
 }
 
 // Nearest-neighbour resizes of one level (estimate_foreground_ml.py:5-55,
@@ -320,9 +489,15 @@ __global__ void k_fg_init(const S* __restrict__ means, S* __restrict__ F, S* __r
     }
 }
 
-__global__ void k_fill_u64(unsigned long long* p, long long n) {
+template <typename W>
+__global__ void k_fill_words(W* p, long long n) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0ull;
+    if (i < n) p[i] = W(0);
+}
+
+__global__ void k_fill_i32(int* p, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 }  // namespace
@@ -333,13 +508,14 @@ int fb_ml_t(const S* img0, const S* alpha0, int h0, int w0, int n_levels, const 
             S* F_out, S* B_out, hipStream_t s) {
     const long long n0 = (long long)h0 * w0;
     void *imgb = nullptr, *alphab = nullptr, *Fa = nullptr, *Ba = nullptr, *Fb = nullptr,
-         *Bb = nullptr, *means = nullptr, *prog = nullptr, *fail = nullptr;
+         *Bb = nullptr, *means = nullptr, *prog = nullptr, *fail = nullptr, *cs = nullptr,
+         *cd = nullptr;
     Reducer red{};
     int rc = HM_OK;
     const int max_strips = (int)div_up(h0, kWave);
     auto cleanup = [&]() {
         dev_free(imgb); dev_free(alphab); dev_free(Fa); dev_free(Ba); dev_free(Fb); dev_free(Bb);
-        dev_free(means); dev_free(prog); dev_free(fail);
+        dev_free(means); dev_free(prog); dev_free(fail); dev_free(cs); dev_free(cd);
         free_reducer(&red);
     };
     const unsigned mean_blocks = div_up(n0, (long long)kBlockMean * 16) > 4096
@@ -352,16 +528,17 @@ int fb_ml_t(const S* img0, const S* alpha0, int h0, int w0, int n_levels, const 
         (rc = dev_alloc(&Fb, sizeof(S) * n0 * 3)) != HM_OK ||
         (rc = dev_alloc(&Bb, sizeof(S) * n0 * 3)) != HM_OK ||
         (rc = dev_alloc(&means, sizeof(S) * 8)) != HM_OK ||
-        (rc = dev_alloc(&prog, sizeof(unsigned long long) * (max_strips + 1))) != HM_OK ||
+        (rc = dev_alloc(&prog, sizeof(typename FgWord<S>::U) * 6 * (long long)max_strips * w0)) !=
+            HM_OK ||
         (rc = dev_alloc(&fail, sizeof(int))) != HM_OK ||
+        (rc = dev_alloc(&cs, sizeof(S) * n0 * 10)) != HM_OK ||
+        (rc = dev_alloc(&cd, sizeof(double) * n0 * 3)) != HM_OK ||
         (rc = make_reducer(&red, (int)mean_blocks + 1, 8)) != HM_OK) {
         cleanup();
         return rc;
     }
-    k_fill_u64<<<div_up(max_strips + 1, 64), 64, 0, s>>>(static_cast<unsigned long long*>(prog),
-                                                         max_strips + 1);
-    hipError_t e = hipMemsetAsync(fail, 0, sizeof(int), s);
-    if (e != hipSuccess) { cleanup(); set_error("hipMemsetAsync: %s", hipGetErrorString(e)); return HM_ERR_HIP; }
+    k_fill_i32<<<1, 64, 0, s>>>(static_cast<int*>(fail), 1);
+    hipError_t e = hipSuccess;
     k_fg_means<S><<<mean_blocks, kBlockMean, 0, s>>>(img0, alpha0, n0, red, static_cast<S*>(means));
     k_fg_means_final<S><<<1, kBlockMean, 0, s>>>(red, mean_blocks, static_cast<S*>(means));
     // Level "-1": 1x1 F/B initialised with the mean colours.
@@ -369,7 +546,8 @@ int fb_ml_t(const S* img0, const S* alpha0, int h0, int w0, int n_levels, const 
     S* Bp = static_cast<S*>(Ba);
     k_fg_init<S><<<1, 64, 0, s>>>(static_cast<S*>(means), Fp, Bp);
     int hp = 1, wp = 1;
-    unsigned long long tag = 0;
+    using W = typename FgWord<S>::U;
+    W* halo = static_cast<W*>(prog);
     for (int l = 0; l <= n_levels; ++l) {
         const int h = lh[l], w = lw[l];
         const bool full = (h == h0 && w == w0);
@@ -382,12 +560,18 @@ int fb_ml_t(const S* img0, const S* alpha0, int h0, int w0, int n_levels, const 
             img0, alpha0, h0, w0, Fp, Bp, hp, wp, static_cast<S*>(imgb), static_cast<S*>(alphab),
             Fc, Bc, h, w, full ? 0 : 1);
         const int n_iter = (w <= small_size && h <= small_size) ? n_small : n_big;
+        if (n_iter > 0) {
+            k_fg_coef<S><<<div_up((long long)h * w, 256), 256, 0, s>>>(
+                img, alpha, h, w, (S)reg, (S)gw, static_cast<S*>(cs), static_cast<double*>(cd));
+            // Halo words of this level start untagged; sweeps cycle tags 1..3.
+            const long long nh = 6LL * div_up(h, kWave) * w;
+            k_fill_words<W><<<div_up(nh, 256), 256, 0, s>>>(halo, nh);
+        }
         for (int it = 0; it < n_iter; ++it) {
-            ++tag;
             ProfScope prof(kProfFgSweep, (long long)h * w, sizeof(S) == 4, s);
-            k_fg_sweep<S><<<div_up(h, kWave), kWave, 0, s>>>(
-                img, alpha, Fc, Bc, h, w, (S)reg, (S)gw, static_cast<unsigned long long*>(prog),
-                tag, static_cast<int*>(fail));
+            k_fg_sweep<S><<<div_up(h, kWave), kFgThreads, 0, s>>>(
+                static_cast<const S*>(cs), static_cast<const double*>(cd), Fc, Bc, h, w, halo,
+                1u + (unsigned)(it % 3), static_cast<int*>(fail));
         }
         e = hipGetLastError();
         if (e != hipSuccess) { cleanup(); set_error("foreground launch: %s", hipGetErrorString(e)); return HM_ERR_HIP; }
