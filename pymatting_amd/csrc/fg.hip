@@ -148,11 +148,13 @@ __global__ void k_fg_coef(const S* __restrict__ img, const S* __restrict__ alpha
 //
 // A workgroup of kFgThreads threads owns one strip. Wave 0 computes; the
 // other waves stage the inputs of the next chunk of C steps in LDS while wave
-// 0 computes the current one (double buffering), so wave 0 waits on LDS, not
-// on global memory. Per lane and step the stage holds the pixel's
+// 0 computes the current one (double buffering), and write the previous
+// chunk's results from LDS to F/B with row-contiguous stores, so wave 0 only
+// touches LDS (its lanes own different rows: its own stores would each touch
+// 64 cache lines). Per lane and step the stage holds the pixel's
 // coefficients and the old F/B one column to the right; wave 1 also stages
-// the new row above the strip (for lane 0, after strip s-1 has published
-// those columns) and the old row below it (for the strip's last lane).
+// the new row above the strip (for lane 0, from strip s-1's tagged halo) and
+// the old row below it (for the strip's last lane).
 constexpr int kFgThreads = 512;
 constexpr int kFgLoaders = kFgThreads - kWave;
 constexpr int kFgS = 16;  // S fields: da 4, b0 3, b1 3, old F/B to the right 6
@@ -161,7 +163,7 @@ constexpr int kFgPad = kWave + 1;  // lane stride (conflict-free staging stores)
 
 template <typename S>
 struct FgChunk {
-    static constexpr int C = sizeof(S) == 4 ? 12 : 6;  // steps per chunk
+    static constexpr int C = sizeof(S) == 4 ? 10 : 5;  // steps per chunk
 };
 
 template <typename S>
@@ -169,9 +171,28 @@ struct FgStage {
     static constexpr int C = FgChunk<S>::C;
     S sv[2][C][kFgS][kFgPad];
     double dv[2][C][kFgD][kFgPad];
+    S out[2][C][6][kFgPad];  // results, written back by the loader waves
     S up[2][C][6];
     S dn[2][C][6];
 };
+
+// Writes the results of chunk [t0, t0 + C) (buffer bf) to F/B; loader threads.
+template <typename S>
+__device__ __forceinline__ void fg_writeback(const FgStage<S>& st, int bf, int t0, int s, int lt,
+                                             S* F, S* B, int h, int w) {
+    constexpr int C = FgChunk<S>::C;
+    for (int q = lt; q < kWave * C; q += kFgLoaders) {
+        const int y = q / C, j = q - y * C;
+        const int r = s * kWave + y, x = t0 + j - y;
+        if (r >= h || x < 0 || x >= w) continue;
+        const long long p = ((long long)r * w + x) * 3;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            F[p + c] = st.out[bf][j][c][y];
+            B[p + c] = st.out[bf][j][3 + c][y];
+        }
+    }
+}
 
 // Stages chunk [t0, t0 + C) into buffer `bf`; run by the loader threads
 // (lt = threadIdx.x - kWave). Returns false if the wait for strip s-1 timed out.
@@ -372,13 +393,9 @@ __global__ __launch_bounds__(kFgThreads) void k_fg_sweep(const S* __restrict__ c
                     res[c] = clamp01_store<S>(b00 * (double)b0 + b01 * (double)b1);
                     res[3 + c] = clamp01_store<S>(b01 * (double)b0 + b11 * (double)b1);
                 }
-                if (active) {
-                    const long long p = (row + xc) * 3;
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        F[p + c] = res[c];
-                        B[p + c] = res[3 + c];
-                    }
+                for (int q = 0; q < 6; ++q) st.out[bf][j][q][lane] = res[q];
+                if (active) {
                     if (lane == last_lane && r + 1 < h) {
                         // The strip's last row, tagged, for strip s+1.
                         auto* dst = halo + ((long long)s * w + xc) * 6;
@@ -394,12 +411,15 @@ __global__ __launch_bounds__(kFgThreads) void k_fg_sweep(const S* __restrict__ c
                     cur_old[q] = next_old[q];
                 }
             }
-        } else if (k + 1 < nchunks) {
-            ok = fg_stage<S>(st, bf ^ 1, t0 + C, s, lt, cs, cd, F, B, h, w, last_lane, halo,
-                             tag, waited) && ok;
+        } else {
+            if (k > 0) fg_writeback<S>(st, bf ^ 1, t0 - C, s, lt, F, B, h, w);
+            if (k + 1 < nchunks)
+                ok = fg_stage<S>(st, bf ^ 1, t0 + C, s, lt, cs, cd, F, B, h, w, last_lane, halo,
+                                 tag, waited) && ok;
         }
         __syncthreads();
     }
+    if (!computer) fg_writeback<S>(st, (nchunks - 1) & 1, (nchunks - 1) * C, s, lt, F, B, h, w);
     if (!ok && lt == 0) atomicExch(fail, 1);
 }
 
