@@ -393,9 +393,9 @@ struct BatchPlan {
         const hm_dia* A = B->A[i];
         return FineOp{A->h, A->w, pick<T>(A->c64, A->c32)};
     }
-    const T* diag(int i, int l) const {
+    const T* inv_diag(int i, int l) const {
         const hm_dia* A = l == 0 ? B->A[i] : B->vc[i]->levels[l].A;
-        return pick<T>(A->diag64, A->diag32);
+        return pick<T>(A->idiag64, A->idiag32);
     }
     CoarseOp coarse_op(int i, int l) const {
         const hm_dia* A = B->vc[i]->levels[l].A;
@@ -501,20 +501,20 @@ struct BatchPlan {
             const VcLevel& cl = B->vc[i]->levels[l + 1];
             const Op op = level_op<Op>(i, l);
             const T* b = level_b(i, l, r);
-            const T* D = diag(i, l);
+            const T* iD = inv_diag(i, l);
             const int h = lev.h, w = lev.w;
             const OutRestrict<T, S> rest{b, static_cast<T*>(cl.b), cl.w, cl.h, nullptr, nodot};
             const unsigned rtx = restr_tx[l][i], ptx = plain_tx[l][i];
             if (pre == 1) {
-                rj.push_back(RestrictJ{op, InJac0<T>{b, D, omega}, rest, h, w, rtx, st + i});
+                rj.push_back(RestrictJ{op, InJac0<T>{b, iD, omega}, rest, h, w, rtx, st + i});
             } else {
                 T* cur = static_cast<T*>(lev.x);
                 T* nxt = static_cast<T*>(lev.t);
-                sj.push_back(SmoothJ{op, InJac0<T>{b, D, omega},
-                                     OutSmooth<T, S>{b, D, omega, cur, nodot}, h, w, ptx, st + i});
+                sj.push_back(SmoothJ{op, InJac0<T>{b, iD, omega},
+                                     OutSmooth<T, S>{b, iD, omega, cur, nodot}, h, w, ptx, st + i});
                 for (int k = 2; k < pre; ++k) {
                     sv[k - 2].push_back(SmoothV{op, InVec<T>{cur},
-                                                OutSmooth<T, S>{b, D, omega, nxt, nodot}, h, w,
+                                                OutSmooth<T, S>{b, iD, omega, nxt, nodot}, h, w,
                                                 ptx, st + i});
                     std::swap(cur, nxt);
                 }
@@ -562,7 +562,7 @@ struct BatchPlan {
             const VcLevel& cl = B->vc[i]->levels[l + 1];
             const Op op = level_op<Op>(i, l);
             const T* b = level_b(i, l, r);
-            const T* D = diag(i, l);
+            const T* iD = inv_diag(i, l);
             const bool top = l == 0;
             T* avail[2] = {nullptr, nullptr};
             int na = 0;
@@ -581,10 +581,10 @@ struct BatchPlan {
             for (int k = 0; k < post; ++k) {
                 T* outp = ((post - 1 - k) % 2 == 0) ? target : other;
                 const DotSink ds = (top && k == post - 1) ? DotSink{st + i, red, fin_kind} : nodot;
-                const OutSmooth<T, S> sm{b, D, omega, outp, ds};
+                const OutSmooth<T, S> sm{b, iD, omega, outp, ds};
                 if (k == 0) {
                     if (pre == 1) {
-                        fj.push_back(FirstJ{op, InProlong<T, BaseJ>{BaseJ{b, D, omega}, e, cl.w,
+                        fj.push_back(FirstJ{op, InProlong<T, BaseJ>{BaseJ{b, iD, omega}, e, cl.w,
                                                                     cl.h},
                                             sm, lev.h, lev.w, ptx, st + i});
                         fv.push_back(FirstV{});

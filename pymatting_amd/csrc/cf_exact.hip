@@ -217,7 +217,8 @@ __global__ void k_assemble_exact(const double* __restrict__ I, const double* __r
 // Diagonal of the mode-adjusted operator (float64 and float32 copies).
 __global__ void k_diag_exact(const double* __restrict__ I, const double* __restrict__ wc, int h,
                              int w, const uint8_t* __restrict__ cls, int mode, double lambda,
-                             double* __restrict__ d64, float* __restrict__ d32) {
+                             double* __restrict__ d64, float* __restrict__ d32,
+                             double* __restrict__ id64, float* __restrict__ id32) {
     const long long n = (long long)h * w;
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -232,6 +233,8 @@ __global__ void k_diag_exact(const double* __restrict__ I, const double* __restr
     }
     d64[i] = v;
     if (d32) d32[i] = (float)v;
+    if (id64) id64[i] = 1.0 / v;
+    if (id32) id32[i] = (float)(1.0 / v);
 }
 
 // Right-hand sides. HM_OP_LAMBDA: b = lambda * is_fg (laplacian.py:46-47).
@@ -333,7 +336,7 @@ int cf_exact_setup(hm_cf* op, hipStream_t s) {
                                      op->wc);
     HM_LAUNCH_CHECK();
     k_diag_exact<<<g, 256, 0, s>>>(op->I64, op->wc, op->h, op->w, op->cls, op->mode, op->lambda,
-                                   op->diag64, op->diag32);
+                                   op->diag64, op->diag32, op->idiag64, op->idiag32);
     HM_LAUNCH_CHECK();
     k_planar_to_f32<<<div_up(9 * n, 256), 256, 0, s>>>(op->wc, op->wc32, 9 * n);
     HM_LAUNCH_CHECK();

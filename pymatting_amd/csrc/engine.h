@@ -341,9 +341,9 @@ struct InZero {
 template <typename T>
 struct InJac0 {
     const T* b;
-    const T* D;
+    const T* iD;  // 1 / D
     T omega;
-    __device__ __forceinline__ T load(int, int, long long g) const { return omega * b[g] / D[g]; }
+    __device__ __forceinline__ T load(int, int, long long g) const { return omega * b[g] * iD[g]; }
 };
 
 // base(x) + P e with P = R^T of the full-weighting restriction (vcycle.py:6-29).
@@ -490,17 +490,17 @@ struct OutSmooth {
     HM_TILE_GEOM(S)
     static constexpr int kProf = 2;
     const T* b;
-    const T* D;
+    const T* iD;  // 1 / D
     T omega;
     T* z;
     DotSink dot;
-    OutSmooth<T, 1> small() const { return {b, D, omega, z, dot}; }
+    OutSmooth<T, 1> small() const { return {b, iD, omega, z, dot}; }
     static constexpr int kPre = 2;
-    __device__ __forceinline__ T preload(int j, long long g) const { return j == 0 ? b[g] : D[g]; }
+    __device__ __forceinline__ T preload(int j, long long g) const { return j == 0 ? b[g] : iD[g]; }
     __device__ __forceinline__ void point(long long g, int, int, T Ax, T xi, double& acc,
                                           const T* pv) const {
         const T bg = pv[0];
-        const T zg = xi + omega * (bg - Ax) / pv[1];
+        const T zg = xi + omega * (bg - Ax) * pv[1];
         z[g] = zg;
         acc += (double)bg * (double)zg;
     }

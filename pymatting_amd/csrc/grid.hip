@@ -186,7 +186,8 @@ int dia_galerkin(const hm_dia* fine, hm_dia* coarse, hipStream_t s) {
 }
 
 __global__ void k_dia_finish(const double* __restrict__ c64, float* __restrict__ c32,
-                             double* __restrict__ d64, float* __restrict__ d32, long long n) {
+                             double* __restrict__ d64, float* __restrict__ d32,
+                             double* __restrict__ id64, float* __restrict__ id32, long long n) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= kSymSlots * n) return;
     const double v = c64[t];
@@ -194,6 +195,8 @@ __global__ void k_dia_finish(const double* __restrict__ c64, float* __restrict__
     if (t < n) {  // stored slot 0 is the diagonal (s = 12)
         d64[t] = v;
         d32[t] = (float)v;
+        id64[t] = 1.0 / v;
+        id32[t] = (float)(1.0 / v);
     }
 }
 
@@ -228,8 +231,8 @@ int dia_unpack(const hm_dia* A, double* full, hipStream_t s) {
 }
 
 int dia_finish(hm_dia* A, hipStream_t s) {
-    k_dia_finish<<<div_up(kSymSlots * A->n, 256), 256, 0, s>>>(A->c64, A->c32, A->diag64,
-                                                               A->diag32, A->n);
+    k_dia_finish<<<div_up(kSymSlots * A->n, 256), 256, 0, s>>>(
+        A->c64, A->c32, A->diag64, A->diag32, A->idiag64, A->idiag32, A->n);
     HM_LAUNCH_CHECK();
     return HM_OK;
 }

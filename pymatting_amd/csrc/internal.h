@@ -21,6 +21,8 @@ struct hm_cf {
     float* wc32 = nullptr;   // float32 copy of the first 9 planes
     double* diag64 = nullptr;
     float* diag32 = nullptr;
+    double* idiag64 = nullptr;  // 1 / diag (float32 copy rounded from float64)
+    float* idiag32 = nullptr;
 };
 
 // Stored symmetric 25-point stencil (slot-major [13][n], see engine.h) with
@@ -32,6 +34,8 @@ struct hm_dia {
     float* c32 = nullptr;
     double* diag64 = nullptr;
     float* diag32 = nullptr;
+    double* idiag64 = nullptr;  // 1 / diag (float32 copy rounded from float64)
+    float* idiag32 = nullptr;
 };
 
 struct hm_csr {

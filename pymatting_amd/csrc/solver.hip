@@ -76,14 +76,14 @@ int vcycle_apply_t(hm_vc* vc, const T* r, T* z, CgState* st, const Reducer* red,
         if (vc->pre == 0) {
             HM_TRY(run_level<T>(op, InZero<T>{}, rest, guard, s));
         } else if (vc->pre == 1) {
-            HM_TRY(run_level<T>(op, InJac0<T>{b, op.D, omega}, rest, guard, s));
+            HM_TRY(run_level<T>(op, InJac0<T>{b, op.iD, omega}, rest, guard, s));
         } else {
             T* cur = vptr<T>(lev.x);
             T* nxt = vptr<T>(lev.t);
-            HM_TRY(run_level<T>(op, InJac0<T>{b, op.D, omega},
-                                OutSmooth<T>{b, op.D, omega, cur, nodot}, guard, s));
+            HM_TRY(run_level<T>(op, InJac0<T>{b, op.iD, omega},
+                                OutSmooth<T>{b, op.iD, omega, cur, nodot}, guard, s));
             for (int k = 2; k < vc->pre; ++k) {
-                HM_TRY(run_level<T>(op, InVec<T>{cur}, OutSmooth<T>{b, op.D, omega, nxt, nodot},
+                HM_TRY(run_level<T>(op, InVec<T>{cur}, OutSmooth<T>{b, op.iD, omega, nxt, nodot},
                                     guard, s));
                 std::swap(cur, nxt);
             }
@@ -103,15 +103,15 @@ int vcycle_apply_t(hm_vc* vc, const T* r, T* z, CgState* st, const Reducer* red,
         T* ts = vptr<T>(cl.t);
         const int m = vc->coarse_sweeps - 1;  // sweeps after the first iterate
         if (m == 0) {
-            k_jac0<T><<<div_up(cl.n, 256), 256, 0, s>>>(cb, op.D, omega, xs, cl.n, guard);
+            k_jac0<T><<<div_up(cl.n, 256), 256, 0, s>>>(cb, op.iD, omega, xs, cl.n, guard);
             HM_LAUNCH_CHECK();
         } else {
             T* cur = (m % 2 == 1) ? xs : ts;
             T* oth = (m % 2 == 1) ? ts : xs;
-            HM_TRY(run_level<T>(op, InJac0<T>{cb, op.D, omega},
-                                OutSmooth<T>{cb, op.D, omega, cur, nodot}, guard, s));
+            HM_TRY(run_level<T>(op, InJac0<T>{cb, op.iD, omega},
+                                OutSmooth<T>{cb, op.iD, omega, cur, nodot}, guard, s));
             for (int k = 1; k < m; ++k) {
-                HM_TRY(run_level<T>(op, InVec<T>{cur}, OutSmooth<T>{cb, op.D, omega, oth, nodot},
+                HM_TRY(run_level<T>(op, InVec<T>{cur}, OutSmooth<T>{cb, op.iD, omega, oth, nodot},
                                     guard, s));
                 std::swap(cur, oth);
             }
@@ -153,7 +153,7 @@ int vcycle_apply_t(hm_vc* vc, const T* r, T* z, CgState* st, const Reducer* red,
             rc = post_stage<T>(op, InZero<T>{}, b, e, cl.h, cl.w, vc->post, omega, target, other,
                                fin, r, guard, s, R);
         else if (vc->pre == 1)
-            rc = post_stage<T>(op, InJac0<T>{b, op.D, omega}, b, e, cl.h, cl.w, vc->post, omega,
+            rc = post_stage<T>(op, InJac0<T>{b, op.iD, omega}, b, e, cl.h, cl.w, vc->post, omega,
                                target, other, fin, r, guard, s, R);
         else
             rc = post_stage<T>(op, InVec<T>{pre_x[l]}, b, e, cl.h, cl.w, vc->post, omega,
@@ -591,6 +591,8 @@ int dia_alloc(int h, int w, hm_dia** out) {
         rc = dev_alloc(reinterpret_cast<void**>(&A->c32), sizeof(float) * kSymSlots * A->n);
     if (rc == HM_OK) rc = dev_alloc(reinterpret_cast<void**>(&A->diag64), sizeof(double) * A->n);
     if (rc == HM_OK) rc = dev_alloc(reinterpret_cast<void**>(&A->diag32), sizeof(float) * A->n);
+    if (rc == HM_OK) rc = dev_alloc(reinterpret_cast<void**>(&A->idiag64), sizeof(double) * A->n);
+    if (rc == HM_OK) rc = dev_alloc(reinterpret_cast<void**>(&A->idiag32), sizeof(float) * A->n);
     if (rc != HM_OK) {
         hm_dia_destroy(A);
         return rc;
@@ -776,6 +778,8 @@ int hm_cf_create(const double* image_hwc, int h, int w, double epsilon, int radi
     if (rc == HM_OK) rc = dev_alloc(reinterpret_cast<void**>(&op->wc32), sizeof(float) * 9 * n);
     if (rc == HM_OK) rc = dev_alloc(reinterpret_cast<void**>(&op->diag64), sizeof(double) * n);
     if (rc == HM_OK) rc = dev_alloc(reinterpret_cast<void**>(&op->diag32), sizeof(float) * n);
+    if (rc == HM_OK) rc = dev_alloc(reinterpret_cast<void**>(&op->idiag64), sizeof(double) * n);
+    if (rc == HM_OK) rc = dev_alloc(reinterpret_cast<void**>(&op->idiag32), sizeof(float) * n);
     if (rc == HM_OK && cls) {
         rc = dev_alloc(reinterpret_cast<void**>(&op->cls), n);
         if (rc == HM_OK) {
@@ -805,6 +809,8 @@ int hm_cf_destroy(hm_cf* op) {
     dev_free(op->wc32);
     dev_free(op->diag64);
     dev_free(op->diag32);
+    dev_free(op->idiag64);
+    dev_free(op->idiag32);
     delete op;
     return HM_OK;
 }
@@ -892,6 +898,8 @@ int hm_dia_destroy(hm_dia* A) {
     dev_free(A->c32);
     dev_free(A->diag64);
     dev_free(A->diag32);
+    dev_free(A->idiag64);
+    dev_free(A->idiag32);
     delete A;
     return HM_OK;
 }

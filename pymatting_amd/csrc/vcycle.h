@@ -36,7 +36,7 @@ struct LevelOp {
     bool coarse = false;  // a Galerkin level (profiling class only)
     CfOp<T> cf{};
     DiaOp<T> dia{};
-    const T* D = nullptr;
+    const T* iD = nullptr;  // 1 / diag
     int h = 0, w = 0;
 };
 
@@ -55,7 +55,7 @@ LevelOp<T> cf_level(const hm_cf* op, int force_mode = -1) {
     L.cf.I = pick<T>(op->I64, op->I32);
     L.cf.W = pick<T>(op->wc, op->wc32);
     L.cf.force_mode = force_mode;
-    L.D = pick<T>(op->diag64, op->diag32);
+    L.iD = pick<T>(op->idiag64, op->idiag32);
     L.h = op->h;
     L.w = op->w;
     return L;
@@ -66,7 +66,7 @@ LevelOp<T> dia_level(const hm_dia* A) {
     LevelOp<T> L;
     L.is_cf = false;
     L.dia = DiaOp<T>{A->h, A->w, pick<T>(A->c64, A->c32)};
-    L.D = pick<T>(A->diag64, A->diag32);
+    L.iD = pick<T>(A->idiag64, A->idiag32);
     L.h = A->h;
     L.w = A->w;
     return L;
@@ -232,13 +232,14 @@ __global__ __launch_bounds__(kBlock) void k_dense_apply(const T* __restrict__ in
     }
 }
 
-// First damped-Jacobi iterate from zero: x = omega b / D (vcycle.py:35-38).
+// First damped-Jacobi iterate from zero: x = omega b / D (vcycle.py:35-38),
+// with iD = 1 / D.
 template <typename T>
-__global__ void k_jac0(const T* __restrict__ b, const T* __restrict__ D, T omega,
+__global__ void k_jac0(const T* __restrict__ b, const T* __restrict__ iD, T omega,
                        T* __restrict__ x, long long n, const CgState* guard) {
     if (solve_done(guard)) return;
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = omega * b[i] / D[i];
+    if (i < n) x[i] = omega * b[i] * iD[i];
 }
 
 // ------------------------------------------------------------------ V-cycle
@@ -277,7 +278,7 @@ int post_stage(const LevelOp<T>& op, const Base& base, const T* b, const T* e, i
     for (int k = 0; k < post; ++k) {
         T* outp = ((post - 1 - k) % 2 == 0) ? target : other;
         const DotSink ds = (k == post - 1) ? fin : nodot;
-        OutSmooth<T> sm{b, op.D, omega, outp, ds};
+        OutSmooth<T> sm{b, op.iD, omega, outp, ds};
         if (k == 0)
             HM_TRY(run_level<T>(op, pro, sm, guard, s));
         else
@@ -388,6 +389,23 @@ __device__ __forceinline__ void vc_tail_body(const TailParams<T>& P) {
     T* lds = reinterpret_cast<T*>(tail_lds_raw);
     const int tid = threadIdx.x;
     const int nlev = P.nlev;
+    // The top grid's stencil in registers when it is not staged in LDS (at
+    // most two points per thread; float32 only, float64 would exceed the
+    // registers of a 1024-thread workgroup), loaded first so it overlaps the
+    // staging.
+    const TailView<T> V0 = tail_view(lds, P.lv[0]);
+    const bool top_reg = sizeof(T) == 4 && !P.lv[0].coef_lds && V0.n <= 2 * kTailBlock;
+    T tc[2][25];
+    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int i = tid + k * kTailBlock;
+            const long long ii = top_reg && i < V0.n ? i : 0;
+#pragma unroll
+            for (int s = 0; s < 25; ++s)
+                tc[k][s] = top_reg ? sym_coef(V0.coef, (long long)V0.n, V0.w, ii, s) : T(0);
+        }
+    }
     // Zero every vector (the halos must read as zero), then stage the top
     // right-hand side, the LDS stencils and the inverse. kStage loads per
     // thread are issued before the stores.
@@ -439,34 +457,66 @@ __device__ __forceinline__ void vc_tail_body(const TailParams<T>& P) {
     // the number of sweeps (one buffer swap per sweep after the first).
     const bool pre_in_t = P.pre >= 2 && ((P.pre - 1) & 1);
     const T omega = P.omega;
+    // A thread's points of level V; for the top grid with its stencil in
+    // registers (reg) the point's register slot k is a compile-time index.
+    auto for_points = [&](const TailView<T>& V, bool reg, auto&& body) {
+        if (reg) {
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int i = tid + k * kTailBlock;
+                if (i < V.n) {
+                    const int y = i / V.w;
+                    body(k, i, y, i - y * V.w);
+                }
+            }
+        } else {
+            for (int i = tid; i < V.n; i += kTailBlock) {
+                const int y = i / V.w;
+                body(0, i, y, i - y * V.w);
+            }
+        }
+    };
+    auto Ax = [&](const TailView<T>& V, bool reg, int k, const T* v, int y, int x) -> T {
+        if (sizeof(T) == 4 && reg) {
+            const T* c = v + y * V.pitch + x;
+            T acc = 0;
+#pragma unroll
+            for (int s = 0; s < 25; ++s) acc += tc[k][s] * c[(s / 5) * V.pitch + s % 5];
+            return acc;
+        }
+        return tail_Ax(V, v, y, x);
+    };
+    auto diag = [&](const TailView<T>& V, bool reg, int k, int i) -> T {
+        return (sizeof(T) == 4 && reg) ? tc[k][12] : V.coef[i];
+    };
     // Descend: pre-smooth, residual, restrict.
     for (int l = 0; l + 1 < nlev; ++l) {
         const TailView<T> V = tail_view(lds, P.lv[l]);
+        const bool reg = l == 0 && top_reg;
         T* cur = V.x;
         T* nxt = V.t;
         if (P.pre > 0) {
-            for (int i = tid; i < V.n; i += kTailBlock) {
-                const int y = i / V.w, x = i - y * V.w;
-                cur[V.at(y, x)] = omega * V.b[V.at(y, x)] / V.coef[i];
-            }
-            for (int k = 1; k < P.pre; ++k) {
+            for_points(V, reg, [&](int k, int i, int y, int x) {
+                const int p = V.at(y, x);
+                cur[p] = omega * V.b[p] / diag(V, reg, k, i);
+            });
+            for (int q = 1; q < P.pre; ++q) {
                 __syncthreads();
-                for (int i = tid; i < V.n; i += kTailBlock) {
-                    const int y = i / V.w, x = i - y * V.w;
+                for_points(V, reg, [&](int k, int i, int y, int x) {
                     const int p = V.at(y, x);
-                    nxt[p] = cur[p] + omega * (V.b[p] - tail_Ax(V, cur, y, x)) / V.coef[i];
-                }
+                    nxt[p] = cur[p] + omega * (V.b[p] - Ax(V, reg, k, cur, y, x)) /
+                                          diag(V, reg, k, i);
+                });
                 T* sw = cur;
                 cur = nxt;
                 nxt = sw;
             }
         }
         __syncthreads();
-        for (int i = tid; i < V.n; i += kTailBlock) {
-            const int y = i / V.w, x = i - y * V.w;
+        for_points(V, reg, [&](int k, int i, int y, int x) {
             const int p = V.at(y, x);
-            V.u[p] = V.b[p] - tail_Ax(V, cur, y, x);
-        }
+            V.u[p] = V.b[p] - Ax(V, reg, k, cur, y, x);
+        });
         __syncthreads();
         const TailView<T> C = tail_view(lds, P.lv[l + 1]);
         for (int I = tid; I < C.n; I += kTailBlock) {
@@ -503,6 +553,7 @@ __device__ __forceinline__ void vc_tail_body(const TailParams<T>& P) {
     for (int l = nlev - 2; l >= 0; --l) {
         const TailView<T> V = tail_view(lds, P.lv[l]);
         const TailView<T> C = tail_view(lds, P.lv[l + 1]);
+        const bool reg = l == 0 && top_reg;
         const T* xp = pre_in_t ? V.t : V.x;
         T* a = pre_in_t ? V.x : V.t;  // must not overwrite the pre-smoothed iterate
         for (int i = tid; i < V.n; i += kTailBlock) {
@@ -512,13 +563,12 @@ __device__ __forceinline__ void vc_tail_body(const TailParams<T>& P) {
         }
         T* cur = a;
         T* nxt = V.u;
-        for (int k = 0; k < P.post; ++k) {
+        for (int q = 0; q < P.post; ++q) {
             __syncthreads();
-            for (int i = tid; i < V.n; i += kTailBlock) {
-                const int y = i / V.w, x = i - y * V.w;
+            for_points(V, reg, [&](int k, int i, int y, int x) {
                 const int p = V.at(y, x);
-                nxt[p] = cur[p] + omega * (V.b[p] - tail_Ax(V, cur, y, x)) / V.coef[i];
-            }
+                nxt[p] = cur[p] + omega * (V.b[p] - Ax(V, reg, k, cur, y, x)) / diag(V, reg, k, i);
+            });
             T* sw = cur;
             cur = nxt;
             nxt = sw;
