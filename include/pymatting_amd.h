@@ -187,6 +187,16 @@ int hm_cg_solve_mixed(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, doub
                       double rtol, double inner_rtol, int max_outer, int maxiter,
                       hm_cg_stats* stats, void* stream);
 
+/* Mixed precision without restarts: one CG whose recurrences (stencil, SpMV,
+ * V-cycle) run in float32 with float64 dot products, the solution
+ * accumulated in float64, and the recursive residual replaced by the true
+ * float64 residual b - A x whenever it has dropped by a factor delta since
+ * the last replacement; convergence (||b - A x|| < rtol ||b||) is confirmed
+ * on a true residual. Same operator arguments as hm_cg_solve_mixed. */
+int hm_cg_solve_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x,
+                         double rtol, double delta, int maxiter, int check_every,
+                         hm_cg_stats* stats, void* stream);
+
 /* z = r * inv_diag (jacobi.py:24-31 applied to a vector). */
 int hm_diag_apply(const double* inv_diag, const double* r, double* z, int64_t n, void* stream);
 

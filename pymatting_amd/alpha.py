@@ -87,9 +87,15 @@ class LambdaSolver:
         self.stats = {}
 
     def solve_system(self, b, rtol=1e-7, atol=0.0, maxiter=10000, inner_rtol=1e-4,
-                     max_outer=20, check_every=8):
+                     max_outer=20, check_every=8, mixed="refine", delta=0.1):
         """x = A^-1 b for a device float64 right-hand side b (length h*w), with
-        this solver's operator, preconditioner and precision."""
+        this solver's operator, preconditioner and precision.
+
+        For precision "mixed", ``mixed`` picks the float64 outer loop:
+        "refine" restarts a float32 PCG on each float64 residual (inner_rtol,
+        max_outer); "reliable" runs one float32 PCG whose solution accumulates
+        in float64 and whose residual is replaced by the float64 one each time
+        it has dropped by ``delta``."""
         import ctypes
         n = self.h * self.w
         b = b.reshape(-1)
@@ -105,18 +111,27 @@ class LambdaSolver:
             stats = _ffi.CgStats()
             cf = self.op.handle if isinstance(self.op, CfOperator) else None
             dia = self.op.handle if isinstance(self.op, DiaOperator) else None
-            _ffi.check(_ffi.lib().hm_cg_solve_mixed(
-                cf, dia, self.M.handle, _ffi.ptr(b), _ffi.ptr(x), float(rtol),
-                float(inner_rtol), int(max_outer), int(maxiter), ctypes.byref(stats),
-                _ffi.stream()))
+            if mixed == "reliable":
+                rc = _ffi.lib().hm_cg_solve_reliable(
+                    cf, dia, self.M.handle, _ffi.ptr(b), _ffi.ptr(x), float(rtol),
+                    float(delta), int(maxiter), int(check_every), ctypes.byref(stats),
+                    _ffi.stream())
+            elif mixed == "refine":
+                rc = _ffi.lib().hm_cg_solve_mixed(
+                    cf, dia, self.M.handle, _ffi.ptr(b), _ffi.ptr(x), float(rtol),
+                    float(inner_rtol), int(max_outer), int(maxiter), ctypes.byref(stats),
+                    _ffi.stream())
+            else:
+                raise ValueError("mixed must be 'refine' or 'reliable'")
+            _ffi.check(rc)
             self.stats = {"iterations": stats.iterations, "norm_b": stats.norm_b,
                           "norm_r": stats.norm_r}
         return x
 
     def solve(self, rtol=1e-7, atol=0.0, maxiter=10000, inner_rtol=1e-4, max_outer=20,
-              check_every=8):
+              check_every=8, mixed="refine", delta=0.1):
         self.x = self.solve_system(self.b, rtol, atol, maxiter, inner_rtol, max_outer,
-                                   check_every)
+                                   check_every, mixed, delta)
         return _finalize(self.x, self.tdev, None, (self.h, self.w))
 
 

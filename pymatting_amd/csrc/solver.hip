@@ -560,6 +560,185 @@ int pcg_mixed(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x, d
     return HM_OK;
 }
 
+// Mixed precision with reliable updates (one CG, no restarts): the CG
+// recurrences run in float32 (stencil, SpMV, V-cycle; float64 dot products)
+// while the solution accumulates in float64 (x += alpha p), and every time
+// the recursive residual has dropped by `delta` since the last replacement it
+// is replaced by the true float64 residual b - A x (rounded to float32)
+// before the preconditioner runs; convergence is declared on a true residual.
+__global__ __launch_bounds__(kBlock) void k_update_x64(double* __restrict__ x,
+                                                       const float* __restrict__ p,
+                                                       float* __restrict__ r,
+                                                       const float* __restrict__ q, long long n,
+                                                       CgState* st, Reducer red) {
+    if (solve_done(st)) return;
+    const double alpha = st->alpha;
+    const float alpha32 = (float)alpha;
+    double acc = 0.0;
+    const long long base = (long long)blockIdx.x * kBlock * kVecPerThread + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kVecPerThread; ++k) {
+        const long long i = base + (long long)k * kBlock;
+        if (i < n) {
+            x[i] += alpha * (double)p[i];
+            const float ri = r[i] - alpha32 * q[i];
+            r[i] = ri;
+            acc += (double)ri * (double)ri;
+        }
+    }
+    double v[1] = {acc};
+    block_partials<kBlock, 1>(v, red);
+}
+
+// r32 = (float)r64 with ||r64||^2 partials (replacing the update's).
+__global__ __launch_bounds__(kBlock) void k_replace_r(const double* __restrict__ r64,
+                                                      float* __restrict__ r32, long long n,
+                                                      const CgState* st, Reducer red) {
+    if (solve_done(st)) return;
+    double acc = 0.0;
+    const long long base = (long long)blockIdx.x * kBlock * kVecPerThread + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kVecPerThread; ++k) {
+        const long long i = base + (long long)k * kBlock;
+        if (i < n) {
+            const double v = r64[i];
+            r32[i] = (float)v;
+            acc += v * v;
+        }
+    }
+    double v[1] = {acc};
+    block_partials<kBlock, 1>(v, red);
+}
+
+int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x, double rtol,
+                 double delta, int maxiter, int check_every, hm_cg_stats* stats, hipStream_t s) {
+    OpHandle A;
+    A.cf = cf;
+    A.dia = dia;
+    PcHandle M;
+    M.kind = HM_PC_VCYCLE;
+    M.vc = vc32;
+    const long long n = A.n();
+    DevBuf r64, r, p, p2, q, z;
+    HM_TRY(r64.alloc(sizeof(double) * n));
+    HM_TRY(r.alloc(sizeof(float) * n));
+    HM_TRY(p.alloc(sizeof(float) * n));
+    HM_TRY(p2.alloc(sizeof(float) * n));
+    HM_TRY(q.alloc(sizeof(float) * n));
+    HM_TRY(z.alloc(sizeof(float) * n));
+    Reducer red{};
+    const unsigned vb = vec_blocks(n);
+    HM_TRY(make_reducer(&red, std::max<int>(reducer_blocks(n, A.h(), A.w()), (int)vb), 3));
+    struct RedGuard { Reducer* r; ~RedGuard() { free_reducer(r); } } rg{&red};
+    red.upd_blocks = vb;
+    Reducer red_upd = red;
+    red_upd.partials += 2 * (size_t)red.max_blocks;
+    DeviceState st;
+    HM_TRY(st.init());
+    HM_TRY(st.push(s));
+    stats->iterations = 0;
+    stats->converged = 0;
+    // norm_b; x = 0; r64 = b; r = (float)b.
+    HM_TRY(launch_dot<double>(b, b, nullptr, n, st.dev, red, kDotAux, s));
+    HM_TRY(st.pull(s));
+    const double norm_b = std::sqrt(st.host->aux);
+    stats->norm_b = norm_b;
+    stats->norm_r = norm_b;
+    if (!(norm_b > 0.0)) {  // x = 0 solves b = 0
+        k_fill<double><<<div_up(n, 256), 256, 0, s>>>(x, 0.0, n);
+        HM_LAUNCH_CHECK();
+        stats->converged = 1;
+        return HM_OK;
+    }
+    std::memset(st.host, 0, sizeof(CgState));
+    st.host->atol = 0.0;
+    st.host->rtol = rtol;
+    st.host->norm_b = norm_b;
+    st.host->rr = norm_b * norm_b;
+    HM_TRY(st.push(s));
+    k_fill<double><<<div_up(n, 256), 256, 0, s>>>(x, 0.0, n);
+    HM_LAUNCH_CHECK();
+    k_replace_r<<<vb, kBlock, 0, s>>>(b, r.as<float>(), n, nullptr, red_upd);
+    HM_LAUNCH_CHECK();
+    k_fill<float><<<div_up(n, 256), 256, 0, s>>>(p.as<float>(), 0.0f, n);
+    HM_LAUNCH_CHECK();
+    HM_TRY(pc_apply<float>(M, r.as<float>(), z.as<float>(), st.dev, red, kDotRzFirst, nullptr, n,
+                           s));
+    float* p_cur = p.as<float>();
+    float* p_next = p2.as<float>();
+    const int K = check_every < 1 ? 8 : check_every;
+    const DotSink pq{st.dev, red, kDotPQ};
+    double rr_replaced = norm_b * norm_b;  // ||r||^2 at the last replacement
+    bool replace_next = false;             // replace in the last iteration of the next batch
+    int enqueued = 0;
+    bool last_replaced = false;
+    while (true) {
+        const int k = std::min(K, maxiter - enqueued);
+        if (k <= 0) break;
+        for (int it = 0; it < k; ++it) {
+            const bool replace = replace_next && it == k - 1;
+            HM_TRY(op_apply_pupdate<float>(A, z.as<float>(), p_cur, p_next, q.as<float>(), pq,
+                                           st.dev, s));
+            std::swap(p_cur, p_next);
+            {
+                ProfScope prof_u(kProfCgUpdate, n, 1, s);
+                k_update_x64<<<vb, kBlock, 0, s>>>(x, p_cur, r.as<float>(), q.as<float>(), n,
+                                                   st.dev, red_upd);
+                HM_LAUNCH_CHECK();
+            }
+            if (replace) {
+                // r = b - A x in float64, its norm for the convergence test.
+                HM_TRY(op_residual<double>(A, b, x, r64.as<double>(), s));
+                k_replace_r<<<vb, kBlock, 0, s>>>(r64.as<double>(), r.as<float>(), n, st.dev,
+                                                  red_upd);
+                HM_LAUNCH_CHECK();
+            }
+            HM_TRY(pc_apply<float>(M, r.as<float>(), z.as<float>(), st.dev, red, kDotUpdateRz,
+                                   st.dev, n, s));
+            last_replaced = replace;
+        }
+        enqueued += k;
+        HM_TRY(st.pull(s));
+        if (st.host->nonfinite) {
+            set_error("non-finite value in the mixed solve");
+            return HM_ERR_VALUE;
+        }
+        if (st.host->done) {
+            if (last_replaced && st.host->iters == enqueued) break;  // tested on a true residual
+            // Confirm on the true residual; continue if the recursion drifted.
+            HM_TRY(op_residual<double>(A, b, x, r64.as<double>(), s));
+            HM_TRY(launch_dot<double>(r64.as<double>(), r64.as<double>(), nullptr, n, st.dev,
+                                      red, kDotAux, s));
+            HM_TRY(st.pull(s));
+            const double rr_true = st.host->aux;
+            if (std::sqrt(rr_true) < rtol * norm_b) break;
+            st.host->done = 0;
+            st.host->rr = rr_true;
+            HM_TRY(st.push(s));
+            k_replace_r<<<vb, kBlock, 0, s>>>(r64.as<double>(), r.as<float>(), n, nullptr,
+                                              red_upd);
+            HM_LAUNCH_CHECK();
+            // The same CG continues from the true residual (beta from the new <r, z>).
+            HM_TRY(pc_apply<float>(M, r.as<float>(), z.as<float>(), st.dev, red, kDotRzUpdate,
+                                   st.dev, n, s));
+            enqueued = st.host->iters;
+            rr_replaced = rr_true;
+            replace_next = false;
+            continue;
+        }
+        if (last_replaced) rr_replaced = st.host->rr;
+        replace_next = st.host->rr < delta * delta * rr_replaced;
+    }
+    stats->iterations = st.host->iters;
+    stats->norm_r = std::sqrt(st.host->rr);
+    stats->converged = st.host->done;
+    if (!st.host->done) {
+        set_error("Conjugate gradient descent did not converge within %d iterations", maxiter);
+        return HM_ERR_NOCONV;
+    }
+    return HM_OK;
+}
+
 // ------------------------------------------------------------ V-cycle set-up
 
 void vc_free(hm_vc* vc) {
@@ -1096,6 +1275,18 @@ int hm_cg_solve(const hm_cg_args* a, hm_cg_stats* stats, void* stream) {
     opt.callback = a->callback;
     opt.cb_ctx = a->cb_ctx;
     return pcg_t<double>(A, M, a->b, a->x0, a->x, opt, stats, static_cast<hipStream_t>(stream));
+}
+
+int hm_cg_solve_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x,
+                         double rtol, double delta, int maxiter, int check_every,
+                         hm_cg_stats* stats, void* stream) {
+    if ((cf == nullptr) == (dia == nullptr) || !vc32 || !b || !x || !stats ||
+        vc32->prec != HM_F32 || !(delta > 0.0 && delta < 1.0)) {
+        set_error("hm_cg_solve_reliable: needs one operator, a float32 V-cycle and 0 < delta < 1");
+        return HM_ERR_ARG;
+    }
+    return pcg_reliable(cf, dia, vc32, b, x, rtol, delta, maxiter, check_every, stats,
+                        static_cast<hipStream_t>(stream));
 }
 
 int hm_cg_solve_mixed(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x,

@@ -58,3 +58,27 @@ def test_lambda_solve_config1_fp64_and_mixed():
     alpha32 = alpha32.cpu().numpy()
     print("mixed", st32, "max|d| vs tight", np.abs(alpha32 - tight).max(), flush=True)
     assert np.abs(alpha32 - tight).max() <= 1e-4
+
+
+@pytest.mark.parametrize("level0", ["matrix_free", "stencil"])
+def test_lambda_solve_config1_reliable_updates(level0):
+    """The mixed form without restarts (float32 CG, float64 solution and
+    residual replacement) reaches the same tight solution within 1e-4 and
+    reports a residual below rtol that it confirmed in float64."""
+    import torch
+    import pymatting_amd as pm
+    from pymatting_amd import synthetic
+    image, trimap, _ = synthetic.config1()
+    tight, _ = ocf.solve_alpha_lambda(image, trimap, rtol=1e-11)
+    solver = pm.LambdaSolver(image, trimap, precision="mixed", level0=level0)
+    for delta in (0.3, 0.05):
+        alpha = solver.solve(rtol=1e-9, mixed="reliable", delta=delta).cpu().numpy()
+        st = solver.stats
+        print(level0, delta, st, "max|d| vs tight", np.abs(alpha - tight).max(), flush=True)
+        assert np.abs(alpha - tight).max() <= 1e-4
+        assert st["norm_r"] < 1e-9 * st["norm_b"]
+        # The reported residual is the float64 residual of the returned x.
+        r = solver.b - solver.op.apply(solver.x)
+        assert abs(float(torch.linalg.norm(r)) - st["norm_r"]) <= 1e-3 * st["norm_r"] + 1e-12
+    with pytest.raises(ValueError):
+        solver.solve(mixed="other")
