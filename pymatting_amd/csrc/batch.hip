@@ -423,6 +423,11 @@ struct BatchPlan {
             size_t lds = 0;
             for (int i = 0; i < count; ++i) {
                 lds = std::max(lds, make_tail_params<T>(B->vc[i], B->tail[i], P[i]));
+                if (B->tail[i] != B->vc[i]->tail_start ||
+                    (P[i].pack_count > 0 && !P[i].pack)) {
+                    set_error("image %d: V-cycle tail without packed contents", i);
+                    return HM_ERR_ARG;
+                }
                 img[i] = (unsigned)i;
             }
             TailParams<T>* dP = nullptr;

@@ -78,6 +78,11 @@ struct hm_vc {
     // stays a symmetric preconditioner) instead of the dense inverse.
     int coarse_sweeps = 0;
     hm::Reducer red{};
+    // The tail's constant LDS contents (stencils of its LDS-resident grids,
+    // then the dense inverse when staged) packed in their LDS order, in this
+    // hierarchy's precision; built once for the tail starting at tail_start.
+    void* tail_pack = nullptr;
+    size_t tail_start = 0;
 };
 
 namespace hm {

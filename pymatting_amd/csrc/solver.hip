@@ -752,6 +752,7 @@ void vc_free(hm_vc* vc) {
     }
     dev_free(vc->dense_inv64);
     dev_free(vc->dense_inv32);
+    dev_free(vc->tail_pack);
     free_reducer(&vc->red);
     delete vc;
 }
@@ -899,6 +900,9 @@ int vc_create(hm_cf* cf, hm_dia* dia, int h, int w, int pre, int post, double om
         rc = dev_alloc(reinterpret_cast<void**>(&vc->dense_inv32), sizeof(float) * last->n * last->n);
     if (rc == HM_OK && last->n > 0) rc = dense_inverse(last, vc->dense_inv64, vc->dense_inv32, s);
     if (rc == HM_OK) rc = make_reducer(&vc->red, reducer_blocks(n0, h, w), 2);
+    if (rc == HM_OK && vc_tail_start(vc) < vc->levels.size() - 1)
+        rc = prec == HM_F64 ? build_tail_pack<double>(vc, vc_tail_start(vc), s)
+                            : build_tail_pack<float>(vc, vc_tail_start(vc), s);
     if (rc != HM_OK) { vc_free(vc); return rc; }
     HM_HIP(hipStreamSynchronize(s));
     *out = vc;

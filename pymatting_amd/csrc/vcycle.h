@@ -312,8 +312,9 @@ struct TailLevel {
     const T* coef;  // [13][n] symmetric storage (global)
     T* b;           // right-hand side (global; only the top level's is read)
     T* x;           // result (global; only the top level's is written)
-    int off;        // offset of this level's LDS block (in T units)
-    int coef_lds;   // 1: coefficients staged in LDS after the vectors
+    int off;        // LDS offset of this level's vectors (in T units)
+    int coef_lds;   // 1: coefficients staged in LDS at coef_off
+    int coef_off;
 };
 
 template <typename T>
@@ -324,11 +325,15 @@ struct TailParams {
     T omega;
     const T* inv;    // dense inverse of the last grid (global)
     int inv_off;     // LDS offset of the staged inverse, or -1 (read from global)
+    const T* pack;   // constant LDS contents [pack_off, pack_off + pack_count) (global)
+    int pack_off, pack_count;
+    int vec_end;     // vectors of every level occupy LDS [0, vec_end)
 };
 
-// LDS block of one level: padded vectors b, x, t, u of (h + 4) x (w + 4),
-// element (y, x) at [(y + 2) * pitch + x + 2], then [13][n] coefficients if
-// staged.
+// LDS layout: the padded vectors b, x, t, u of (h + 4) x (w + 4) of every
+// level (element (y, x) at [(y + 2) * pitch + x + 2]), top grid first; then
+// the [13][n] stencils of the levels that keep them in LDS and the dense
+// inverse, in the order of the hierarchy's packed copy.
 template <typename T>
 struct TailView {
     int h, w, n, pitch, psize;
@@ -354,7 +359,7 @@ __device__ __forceinline__ TailView<T> tail_view(T* lds, const TailLevel<T>& L) 
     v.x = v.b + v.psize;
     v.t = v.x + v.psize;
     v.u = v.t + v.psize;
-    v.coef = L.coef_lds ? static_cast<const T*>(v.u + v.psize) : L.coef;
+    v.coef = L.coef_lds ? static_cast<const T*>(lds + L.coef_off) : L.coef;
     return v;
 }
 
@@ -406,50 +411,39 @@ __device__ __forceinline__ void vc_tail_body(const TailParams<T>& P) {
                 tc[k][s] = top_reg ? sym_coef(V0.coef, (long long)V0.n, V0.w, ii, s) : T(0);
         }
     }
-    // Zero every vector (the halos must read as zero), then stage the top
-    // right-hand side, the LDS stencils and the inverse. kStage loads per
-    // thread are issued before the stores.
+    // Stage the top right-hand side (with its zero halo) and the packed
+    // stencils and inverse, and zero the other vectors (halos must read as
+    // zero; interiors are written before they are read). A round issues all
+    // of its global loads before its LDS stores, overlapping the register
+    // stencil's loads and the zero stores; the stores go to disjoint LDS
+    // ranges, so one barrier ends the phase.
     {
-        const TailLevel<T>& last = P.lv[nlev - 1];
-        const int end = last.off + 4 * tail_psize(last.h, last.w) +
-                        (last.coef_lds ? kSymSlots * last.h * last.w : 0);
-        for (int i = tid; i < end; i += kTailBlock) lds[i] = T(0);
-    }
-    __syncthreads();
-    {
-        constexpr int kStage = 8;
-        for (int l = 0; l <= nlev; ++l) {
-            const bool rhs = l == nlev;
-            const TailLevel<T>& L = P.lv[rhs ? 0 : l];
-            if (!rhs && !L.coef_lds) continue;
-            const TailView<T> V = tail_view(lds, L);
-            const T* src = rhs ? L.b : L.coef;
-            const int cnt = rhs ? V.n : kSymSlots * V.n;
-            T* dst = rhs ? V.b : const_cast<T*>(V.coef);
-            for (int i0 = 0; i0 < cnt; i0 += kTailBlock * kStage) {
-                T v[kStage];
+        constexpr int kStage = 16;
+        const int rhs_n = V0.psize;
+        const int total = rhs_n + P.pack_count;
+        for (int base = 0; base < total; base += kStage * kTailBlock) {
+            T v[kStage];
 #pragma unroll
-                for (int u = 0; u < kStage; ++u) {
-                    const int i = i0 + u * kTailBlock + tid;
-                    v[u] = i < cnt ? src[i] : T(0);
-                }
-#pragma unroll
-                for (int u = 0; u < kStage; ++u) {
-                    const int i = i0 + u * kTailBlock + tid;
-                    if (i >= cnt) continue;
-                    if (rhs) {
-                        const int y = i / V.w;
-                        dst[V.at(y, i - y * V.w)] = v[u];
-                    } else {
-                        dst[i] = v[u];
-                    }
+            for (int u = 0; u < kStage; ++u) {
+                const int g = base + u * kTailBlock + tid;
+                v[u] = T(0);
+                if (g < rhs_n) {
+                    const int y = g / V0.pitch - 2, x = g - (g / V0.pitch) * V0.pitch - 2;
+                    if (y >= 0 && y < V0.h && x >= 0 && x < V0.w) v[u] = P.lv[0].b[y * V0.w + x];
+                } else if (g < total) {
+                    v[u] = P.pack[g - rhs_n];
                 }
             }
-        }
-        if (P.inv_off >= 0) {
-            const TailLevel<T>& L = P.lv[nlev - 1];
-            const int nn = L.h * L.w * L.h * L.w;
-            for (int i = tid; i < nn; i += kTailBlock) lds[P.inv_off + i] = P.inv[i];
+            if (base == 0)
+                for (int i = rhs_n + tid; i < P.vec_end; i += kTailBlock) lds[i] = T(0);
+#pragma unroll
+            for (int u = 0; u < kStage; ++u) {
+                const int g = base + u * kTailBlock + tid;
+                if (g < rhs_n)
+                    lds[g] = v[u];
+                else if (g < total)
+                    lds[P.pack_off + g - rhs_n] = v[u];
+            }
         }
     }
     __syncthreads();
@@ -655,7 +649,7 @@ size_t make_tail_params(const hm_vc* vc, size_t start, TailParams<T>& P) {
         inv_staged = false;
     }
     int off = 0;
-    for (size_t l = start; l < L; ++l) {
+    for (size_t l = start; l < L; ++l) {  // vectors
         const VcLevel& V = vc->levels[l];
         TailLevel<T>& t = P.lv[l - start];
         t.h = V.h;
@@ -665,17 +659,56 @@ size_t make_tail_params(const hm_vc* vc, size_t start, TailParams<T>& P) {
         t.x = static_cast<T*>(V.x);
         t.off = off;
         t.coef_lds = (l > start || top_coef) ? 1 : 0;
-        off += 4 * tail_psize(V.h, V.w) + (t.coef_lds ? kSymSlots * (int)V.n : 0);
+        off += 4 * tail_psize(V.h, V.w);
+    }
+    P.vec_end = off;
+    P.pack_off = off;
+    for (size_t l = start; l < L; ++l) {  // staged stencils
+        TailLevel<T>& t = P.lv[l - start];
+        t.coef_off = t.coef_lds ? off : -1;
+        if (t.coef_lds) off += kSymSlots * (int)vc->levels[l].n;
     }
     P.inv_off = inv_staged ? off : -1;
     if (inv_staged) off += (int)(vc->levels[L - 1].n * vc->levels[L - 1].n);
+    P.pack_count = off - P.pack_off;
+    P.pack = static_cast<const T*>(vc->tail_pack);
     return (size_t)off * sizeof(T);
+}
+
+// Builds the hierarchy's packed tail contents for the tail at `start`.
+template <typename T>
+int build_tail_pack(hm_vc* vc, size_t start, hipStream_t s) {
+    TailParams<T> P;
+    make_tail_params<T>(vc, start, P);
+    dev_free(vc->tail_pack);
+    vc->tail_pack = nullptr;
+    vc->tail_start = start;
+    if (P.pack_count == 0) return HM_OK;
+    HM_TRY(dev_alloc(&vc->tail_pack, sizeof(T) * (size_t)P.pack_count));
+    T* pack = static_cast<T*>(vc->tail_pack);
+    for (int l = 0; l < P.nlev; ++l) {
+        const TailLevel<T>& t = P.lv[l];
+        if (!t.coef_lds) continue;
+        HM_HIP(hipMemcpyAsync(pack + (t.coef_off - P.pack_off), t.coef,
+                              sizeof(T) * kSymSlots * (size_t)t.h * t.w,
+                              hipMemcpyDeviceToDevice, s));
+    }
+    if (P.inv_off >= 0) {
+        const size_t nn = (size_t)vc->levels.back().n * vc->levels.back().n;
+        HM_HIP(hipMemcpyAsync(pack + (P.inv_off - P.pack_off), P.inv, sizeof(T) * nn,
+                              hipMemcpyDeviceToDevice, s));
+    }
+    return HM_OK;
 }
 
 template <typename T>
 int launch_tail(hm_vc* vc, size_t start, const CgState* guard, hipStream_t s) {
     TailParams<T> P;
     const size_t lds = make_tail_params<T>(vc, start, P);
+    if (start != vc->tail_start || (P.pack_count > 0 && !P.pack)) {
+        set_error("V-cycle tail at level %zu has no packed contents", start);
+        return HM_ERR_ARG;
+    }
     static bool attr_set[2] = {false, false};
     if (!attr_set[sizeof(T) == 4]) {
         HM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vc_tail<T>),
