@@ -214,6 +214,119 @@ __global__ void k_assemble_exact(const double* __restrict__ I, const double* __r
     }
 }
 
+// The stored symmetric form (slots 12..24, planes [13][n]) with its float32
+// copy, diagonal and inverse diagonal, per tile of kAsmX x kAsmY pixels. The
+// window data (flag, mean, inverse covariance) of the tile's windows and the
+// pixels of its 5 x 5 neighbourhoods are staged in LDS; each pixel i then
+// forms c_k = M_k (I_i - mu_k) once for each of its nine windows and sums
+// L_ij = sum_k [delta_ij - (1 + c_k . (I_j - mu_k)) / 9] over the windows
+// containing both pixels in raster order of the centre, with the operations
+// of entry_exact, so every entry is the same double.
+constexpr int kAsmX = 64, kAsmY = 4;
+constexpr int kAsmWX = kAsmX + 2, kAsmWY = kAsmY + 2;  // window centres
+constexpr int kAsmIX = kAsmX + 4, kAsmIY = kAsmY + 4;  // pixels
+
+__global__ __launch_bounds__(kAsmX * kAsmY) void k_assemble_sym(
+    const double* __restrict__ I, const double* __restrict__ wc, int h, int w,
+    const uint8_t* __restrict__ cls, int mode, double lambda, double* __restrict__ c64,
+    float* __restrict__ c32, double* __restrict__ d64, float* __restrict__ d32,
+    double* __restrict__ id64, float* __restrict__ id32) {
+    __shared__ double sw[10][kAsmWY * kAsmWX];  // flag, mu0..2, m00 m01 m02 m11 m12 m22
+    __shared__ double si[3][kAsmIY * kAsmIX];
+    const long long n = (long long)h * w;
+    const int x0 = blockIdx.x * kAsmX, y0 = blockIdx.y * kAsmY;
+    const int tid = threadIdx.x;
+    for (int t = tid; t < kAsmWY * kAsmWX; t += kAsmX * kAsmY) {
+        const int ky = y0 - 1 + t / kAsmWX, kx = x0 - 1 + t % kAsmWX;
+        const bool in = ky >= 0 && ky < h && kx >= 0 && kx < w;
+        const long long k = in ? (long long)ky * w + kx : 0;
+        sw[0][t] = in ? wc[9 * n + k] : 0.0;  // outside the image: no window
+#pragma unroll
+        for (int q = 0; q < 9; ++q) sw[1 + q][t] = in ? wc[q * n + k] : 0.0;
+    }
+    for (int t = tid; t < kAsmIY * kAsmIX; t += kAsmX * kAsmY) {
+        const int y = y0 - 2 + t / kAsmIX, x = x0 - 2 + t % kAsmIX;
+        const bool in = y >= 0 && y < h && x >= 0 && x < w;
+        const long long q = in ? (long long)y * w + x : 0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) si[c][t] = in ? I[c * n + q] : 0.0;
+    }
+    __syncthreads();
+    const int tx = tid % kAsmX, ty = tid / kAsmX;
+    const int xi = x0 + tx, yi = y0 + ty;
+    if (xi >= w || yi >= h) return;
+    const long long i = (long long)yi * w + xi;
+    const int ib = (ty + 2) * kAsmIX + tx + 2;
+    const double Ii0 = si[0][ib], Ii1 = si[1][ib], Ii2 = si[2][ib];
+    // Window (a, b) is centred at (yi + a - 1, xi + b - 1). Windows outside
+    // rows/columns 1..h-2 / 1..w-2 have flag 0 in wc.
+    double cw[9][3], mw[9][3];
+    bool vw[9];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const int q = a * 3 + b;
+            const int wi = (ty + a) * kAsmWX + tx + b;
+            vw[q] = sw[0][wi] != 0.0;
+            const double mu0 = sw[1][wi], mu1 = sw[2][wi], mu2 = sw[3][wi];
+            const double m00 = sw[4][wi], m01 = sw[5][wi], m02 = sw[6][wi];
+            const double m11 = sw[7][wi], m12 = sw[8][wi], m22 = sw[9][wi];
+            const double s = Ii0 - mu0, tt = Ii1 - mu1, u = Ii2 - mu2;
+            cw[q][0] = m00 * s + m01 * tt + m02 * u;
+            cw[q][1] = m01 * s + m11 * tt + m12 * u;
+            cw[q][2] = m02 * s + m12 * tt + m22 * u;
+            mw[q][0] = mu0;
+            mw[q][1] = mu1;
+            mw[q][2] = mu2;
+        }
+    }
+    const bool known_i = cls && cls[i] != 0;
+#pragma unroll
+    for (int s = 12; s < 25; ++s) {
+        const int dy = s / 5 - 2, dx = s % 5 - 2;
+        const int yj = yi + dy, xj = xi + dx;
+        double v = 0.0;
+        if (yj < h && xj >= 0 && xj < w) {  // yj >= yi >= 0
+            const long long j = (long long)yj * w + xj;
+            const bool known_j = cls && cls[j] != 0;
+            if (mode == HM_OP_MASKED && known_i) {
+                v = (s == 12) ? 1.0 : 0.0;
+            } else if (mode == HM_OP_MASKED && known_j) {
+                v = 0.0;
+            } else {
+                const int jb = (ty + 2 + dy) * kAsmIX + tx + 2 + dx;
+                const double Ij0 = si[0][jb], Ij1 = si[1][jb], Ij2 = si[2][jb];
+                const double delta = (s == 12) ? 1.0 : 0.0;
+                double value = 0.0;
+#pragma unroll
+                for (int a = (dy > 0 ? dy : 0); a <= (dy < 0 ? dy : 0) + 2; ++a) {
+#pragma unroll
+                    for (int b = (dx > 0 ? dx : 0); b <= (dx < 0 ? dx : 0) + 2; ++b) {
+                        const int q = a * 3 + b;
+                        if (!vw[q]) continue;
+                        const double temp = cw[q][0] * (Ij0 - mw[q][0]) +
+                                            cw[q][1] * (Ij1 - mw[q][1]) +
+                                            cw[q][2] * (Ij2 - mw[q][2]);
+                        value += delta - (1 + temp) / 9;
+                    }
+                }
+                v = value;
+                if (mode == HM_OP_LAMBDA && s == 12 && known_i) v = v + lambda;
+            }
+        }
+        const long long o = (long long)(s - 12) * n + i;
+        c64[o] = v;
+        c32[o] = (float)v;
+        if (s == 12) {
+            d64[i] = v;
+            d32[i] = (float)v;
+            id64[i] = 1.0 / v;
+            id32[i] = (float)(1.0 / v);
+        }
+    }
+}
+
 // Diagonal of the mode-adjusted operator (float64 and float32 copies).
 __global__ void k_diag_exact(const double* __restrict__ I, const double* __restrict__ wc, int h,
                              int w, const uint8_t* __restrict__ cls, int mode, double lambda,
@@ -335,11 +448,26 @@ int cf_exact_setup(hm_cf* op, hipStream_t s) {
     k_window_exact<<<g, 256, 0, s>>>(op->I64, op->h, op->w, op->eps, op->cls, op->skip_known,
                                      op->wc);
     HM_LAUNCH_CHECK();
-    k_diag_exact<<<g, 256, 0, s>>>(op->I64, op->wc, op->h, op->w, op->cls, op->mode, op->lambda,
-                                   op->diag64, op->diag32, op->idiag64, op->idiag32);
-    HM_LAUNCH_CHECK();
-    k_planar_to_f32<<<div_up(9 * n, 256), 256, 0, s>>>(op->wc, op->wc32, 9 * n);
-    HM_LAUNCH_CHECK();
+    op->ready = 0;
+    return HM_OK;
+}
+
+int cf_prepare(hm_cf* op, int need, hipStream_t s) {
+    const long long n = op->n;
+    if ((need & kCfDiag) && !(op->ready & kCfDiag)) {
+        k_diag_exact<<<div_up(n, 256), 256, 0, s>>>(op->I64, op->wc, op->h, op->w, op->cls,
+                                                    op->mode, op->lambda, op->diag64, op->diag32,
+                                                    op->idiag64, op->idiag32);
+        HM_LAUNCH_CHECK();
+        op->ready |= kCfDiag;
+    }
+    if ((need & kCfF32) && !(op->ready & kCfF32)) {
+        k_planar_to_f32<<<div_up(9 * n, 256), 256, 0, s>>>(op->wc, op->wc32, 9 * n);
+        HM_LAUNCH_CHECK();
+        k_planar_to_f32<<<div_up(3 * n, 256), 256, 0, s>>>(op->I64, op->I32, 3 * n);
+        HM_LAUNCH_CHECK();
+        op->ready |= kCfF32;
+    }
     return HM_OK;
 }
 
@@ -347,8 +475,7 @@ int cf_upload_image(const double* hwc, hm_cf* op, hipStream_t s) {
     const long long n = op->n;
     k_hwc_to_planar_f64<<<div_up(n, 256), 256, 0, s>>>(hwc, op->I64, n);
     HM_LAUNCH_CHECK();
-    k_planar_to_f32<<<div_up(3 * n, 256), 256, 0, s>>>(op->I64, op->I32, 3 * n);
-    HM_LAUNCH_CHECK();
+    op->ready = 0;
     return HM_OK;
 }
 
@@ -361,6 +488,19 @@ int cf_assemble(hm_cf* op, double* values, long long stride_i, long long stride_
     k_assemble_exact<<<div_up(op->n, 256), 256, 0, s>>>(op->I64, op->wc, op->h, op->w, op->cls,
                                                          mode, op->lambda, values, stride_i,
                                                          stride_s, s_lo, indices, indptr);
+    HM_LAUNCH_CHECK();
+    return HM_OK;
+}
+
+int cf_assemble_sym(hm_cf* op, hm_dia* A, hipStream_t s) {
+    if (A->h != op->h || A->w != op->w) {
+        set_error("internal: stored operator shape mismatch");
+        return HM_ERR_ARG;
+    }
+    const dim3 grid(div_up(op->w, kAsmX), div_up(op->h, kAsmY));
+    k_assemble_sym<<<grid, kAsmX * kAsmY, 0, s>>>(op->I64, op->wc, op->h, op->w, op->cls,
+                                                   op->mode, op->lambda, A->c64, A->c32,
+                                                   A->diag64, A->diag32, A->idiag64, A->idiag32);
     HM_LAUNCH_CHECK();
     return HM_OK;
 }

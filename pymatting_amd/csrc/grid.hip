@@ -110,33 +110,46 @@ void free_reducer(Reducer* r) {
 // Pairs (e_a, e_b) of fine offsets around coincident coarse points that keep
 // the fine coupling b - a = 2 D + e_b - e_a inside the 5-point reach, for a
 // coarse offset D in {-2..2}. Counts per D: 1, 6, 9, 6, 1.
-struct PairTable {
-    signed char ea[5][9], eb[5][9];
-    int cnt[5];
-};
+__device__ __forceinline__ double w1(int e) { return e == 0 ? 0.5 : 0.25; }
 
-__constant__ PairTable c_pairs;
-
-static PairTable build_pairs() {
-    PairTable t;
-    std::memset(&t, 0, sizeof(t));
-    for (int D = -2; D <= 2; ++D) {
-        int k = 0;
-        for (int ea = -1; ea <= 1; ++ea)
-            for (int eb = -1; eb <= 1; ++eb) {
-                int d = 2 * D + eb - ea;
-                if (d >= -2 && d <= 2) {
-                    t.ea[D + 2][k] = (signed char)ea;
-                    t.eb[D + 2][k] = (signed char)eb;
-                    ++k;
+// (R A P)_{I,J} for J = I + (Dy, Dx) with the bilinear prolongation P (weights
+// w1(ey) w1(ex) from fine point 2I + e to coarse point I) and R = P^T: the sum
+// over fine pairs a = 2I + ea, b = 2J + eb with b - a inside the 5 x 5 stencil,
+// ea (then eb) ascending per dimension, rows outer. Compile-time offsets and
+// clamped (always valid) addresses with out-of-grid pairs selected away let
+// every gather of a point issue at once.
+template <int Dy, int Dx>
+__device__ __forceinline__ double galerkin_entry(const double* __restrict__ Af, long long nf,
+                                                 int hf, int wf, int Y, int X) {
+    double sum = 0.0;
+#pragma unroll
+    for (int eay = -1; eay <= 1; ++eay) {
+#pragma unroll
+        for (int eby = -1; eby <= 1; ++eby) {
+            const int dyf = 2 * Dy + eby - eay;
+            if (dyf < -2 || dyf > 2) continue;
+            const int ay = 2 * Y + eay, by = 2 * (Y + Dy) + eby;
+            const bool oky = ay >= 0 && ay < hf && by >= 0 && by < hf;
+            const int ayc = ay < 0 ? 0 : (ay >= hf ? hf - 1 : ay);
+            const double wy = w1(eay) * w1(eby);
+#pragma unroll
+            for (int eax = -1; eax <= 1; ++eax) {
+#pragma unroll
+                for (int ebx = -1; ebx <= 1; ++ebx) {
+                    const int dxf = 2 * Dx + ebx - eax;
+                    if (dxf < -2 || dxf > 2) continue;
+                    const int ax = 2 * X + eax, bx = 2 * (X + Dx) + ebx;
+                    const bool ok = oky && ax >= 0 && ax < wf && bx >= 0 && bx < wf;
+                    const int axc = ax < 0 ? 0 : (ax >= wf ? wf - 1 : ax);
+                    const double v =
+                        sym_coef(Af, nf, wf, (long long)ayc * wf + axc, (dyf + 2) * 5 + dxf + 2);
+                    sum += ok ? wy * (w1(eax) * w1(ebx)) * v : 0.0;
                 }
             }
-        t.cnt[D + 2] = k;
+        }
     }
-    return t;
+    return sum;
 }
-
-__device__ __forceinline__ double w1(int e) { return e == 0 ? 0.5 : 0.25; }
 
 // One thread per (coarse point, stored coarse slot s >= 12); both operators
 // use the symmetric storage of engine.h.
@@ -151,33 +164,19 @@ __global__ void k_galerkin(const double* __restrict__ Af, int hf, int wf,
     const int Dy = s / 5 - 2, Dx = s % 5 - 2;
     double sum = 0.0;
     if (Y + Dy >= 0 && Y + Dy < hc && X + Dx >= 0 && X + Dx < wc) {
-        const int ny = c_pairs.cnt[Dy + 2], nx = c_pairs.cnt[Dx + 2];
-        for (int py = 0; py < ny; ++py) {
-            const int eay = c_pairs.ea[Dy + 2][py], eby = c_pairs.eb[Dy + 2][py];
-            const int ay = 2 * Y + eay, by = 2 * (Y + Dy) + eby;
-            if (ay < 0 || ay >= hf || by < 0 || by >= hf) continue;
-            const double wy = w1(eay) * w1(eby);
-            const int sy = by - ay + 2;
-            for (int px = 0; px < nx; ++px) {
-                const int eax = c_pairs.ea[Dx + 2][px], ebx = c_pairs.eb[Dx + 2][px];
-                const int ax = 2 * X + eax, bx = 2 * (X + Dx) + ebx;
-                if (ax < 0 || ax >= wf || bx < 0 || bx >= wf) continue;
-                const int sx = bx - ax + 2;
-                const long long a = (long long)ay * wf + ax;
-                sum += wy * (w1(eax) * w1(ebx)) * sym_coef(Af, nf, wf, a, sy * 5 + sx);
-            }
+        switch (s) {
+#define HM_GAL(S_) \
+    case S_: sum = galerkin_entry<S_ / 5 - 2, S_ % 5 - 2>(Af, nf, hf, wf, Y, X); break;
+            HM_GAL(12) HM_GAL(13) HM_GAL(14) HM_GAL(15) HM_GAL(16) HM_GAL(17) HM_GAL(18)
+            HM_GAL(19) HM_GAL(20) HM_GAL(21) HM_GAL(22) HM_GAL(23) HM_GAL(24)
+#undef HM_GAL
+            default: break;
         }
     }
     Ac[t] = sum;
 }
 
 int dia_galerkin(const hm_dia* fine, hm_dia* coarse, hipStream_t s) {
-    static bool pairs_ready = false;
-    if (!pairs_ready) {
-        PairTable t = build_pairs();
-        HM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_pairs), &t, sizeof(t)));
-        pairs_ready = true;
-    }
     const long long total = kSymSlots * coarse->n;
     k_galerkin<<<div_up(total, 256), 256, 0, s>>>(fine->c64, fine->h, fine->w, coarse->c64,
                                                   coarse->h, coarse->w);
@@ -327,11 +326,85 @@ __global__ __launch_bounds__(1024) void k_gauss_jordan(int n, double* __restrict
     }
 }
 
+// The same elimination for n <= kGjLds with [A | I] held in LDS: one wave
+// finds the pivot (one row per lane, ties to the lower row), three barriers
+// per column.
+constexpr int kGjLds = 64;
+
+__global__ __launch_bounds__(256) void k_gauss_jordan_lds(int n, const double* __restrict__ M,
+                                                          double* __restrict__ inv64,
+                                                          float* __restrict__ inv32,
+                                                          int* __restrict__ status) {
+    extern __shared__ double S[];  // n x 2n
+    __shared__ double fac[kGjLds];
+    __shared__ int piv_s;
+    const int n2 = 2 * n, tid = threadIdx.x;
+    for (int t = tid; t < n * n2; t += blockDim.x) S[t] = M[t];
+    __syncthreads();
+    for (int k = 0; k < n; ++k) {
+        if (tid < 64) {
+            const int r = k + tid;
+            double v = r < n ? fabs(S[r * n2 + k]) : -1.0;
+            if (!(v >= 0.0)) v = -1.0;  // NaN never pivots
+            int idx = r;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ov = __shfl_xor(v, off);
+                const int oi = __shfl_xor(idx, off);
+                if (ov > v || (ov == v && oi < idx)) {
+                    v = ov;
+                    idx = oi;
+                }
+            }
+            if (tid == 0) {
+                piv_s = idx;
+                if (!(v > 0.0)) *status = 1;
+            }
+        }
+        __syncthreads();
+        const int piv = piv_s < n ? piv_s : k;
+        if (piv != k)
+            for (int c = tid; c < n2; c += blockDim.x) {
+                const double t0 = S[k * n2 + c];
+                S[k * n2 + c] = S[piv * n2 + c];
+                S[piv * n2 + c] = t0;
+            }
+        __syncthreads();
+        const double inv_p = 1.0 / S[k * n2 + k];
+        if (tid < n) fac[tid] = S[tid * n2 + k];
+        __syncthreads();
+        for (int c = tid; c < n2; c += blockDim.x) S[k * n2 + c] *= inv_p;
+        __syncthreads();
+        for (int t = tid; t < n * n2; t += blockDim.x) {
+            const int r = t / n2, c = t - r * n2;
+            if (r != k) S[t] -= fac[r] * S[k * n2 + c];
+        }
+        __syncthreads();
+    }
+    for (int t = tid; t < n * n; t += blockDim.x) {
+        const int r = t / n, c = t - r * n;
+        const double v = S[r * n2 + n + c];
+        inv64[t] = v;
+        inv32[t] = (float)v;
+    }
+}
+
 static int gauss_jordan(int n, double* M, double* inv64, float* inv32, hipStream_t s) {
     int* status = nullptr;
     HM_TRY(dev_alloc(reinterpret_cast<void**>(&status), sizeof(int)));
     HM_HIP(hipMemsetAsync(status, 0, sizeof(int), s));
-    k_gauss_jordan<<<1, 1024, 0, s>>>(n, M, inv64, inv32, status);
+    if (n <= kGjLds) {
+        static bool attr_set = false;
+        if (!attr_set) {
+            HM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gauss_jordan_lds),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)sizeof(double) * 2 * kGjLds * kGjLds));
+            attr_set = true;
+        }
+        k_gauss_jordan_lds<<<1, 256, sizeof(double) * 2 * n * n, s>>>(n, M, inv64, inv32, status);
+    } else {
+        k_gauss_jordan<<<1, 1024, 0, s>>>(n, M, inv64, inv32, status);
+    }
     HM_LAUNCH_CHECK();
     int host_status = 0;
     HM_HIP(hipMemcpyAsync(&host_status, status, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -23,6 +23,7 @@ struct hm_cf {
     float* diag32 = nullptr;
     double* idiag64 = nullptr;  // 1 / diag (float32 copy rounded from float64)
     float* idiag32 = nullptr;
+    int ready = 0;  // kCf* parts computed so far (cf_prepare)
 };
 
 // Stored symmetric 25-point stencil (slot-major [13][n], see engine.h) with
@@ -90,8 +91,16 @@ namespace hm {
 // cf_exact.hip
 int cf_upload_image(const double* hwc, hm_cf* op, hipStream_t s);
 int cf_exact_setup(hm_cf* op, hipStream_t s);
+// Parts of a matrix-free operator computed on first need: the diagonal and
+// its inverse (Jacobi smoothing, jacobi()), and the float32 copies of the
+// image and window data (float32 stencil passes).
+constexpr int kCfDiag = 1, kCfF32 = 2;
+int cf_prepare(hm_cf* op, int need, hipStream_t s);
 int cf_assemble(hm_cf* op, double* values, long long stride_i, long long stride_s, int s_lo,
                 int64_t* indices, int64_t* indptr, int mode, hipStream_t s);
+// The stored symmetric stencil of op's mode-adjusted operator with its
+// float32 copy, diagonal and inverse diagonal (A allocated at op's shape).
+int cf_assemble_sym(hm_cf* op, hm_dia* A, hipStream_t s);
 int cf_rhs(hm_cf* op, double* b, hipStream_t s);
 int cf_system_counts(hm_cf* op, int64_t* counts, hipStream_t s);
 int cf_system_fill(hm_cf* op, const int64_t* indptr, int64_t* indices, double* values,

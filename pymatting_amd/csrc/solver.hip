@@ -221,9 +221,17 @@ inline int host_call(hm_apply_cb cb, void* ctx, const double* x, double* y, hipS
     return HM_OK;
 }
 
+// float32 passes of a matrix-free operator read its float32 copies.
+template <typename T>
+inline int cf_ready(const OpHandle& A, hipStream_t s) {
+    if (A.cf && sizeof(T) == 4) return cf_prepare(const_cast<hm_cf*>(A.cf), kCfF32, s);
+    return HM_OK;
+}
+
 template <typename T>
 int op_apply(const OpHandle& A, const T* x, T* y, const DotSink& ds, const CgState* guard,
              hipStream_t s) {
+    HM_TRY(cf_ready<T>(A, s));
     if (A.cf) return run_level<T>(cf_level<T>(A.cf), InVec<T>{x}, OutApply<T>{y, ds}, guard, s);
     if (A.dia)
         return run_level<T>(dia_level<T>(A.dia), InVec<T>{x}, OutApply<T>{y, ds}, guard, s);
@@ -250,6 +258,7 @@ int op_apply_pupdate(const OpHandle& A, const T* z, const T* p_old, T* p_new, T*
                      const DotSink& ds, CgState* st, hipStream_t s) {
     const InPupdate<T> in{z, p_old, st};
     OutApply<T> out{q, ds, p_new};
+    HM_TRY(cf_ready<T>(A, s));
     if (A.cf) return run_level<T>(cf_level<T>(A.cf), in, out, st, s);
     if (A.dia) return run_level<T>(dia_level<T>(A.dia), in, out, st, s);
     const long long n = A.n();
@@ -270,6 +279,7 @@ __global__ void k_sub_from(const double* __restrict__ b, double* __restrict__ r,
 template <typename T>
 int op_residual(const OpHandle& A, const T* b, const T* x, T* r, hipStream_t s) {
     const DotSink nodot{nullptr, Reducer{}, 0};
+    HM_TRY(cf_ready<T>(A, s));
     if (A.cf)
         return run_level<T>(cf_level<T>(A.cf), InVec<T>{x}, OutResidual<T>{b, r, nodot}, nullptr,
                             s);
@@ -835,7 +845,8 @@ int vc_create(hm_cf* cf, hm_dia* dia, int h, int w, int pre, int post, double om
     } else {
         vc->cf = cf;
         L0.A = dia;  // null when matrix-free
-        if (pre >= 2 || post >= 2) {
+        if (cf) rc = cf_prepare(cf, kCfDiag | (prec == HM_F32 ? kCfF32 : 0), s);
+        if (rc == HM_OK && (pre >= 2 || post >= 2)) {
             rc = dev_alloc(&L0.x, es * n0);
             if (rc == HM_OK) rc = dev_alloc(&L0.t, es * n0);
             if (rc == HM_OK) rc = dev_alloc(&L0.u, es * n0);
@@ -1029,6 +1040,7 @@ int hm_cf_diagonal(hm_cf* op, double* d, void* stream) {
         set_error("hm_cf_diagonal: null argument");
         return HM_ERR_ARG;
     }
+    HM_TRY(cf_prepare(op, kCfDiag, static_cast<hipStream_t>(stream)));
     HM_HIP(hipMemcpyAsync(d, op->diag64, sizeof(double) * op->n, hipMemcpyDeviceToDevice,
                           static_cast<hipStream_t>(stream)));
     return HM_OK;
@@ -1065,8 +1077,7 @@ int hm_dia_from_cf(hm_cf* op, void* stream, hm_dia** out) {
     hm_dia* A = nullptr;
     HM_TRY(dia_alloc(op->h, op->w, &A));
     // The 13 stored slots (dy, dx) >= (0, 0), assembled exactly in place.
-    int rc = cf_assemble(op, A->c64, 1, A->n, 12, nullptr, nullptr, op->mode, s);
-    if (rc == HM_OK) rc = dia_finish(A, s);
+    int rc = cf_assemble_sym(op, A, s);
     if (rc != HM_OK) {
         hm_dia_destroy(A);
         return rc;

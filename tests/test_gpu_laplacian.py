@@ -103,3 +103,62 @@ def test_masked_rhs_exact():
     b = op.rhs().cpu().numpy()
     assert np.array_equal(b[unknown], ref)
     assert np.all(b[known] == 0)
+
+
+def _slot_array(A, h, w):
+    """[25][n] stencil array of a CSR matrix on an (h, w) grid (oracle side).
+    The reference's CSR keeps 25 entries per row, out-of-image slots as
+    explicit zeros in column 0; zeros are dropped here."""
+    A = scipy.sparse.coo_matrix(A)
+    keep = A.data != 0
+    A = scipy.sparse.coo_matrix((A.data[keep], (A.row[keep], A.col[keep])), shape=A.shape)
+    n = h * w
+    dy = A.col // w - A.row // w
+    dx = A.col % w - A.row % w
+    assert np.all(np.abs(dy) <= 2) and np.all(np.abs(dx) <= 2)
+    coef = np.zeros((25, n))
+    coef[(dy + 2) * 5 + dx + 2, A.row] = A.data
+    return coef
+
+
+@pytest.mark.parametrize("case", list(range(7)))
+def test_stored_stencil_bit_exact(case):
+    """The stored 13-slot form assembled from the window data equals the
+    oracle's CSR entries of those slots bit for bit in every operator mode: the Laplacian,
+    L + lambda diag(known) of make_linear_system, and the masked system
+    [L_UU 0; 0 I] of estimate_alpha_cf (odd sizes cross the tile edges)."""
+    import torch
+    from pymatting_amd.laplacian import CfOperator, DiaOperator
+    from pymatting_amd.util import device_classes
+    name, image, _ = list(_images())[case]
+    h, w = image.shape[:2]
+    rng = np.random.default_rng(case)
+    trimap = rng.choice([0.0, 0.5, 1.0], size=(h, w), p=[0.3, 0.4, 0.3])
+    trimap.flat[0], trimap.flat[-1] = 0.0, 1.0  # both known classes present
+    print("case", name, image.shape, flush=True)
+    L = ocf.cf_laplacian(image)
+    A, _ = ocf.make_linear_system(L, trimap)
+    known = ocf.trimap_split(trimap)[2]
+    M = scipy.sparse.lil_matrix(L)
+    M[known, :] = 0
+    M[:, known] = 0
+    M = scipy.sparse.csr_matrix(M) + scipy.sparse.diags(known.astype(np.float64))
+    classes, tdev = device_classes(torch.from_numpy(trimap).cuda())
+    for mode, ref in (("laplacian", L), ("lambda", A), ("masked", M)):
+        kw = {} if mode == "laplacian" else dict(classes=classes, trimap=tdev)
+        cf = CfOperator(image, mode=mode, lambda_value=100.0 if mode == "lambda" else 0.0, **kw)
+        coef = DiaOperator.from_cf(cf).coef.cpu().numpy()
+        expect = _slot_array(ref, h, w)
+        # Stored slots 12..24 ((dy, dx) >= (0, 0)) are bit-exact; the other
+        # twelve mirror them: slot s at i is the reference's L_ij for
+        # j = i + (dy, dx), i.e. slot 24 - s at j (the reference's own L_ji
+        # rounds differently).
+        assert np.array_equal(coef[12:], expect[12:]), \
+            (mode, np.abs(coef[12:] - expect[12:]).max())
+        mirror = np.zeros((12, h * w))
+        for s in range(12):
+            dy, dx = s // 5 - 2, s % 5 - 2
+            for y in range(max(0, -dy), min(h, h - dy)):
+                for x in range(max(0, -dx), min(w, w - dx)):
+                    mirror[s, y * w + x] = expect[24 - s, (y + dy) * w + x + dx]
+        assert np.array_equal(coef[:12], mirror), mode

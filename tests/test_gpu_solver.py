@@ -131,3 +131,24 @@ def test_dia_operator_and_csr_paths_agree():
     x1 = pm.cg(A, b, M=pm.vcycle(A, trimap.shape), rtol=1e-10)
     x2 = pm.cg(plain, b, M=pm.vcycle(plain, trimap.shape), rtol=1e-10)
     assert np.abs(x1 - x2).max() < 1e-8
+
+
+@pytest.mark.parametrize("n", [1, 7, 28, 64, 65, 100])
+def test_direct_coarse_inverse(n):
+    """The coarsest-grid dense inverse (Gauss-Jordan with partial pivoting;
+    LDS-resident up to 64 unknowns, global memory above): M(A x) = x for a
+    random SPD matrix solved directly (vcycle.py:59-60), and a singular
+    matrix is reported."""
+    from pymatting_amd.preconditioner import VCycle
+    rng = np.random.default_rng(n)
+    B = rng.standard_normal((n, n))
+    A = B @ B.T + n * np.eye(n)
+    x = rng.standard_normal(n)
+    M = VCycle(A, (1, n), direct_solve_size=n)
+    got = M(A @ x)
+    assert np.abs(got - x).max() <= 1e-10 * np.abs(x).max()
+    S = A.copy()  # a zero last row and column stay exactly zero
+    S[:, -1] = 0.0
+    S[-1, :] = 0.0
+    with pytest.raises(ValueError):
+        VCycle(S, (1, n), direct_solve_size=n)
