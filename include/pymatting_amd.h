@@ -192,7 +192,9 @@ int hm_cg_solve_mixed(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, doub
  * accumulated in float64, and the recursive residual replaced by the true
  * float64 residual b - A x whenever it has dropped by a factor delta since
  * the last replacement; convergence (||b - A x|| < rtol ||b||) is confirmed
- * on a true residual. Same operator arguments as hm_cg_solve_mixed. */
+ * on a true residual. The state is read back every min(check_every, 8)
+ * iterations (replacements are decided there). Same operator arguments as
+ * hm_cg_solve_mixed. */
 int hm_cg_solve_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x,
                          double rtol, double delta, int maxiter, int check_every,
                          hm_cg_stats* stats, void* stream);

@@ -600,6 +600,11 @@ __global__ __launch_bounds__(kBlock) void k_update_x64(double* __restrict__ x,
     block_partials<kBlock, 1>(v, red);
 }
 
+// Records ||r||^2 of a replacement iteration (the true residual's).
+__global__ void k_note_rr(CgState* st) {
+    if (!solve_done(st)) st->aux = st->rr;
+}
+
 // r32 = (float)r64 with ||r64||^2 partials (replacing the update's).
 __global__ __launch_bounds__(kBlock) void k_replace_r(const double* __restrict__ r64,
                                                       float* __restrict__ r32, long long n,
@@ -676,17 +681,25 @@ int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x
                            s));
     float* p_cur = p.as<float>();
     float* p_next = p2.as<float>();
-    const int K = check_every < 1 ? 8 : check_every;
+    // The state is read back every K iterations. A replacement is placed at
+    // the iteration where the residual norm is predicted (from the rate of
+    // the last batch) to have dropped by delta since the last replacement;
+    // deciding this only at read-backs far apart lets the float32 recursion
+    // drift from the true residual and stall the convergence.
+    constexpr int kReliableBatch = 8;
+    const int K = check_every < 1 ? kReliableBatch : std::min(check_every, kReliableBatch);
     const DotSink pq{st.dev, red, kDotPQ};
     double rr_replaced = norm_b * norm_b;  // ||r||^2 at the last replacement
-    bool replace_next = false;             // replace in the last iteration of the next batch
+    double rr_prev = rr_replaced, rho2 = 0.0;  // rate: ||r||^2 ratio per iteration
+    int it_prev = 0;   // iterations at the last read-back
+    int rep_idx = -1;  // replacement index in the next batch (none before a rate is known)
     int enqueued = 0;
-    bool last_replaced = false;
     while (true) {
         const int k = std::min(K, maxiter - enqueued);
         if (k <= 0) break;
+        const int rep_at = rep_idx < k ? rep_idx : -1;
         for (int it = 0; it < k; ++it) {
-            const bool replace = replace_next && it == k - 1;
+            const bool replace = it == rep_at;
             HM_TRY(op_apply_pupdate<float>(A, z.as<float>(), p_cur, p_next, q.as<float>(), pq,
                                            st.dev, s));
             std::swap(p_cur, p_next);
@@ -705,8 +718,12 @@ int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x
             }
             HM_TRY(pc_apply<float>(M, r.as<float>(), z.as<float>(), st.dev, red, kDotUpdateRz,
                                    st.dev, n, s));
-            last_replaced = replace;
+            if (replace) {
+                k_note_rr<<<1, 1, 0, s>>>(st.dev);
+                HM_LAUNCH_CHECK();
+            }
         }
+        const int rep_iter = rep_at >= 0 ? enqueued + rep_at + 1 : -1;
         enqueued += k;
         HM_TRY(st.pull(s));
         if (st.host->nonfinite) {
@@ -714,8 +731,10 @@ int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x
             return HM_ERR_VALUE;
         }
         if (st.host->done) {
-            if (last_replaced && st.host->iters == enqueued) break;  // tested on a true residual
-            // Confirm on the true residual; continue if the recursion drifted.
+            if (st.host->iters == rep_iter) break;  // tested on a true residual
+            // Confirm on the true residual. If the recursion drifted, restart
+            // CG from the true residual (beta = 0): the old direction belongs
+            // to a residual that may be far smaller than the true one.
             HM_TRY(op_residual<double>(A, b, x, r64.as<double>(), s));
             HM_TRY(launch_dot<double>(r64.as<double>(), r64.as<double>(), nullptr, n, st.dev,
                                       red, kDotAux, s));
@@ -724,20 +743,34 @@ int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x
             if (std::sqrt(rr_true) < rtol * norm_b) break;
             st.host->done = 0;
             st.host->rr = rr_true;
+            st.host->beta = 0.0;
             HM_TRY(st.push(s));
             k_replace_r<<<vb, kBlock, 0, s>>>(r64.as<double>(), r.as<float>(), n, nullptr,
                                               red_upd);
             HM_LAUNCH_CHECK();
-            // The same CG continues from the true residual (beta from the new <r, z>).
-            HM_TRY(pc_apply<float>(M, r.as<float>(), z.as<float>(), st.dev, red, kDotRzUpdate,
-                                   st.dev, n, s));
+            HM_TRY(pc_apply<float>(M, r.as<float>(), z.as<float>(), st.dev, red, kDotRzFirst,
+                                   nullptr, n, s));
             enqueued = st.host->iters;
-            rr_replaced = rr_true;
-            replace_next = false;
+            rr_replaced = rr_prev = rr_true;
+            it_prev = enqueued;
+            rep_idx = -1;
             continue;
         }
-        if (last_replaced) rr_replaced = st.host->rr;
-        replace_next = st.host->rr < delta * delta * rr_replaced;
+        const double rr = st.host->rr;
+        const int its = st.host->iters;
+        if (rep_at >= 0) rr_replaced = st.host->aux;
+        if (its > it_prev && rr < rr_prev) rho2 = std::pow(rr / rr_prev, 1.0 / (its - it_prev));
+        rr_prev = rr;
+        it_prev = its;
+        // Iterations from now until ||r||^2 < delta^2 rr_replaced.
+        const double target = delta * delta * rr_replaced;
+        rep_idx = -1;
+        if (rr < target) {
+            rep_idx = 0;
+        } else if (rho2 > 0.0 && rho2 < 1.0) {
+            const double j = std::ceil(std::log(target / rr) / std::log(rho2));
+            if (j <= K) rep_idx = std::max(1, (int)j) - 1;
+        }
     }
     stats->iterations = st.host->iters;
     stats->norm_r = std::sqrt(st.host->rr);
