@@ -157,9 +157,11 @@ def cpu_baseline():
 
 def bench_c3(args):
     """64 images, heights and widths uniform in 256..1024 (step 16), unknown
-    bands 5-40 px: all alpha solves (set-up + PCG + clip, rtol 1e-7 float64 and
-    the mixed form at rtol 1e-8) batched, against a loop of one-image solves
-    with the same settings. Tile shapes and iterations per graph come from
+    bands 5-40 px: all alpha solves (set-up + PCG + clip) batched, against a
+    loop of one-image solves with the same settings. Both forms are run at
+    tolerances that reach max|d alpha| <= 1e-4 of the converged solutions
+    (float64 PCG at rtol 1e-9; the mixed form at rtol 1e-8 with inner
+    rtol 1e-3). Tile shapes and iterations per graph come from
     BatchSolver.autotune (run once, outside the timed steps)."""
     import torch
     import pymatting_amd as pm
@@ -169,7 +171,7 @@ def bench_c3(args):
     images = [torch.from_numpy(d[0]).cuda() for d in data]
     trimaps = [torch.from_numpy(d[1]).cuda() for d in data]
     npix = sum(int(t.numel()) for t in trimaps)
-    forms = {"f64": dict(precision="f64", rtol=1e-7),
+    forms = {"f64": dict(precision="f64", rtol=1e-9),
              "mixed": dict(precision="mixed", rtol=1e-8, inner_rtol=1e-3)}
 
     def batched(form, tuned=None):

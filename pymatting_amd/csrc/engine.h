@@ -346,12 +346,59 @@ struct InJac0 {
     __device__ __forceinline__ T load(int, int, long long g) const { return omega * b[g] * iD[g]; }
 };
 
+// Inputs that stage data in LDS before the tile's loads (InProlong) provide
+// Shared<RX, RY>, stage() and load_staged(); the others are read directly.
+template <class In>
+struct InStaged { static constexpr bool value = false; };
+
 // base(x) + P e with P = R^T of the full-weighting restriction (vcycle.py:6-29).
 template <typename T, class Base>
 struct InProlong {
     Base base;
     const T* e;  // coarse vector (h2 x w2)
     int w2, h2;
+    // The coarse block under a tile's input region of RX x RY fine points
+    // starting at the even (ox0 - 2, oy0 - 2), with zeros outside the coarse
+    // grid; each coarse value is read from memory once per tile.
+    template <int RX, int RY>
+    struct Shared {
+        static constexpr int SW = RX / 2 + 2, SH = RY / 2 + 2;
+        T e[SH][SW];
+    };
+    template <int RX, int RY>
+    __device__ __forceinline__ void stage(Shared<RX, RY>& sh, int ox0, int oy0) const {
+        using S = Shared<RX, RY>;
+        const int cx0 = (ox0 - 2) >> 1, cy0 = (oy0 - 2) >> 1;
+        for (int t = threadIdx.x; t < S::SW * S::SH; t += kBlock) {
+            const int ly = t / S::SW, lx = t - ly * S::SW;
+            const int cx = cx0 + lx, cy = cy0 + ly;
+            const bool v = cx >= 0 && cx < w2 && cy >= 0 && cy < h2;
+            sh.e[ly][lx] = v ? e[(long long)cy * w2 + cx] : T(0);
+        }
+    }
+    template <int RX, int RY>
+    __device__ __forceinline__ T load_staged(const Shared<RX, RY>& sh, int ox0, int oy0, int gx,
+                                             int gy, long long g) const {
+        const T v = base.load(gx, gy, g);
+        // Same taps and weights as load(); zeros stand for the dropped taps.
+        const int lx0 = (gx >> 1) - ((ox0 - 2) >> 1), ly0 = (gy >> 1) - ((oy0 - 2) >> 1);
+        const bool ox = gx & 1, oy = gy & 1;
+        const T wx0 = ox ? T(0.25) : T(0.5), wy0 = oy ? T(0.25) : T(0.5);
+        T pe = 0;
+        {
+            T row = 0;
+            row += wx0 * sh.e[ly0][lx0];
+            if (ox) row += T(0.25) * sh.e[ly0][lx0 + 1];
+            pe += wy0 * row;
+        }
+        if (oy) {
+            T row = 0;
+            row += wx0 * sh.e[ly0 + 1][lx0];
+            if (ox) row += T(0.25) * sh.e[ly0 + 1][lx0 + 1];
+            pe += T(0.25) * row;
+        }
+        return v + pe;
+    }
     __device__ __forceinline__ T load(int gx, int gy, long long g) const {
         const T v = base.load(gx, gy, g);
         // Fine (gx, gy) takes 1/2 of the coincident coarse point per axis when
@@ -380,6 +427,14 @@ struct InProlong {
         return v + pe;
     }
 };
+
+template <typename T, class Base>
+struct InStaged<InProlong<T, Base>> { static constexpr bool value = true; };
+
+template <class In, int RX, int RY, bool = InStaged<In>::value>
+struct InShared { struct type {}; };
+template <class In, int RX, int RY>
+struct InShared<In, RX, RY, true> { using type = typename In::template Shared<RX, RY>; };
 
 // ------------------------------------------------------------------- outputs
 
@@ -610,6 +665,13 @@ __device__ __forceinline__ void tile_body(const Op& op, const In& in, const Out&
         }
     }
 
+    __shared__ typename InShared<In, RX, RY>::type sin;
+    static_assert(!InStaged<In>::value || (!IsRestrict<Out>::value && OX % 2 == 0 && OY % 2 == 0),
+                  "staged inputs assume plain tiles starting at even coordinates");
+    if constexpr (InStaged<In>::value) {
+        in.template stage<RX, RY>(sin, ox0, oy0);
+        __syncthreads();
+    }
     constexpr int NR = (RX * RY + kBlock - 1) / kBlock;
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
@@ -619,7 +681,12 @@ __device__ __forceinline__ void tile_body(const Op& op, const In& in, const Out&
             const int gx = ox0 - 2 + lx, gy = oy0 - 2 + ly;
             const bool inb = gx >= 0 && gx < w && gy >= 0 && gy < h;
             const long long g = inb ? (long long)gy * w + gx : 0;
-            const T xv = inb ? in.load(gx, gy, g) : T(0);
+            T xv = T(0);
+            if constexpr (InStaged<In>::value) {
+                if (inb) xv = in.template load_staged<RX, RY>(sin, ox0, oy0, gx, gy, g);
+            } else {
+                if (inb) xv = in.load(gx, gy, g);
+            }
             sx[ly][lx] = xv;
             op.template load_point<OX, OY>(sh, ly, lx, inb, g, xv);
         }
