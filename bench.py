@@ -73,12 +73,13 @@ def bytes_per_point(kind, s):
         "coarse_restrict": 13 * s + 2 * s + s / 4,
         "coarse_smooth": 13 * s + 2 * s + s / 4 + s,
         "coarse_residual": 13 * s + 3 * s,
+        "cg_update_x64": 8 + s + 2 * s + s + 8,            # x (f64), p, r, q -> x, r
     }[kind]
 
 
 KINDS = ["cf_apply", "cf_restrict", "cf_smooth", "cf_residual", "dia_apply", "dia_restrict",
          "dia_smooth", "dia_residual", "cg_update", "cg_pupdate", "tail", "fg_sweep",
-         "coarse_apply", "coarse_restrict", "coarse_smooth", "coarse_residual"]
+         "coarse_apply", "coarse_restrict", "coarse_smooth", "coarse_residual", "cg_update_x64"]
 
 
 # Algorithmic bytes per grid point of the stored-stencil SpMV y = A x:
@@ -279,6 +280,19 @@ def bench_c4(args):
 
     alpha_s, st_s, t_s = timed("stencil")
     alpha_f, st_f, t_f = timed("matrix_free")
+
+    def step_reliable():
+        solver = pm.LambdaSolver(image, trimap, level0="stencil", **base)
+        return solver.solve(rtol=4e-10, mixed="reliable"), solver.stats
+
+    for _ in range(args.warmup):
+        step_reliable()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        alpha_r, st_r = step_reliable()
+    torch.cuda.synchronize()
+    t_r = (time.perf_counter() - t0) / args.steps
     conv = pm.LambdaSolver(image, trimap, precision="f64", level0="stencil")
     alpha_c = conv.solve(rtol=1e-11, maxiter=100000)
 
@@ -313,6 +327,10 @@ def bench_c4(args):
         "matrix_free_fp32": {"value": round(n / t_f / 1e6, 3), "ms_per_step": round(t_f * 1e3, 3),
                              "iterations": st_f["iterations"],
                              "max_abs_dalpha": err(alpha_f)[0]},
+        "stored_fp32_reliable_updates": {"value": round(n / t_r / 1e6, 3),
+                                         "ms_per_step": round(t_r * 1e3, 3), "rtol": 4e-10,
+                                         "iterations": st_r["iterations"],
+                                         "max_abs_dalpha": err(alpha_r)[0]},
         "foreground_f32": {"ms": round(t_fg * 1e3, 3), "Mpix_per_s": round(n / t_fg / 1e6, 2),
                            "max_abs_dF_vs_f64": dF, "max_abs_dB_vs_f64": dB},
     }

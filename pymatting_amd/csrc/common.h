@@ -136,6 +136,7 @@ enum ProfKind {
     kProfDiaApply, kProfDiaRestrict, kProfDiaSmooth, kProfDiaResidual,
     kProfCgUpdate, kProfCgPupdate, kProfDense, kProfFgSweep,
     kProfCoarseApply, kProfCoarseRestrict, kProfCoarseSmooth, kProfCoarseResidual,
+    kProfCgUpdateX64,  // float32 CG update with the solution in float64
     kProfCount
 };
 

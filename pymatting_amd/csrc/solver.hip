@@ -704,7 +704,7 @@ int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x
                                            st.dev, s));
             std::swap(p_cur, p_next);
             {
-                ProfScope prof_u(kProfCgUpdate, n, 1, s);
+                ProfScope prof_u(kProfCgUpdateX64, n, 1, s);
                 k_update_x64<<<vb, kBlock, 0, s>>>(x, p_cur, r.as<float>(), q.as<float>(), n,
                                                    st.dev, red_upd);
                 HM_LAUNCH_CHECK();
