@@ -443,6 +443,7 @@ struct DotSink {
     CgState* st;   // nullable: no reduction
     Reducer red;
     int kind;      // see dot_finalize
+    bool defer = false;  // partials only: the consuming kernel sums them
 };
 
 enum DotKind {

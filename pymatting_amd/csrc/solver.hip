@@ -576,53 +576,86 @@ int pcg_mixed(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x, d
 // the recursive residual has dropped by `delta` since the last replacement it
 // is replaced by the true float64 residual b - A x (rounded to float32)
 // before the preconditioner runs; convergence is declared on a true residual.
-__global__ __launch_bounds__(kBlock) void k_update_x64(double* __restrict__ x,
-                                                       const float* __restrict__ p,
-                                                       float* __restrict__ r,
-                                                       const float* __restrict__ q, long long n,
-                                                       CgState* st, Reducer red) {
-    if (solve_done(st)) return;
-    const double alpha = st->alpha;
-    const float alpha32 = (float)alpha;
-    double acc = 0.0;
-    const long long base = (long long)blockIdx.x * kBlock * kVecPerThread + threadIdx.x;
-#pragma unroll
-    for (int k = 0; k < kVecPerThread; ++k) {
-        const long long i = base + (long long)k * kBlock;
-        if (i < n) {
-            x[i] += alpha * (double)p[i];
-            const float ri = r[i] - alpha32 * q[i];
-            r[i] = ri;
-            acc += (double)ri * (double)ri;
-        }
-    }
-    double v[1] = {acc};
-    block_partials<kBlock, 1>(v, red);
-}
 
 // Records ||r||^2 of a replacement iteration (the true residual's).
 __global__ void k_note_rr(CgState* st) {
     if (!solve_done(st)) st->aux = st->rr;
 }
 
-// r32 = (float)r64 with ||r64||^2 partials (replacing the update's).
-__global__ __launch_bounds__(kBlock) void k_replace_r(const double* __restrict__ r64,
-                                                      float* __restrict__ r32, long long n,
-                                                      const CgState* st, Reducer red) {
+// The reliable-update CG step with the alpha finalize folded in: each of
+// kUpdBlocks workgroups sums the SpMV's <p, A p> partials in the same fixed
+// order (so all agree), forms alpha = rz / pq, and updates x (float64) and r
+// (float32) over its grid-stride share, with ||r||^2 partials; workgroup 0
+// records pq and alpha in the state (cg.py:90-100).
+constexpr int kUpdBlocks = 256;
+
+__global__ __launch_bounds__(kFinBlock) void k_update_x64_pq(double* __restrict__ x,
+                                                             const float* __restrict__ p,
+                                                             float* __restrict__ r,
+                                                             const float* __restrict__ q,
+                                                             long long n, CgState* st,
+                                                             Reducer red_pq, unsigned pq_blocks,
+                                                             Reducer red) {
     if (solve_done(st)) return;
+    __shared__ double s_alpha;
+    {
+        const double* src[1] = {red_pq.partials};
+        const unsigned cnt[1] = {pq_blocks};
+        double sum[1];
+        finalize_sums<1>(src, cnt, sum);
+        if (threadIdx.x == 0) {
+            if (blockIdx.x == 0) dot_finalize(st, kDotPQ, sum[0]);
+            s_alpha = st->rz / sum[0];
+        }
+        __syncthreads();
+    }
+    const double alpha = s_alpha;
+    const float alpha32 = (float)alpha;
+    constexpr int U = 4;
+    const long long stride = (long long)gridDim.x * kFinBlock;
     double acc = 0.0;
-    const long long base = (long long)blockIdx.x * kBlock * kVecPerThread + threadIdx.x;
+    for (long long i0 = (long long)blockIdx.x * kFinBlock + threadIdx.x; i0 < n;
+         i0 += U * stride) {
+        double xv[U];
+        float pv[U], rv[U], qv[U];
 #pragma unroll
-    for (int k = 0; k < kVecPerThread; ++k) {
-        const long long i = base + (long long)k * kBlock;
-        if (i < n) {
-            const double v = r64[i];
-            r32[i] = (float)v;
-            acc += v * v;
+        for (int u = 0; u < U; ++u) {
+            const long long i = i0 + u * stride;
+            const bool in = i < n;
+            xv[u] = in ? x[i] : 0.0;
+            pv[u] = in ? p[i] : 0.f;
+            rv[u] = in ? r[i] : 0.f;
+            qv[u] = in ? q[i] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long i = i0 + u * stride;
+            if (i >= n) continue;
+            x[i] = xv[u] + alpha * (double)pv[u];
+            const float ri = rv[u] - alpha32 * qv[u];
+            r[i] = ri;
+            acc += (double)ri * (double)ri;
         }
     }
     double v[1] = {acc};
-    block_partials<kBlock, 1>(v, red);
+    block_partials<kFinBlock, 1>(v, red);
+}
+
+// r32 = (float)r64 with ||r64||^2 partials (replacing the update's; the same
+// kUpdBlocks-workgroup grid, so the partial counts match).
+__global__ __launch_bounds__(kFinBlock) void k_replace_r(const double* __restrict__ r64,
+                                                         float* __restrict__ r32, long long n,
+                                                         const CgState* st, Reducer red) {
+    if (solve_done(st)) return;
+    double acc = 0.0;
+    const long long stride = (long long)gridDim.x * kFinBlock;
+    for (long long i = (long long)blockIdx.x * kFinBlock + threadIdx.x; i < n; i += stride) {
+        const double v = r64[i];
+        r32[i] = (float)v;
+        acc += v * v;
+    }
+    double v[1] = {acc};
+    block_partials<kFinBlock, 1>(v, red);
 }
 
 int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x, double rtol,
@@ -642,10 +675,9 @@ int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x
     HM_TRY(q.alloc(sizeof(float) * n));
     HM_TRY(z.alloc(sizeof(float) * n));
     Reducer red{};
-    const unsigned vb = vec_blocks(n);
-    HM_TRY(make_reducer(&red, std::max<int>(reducer_blocks(n, A.h(), A.w()), (int)vb), 3));
+    HM_TRY(make_reducer(&red, std::max<int>(reducer_blocks(n, A.h(), A.w()), kUpdBlocks), 3));
     struct RedGuard { Reducer* r; ~RedGuard() { free_reducer(r); } } rg{&red};
-    red.upd_blocks = vb;
+    red.upd_blocks = kUpdBlocks;
     Reducer red_upd = red;
     red_upd.partials += 2 * (size_t)red.max_blocks;
     DeviceState st;
@@ -673,7 +705,7 @@ int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x
     HM_TRY(st.push(s));
     k_fill<double><<<div_up(n, 256), 256, 0, s>>>(x, 0.0, n);
     HM_LAUNCH_CHECK();
-    k_replace_r<<<vb, kBlock, 0, s>>>(b, r.as<float>(), n, nullptr, red_upd);
+    k_replace_r<<<kUpdBlocks, kFinBlock, 0, s>>>(b, r.as<float>(), n, nullptr, red_upd);
     HM_LAUNCH_CHECK();
     k_fill<float><<<div_up(n, 256), 256, 0, s>>>(p.as<float>(), 0.0f, n);
     HM_LAUNCH_CHECK();
@@ -688,7 +720,10 @@ int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x
     // drift from the true residual and stall the convergence.
     constexpr int kReliableBatch = 8;
     const int K = check_every < 1 ? kReliableBatch : std::min(check_every, kReliableBatch);
-    const DotSink pq{st.dev, red, kDotPQ};
+    // <p, A p> partials of the SpMV, summed by the update kernel.
+    DotSink pq{st.dev, red, kDotPQ};
+    pq.defer = true;
+    const unsigned pq_blocks = tile_map(OutApply<float>{}, A.h(), A.w(), A.h() / 2, A.w() / 2).ntiles;
     double rr_replaced = norm_b * norm_b;  // ||r||^2 at the last replacement
     double rr_prev = rr_replaced, rho2 = 0.0;  // rate: ||r||^2 ratio per iteration
     int it_prev = 0;   // iterations at the last read-back
@@ -705,14 +740,14 @@ int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x
             std::swap(p_cur, p_next);
             {
                 ProfScope prof_u(kProfCgUpdateX64, n, 1, s);
-                k_update_x64<<<vb, kBlock, 0, s>>>(x, p_cur, r.as<float>(), q.as<float>(), n,
-                                                   st.dev, red_upd);
+                k_update_x64_pq<<<kUpdBlocks, kFinBlock, 0, s>>>(
+                    x, p_cur, r.as<float>(), q.as<float>(), n, st.dev, red, pq_blocks, red_upd);
                 HM_LAUNCH_CHECK();
             }
             if (replace) {
                 // r = b - A x in float64, its norm for the convergence test.
                 HM_TRY(op_residual<double>(A, b, x, r64.as<double>(), s));
-                k_replace_r<<<vb, kBlock, 0, s>>>(r64.as<double>(), r.as<float>(), n, st.dev,
+                k_replace_r<<<kUpdBlocks, kFinBlock, 0, s>>>(r64.as<double>(), r.as<float>(), n, st.dev,
                                                   red_upd);
                 HM_LAUNCH_CHECK();
             }
@@ -745,7 +780,7 @@ int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x
             st.host->rr = rr_true;
             st.host->beta = 0.0;
             HM_TRY(st.push(s));
-            k_replace_r<<<vb, kBlock, 0, s>>>(r64.as<double>(), r.as<float>(), n, nullptr,
+            k_replace_r<<<kUpdBlocks, kFinBlock, 0, s>>>(r64.as<double>(), r.as<float>(), n, nullptr,
                                               red_upd);
             HM_LAUNCH_CHECK();
             HM_TRY(pc_apply<float>(M, r.as<float>(), z.as<float>(), st.dev, red, kDotRzFirst,

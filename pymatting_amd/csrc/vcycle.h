@@ -25,7 +25,7 @@ int launch_tile(const Op& op, const In& in, const Out& out, int h, int w, int h2
                    sizeof(T) == 4, s);
     tile_kernel<T, Op, In, Out><<<map.ntiles, kBlock, 0, s>>>(op, in, out, h, w, guard, map);
     HM_LAUNCH_CHECK();
-    if (out.dot.st != nullptr)
+    if (out.dot.st != nullptr && !out.dot.defer)
         return launch_finalize(1, out.dot.red, map.ntiles, out.dot.st, out.dot.kind, guard, s);
     return HM_OK;
 }
