@@ -468,7 +468,7 @@ def main():
     # Tolerances at which each form reaches max|d alpha| <= 1e-4 (checked below).
     # The mixed form's float64 loop: reliable updates (one float32 CG, float64
     # solution, float64 residual replacement); restarted refinement beside it.
-    MIXED = dict(precision="mixed", level0="stencil", rtol=1e-9, mixed="reliable", delta=0.1)
+    MIXED = dict(precision="mixed", level0="stencil", rtol=1e-9, mixed="reliable", delta=0.01)
     MIXED_REFINE = dict(precision="mixed", level0="stencil", rtol=1e-8, inner_rtol=1e-3)
     FP64 = dict(precision="f64", level0="stencil", rtol=1e-9)
     FP64_REF_RTOL = dict(precision="f64", level0="stencil", rtol=1e-7)

@@ -87,7 +87,7 @@ class LambdaSolver:
         self.stats = {}
 
     def solve_system(self, b, rtol=1e-7, atol=0.0, maxiter=10000, inner_rtol=1e-4,
-                     max_outer=20, check_every=8, mixed="refine", delta=0.1):
+                     max_outer=20, check_every=8, mixed="refine", delta=0.01):
         """x = A^-1 b for a device float64 right-hand side b (length h*w), with
         this solver's operator, preconditioner and precision.
 
@@ -129,7 +129,7 @@ class LambdaSolver:
         return x
 
     def solve(self, rtol=1e-7, atol=0.0, maxiter=10000, inner_rtol=1e-4, max_outer=20,
-              check_every=8, mixed="refine", delta=0.1):
+              check_every=8, mixed="refine", delta=0.01):
         self.x = self.solve_system(self.b, rtol, atol, maxiter, inner_rtol, max_outer,
                                    check_every, mixed, delta)
         return _finalize(self.x, self.tdev, None, (self.h, self.w))

@@ -304,7 +304,7 @@ int post_stage(const LevelOp<T>& op, const Base& base, const T* b, const T* e, i
 constexpr int kTailMaxLevels = 24;
 constexpr int kTailBlock = 1024;
 constexpr int kTailLdsBytes = 152 * 1024;
-constexpr int kTailMaxTop = 4096;  // largest top grid of a tail (points)
+constexpr int kTailMaxTop = 1024;  // largest top grid of a tail (points)
 
 template <typename T>
 struct TailLevel {
