@@ -374,6 +374,69 @@ struct PcgOptions {
     void* cb_ctx = nullptr;
 };
 
+// The CG update with the alpha finalize folded in (cg.py:90-100): each of
+// kUpdBlocks workgroups sums the SpMV's <p, A p> partials in the fixed order
+// of k_finalize (so all agree with it and with each other), forms
+// alpha = rz / pq, and updates x and r over its grid-stride share with
+// ||r||^2 partials; workgroup 0 records pq and alpha in the state. x is
+// float64 in the reliable-update solve (TX = double, T = float), else T.
+constexpr int kUpdBlocks = 256;
+
+template <typename TX, typename T>
+__global__ __launch_bounds__(kFinBlock) void k_cg_update_pq(TX* __restrict__ x,
+                                                            const T* __restrict__ p,
+                                                            T* __restrict__ r,
+                                                            const T* __restrict__ q, long long n,
+                                                            CgState* st, Reducer red_pq,
+                                                            unsigned pq_blocks, Reducer red) {
+    if (solve_done(st)) return;
+    __shared__ double s_alpha;
+    {
+        const double* src[1] = {red_pq.partials};
+        const unsigned cnt[1] = {pq_blocks};
+        double sum[1];
+        finalize_sums<1>(src, cnt, sum);
+        if (threadIdx.x == 0) {
+            if (blockIdx.x == 0) dot_finalize(st, kDotPQ, sum[0]);
+            s_alpha = st->rz / sum[0];
+        }
+        __syncthreads();
+    }
+    const double alpha = s_alpha;
+    const T alphaT = (T)alpha;
+    constexpr int U = 4;
+    const long long stride = (long long)gridDim.x * kFinBlock;
+    double acc = 0.0;
+    for (long long i0 = (long long)blockIdx.x * kFinBlock + threadIdx.x; i0 < n;
+         i0 += U * stride) {
+        TX xv[U];
+        T pv[U], rv[U], qv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long i = i0 + u * stride;
+            const bool in = i < n;
+            xv[u] = in ? x[i] : TX(0);
+            pv[u] = in ? p[i] : T(0);
+            rv[u] = in ? r[i] : T(0);
+            qv[u] = in ? q[i] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long i = i0 + u * stride;
+            if (i >= n) continue;
+            if constexpr (std::is_same<TX, T>::value)
+                x[i] = xv[u] + alphaT * pv[u];
+            else
+                x[i] = xv[u] + alpha * (TX)pv[u];
+            const T ri = rv[u] - alphaT * qv[u];
+            r[i] = ri;
+            acc += (double)ri * (double)ri;
+        }
+    }
+    double v[1] = {acc};
+    block_partials<kFinBlock, 1>(v, red);
+}
+
 // Preconditioned CG (cg.py:4-117) on device vectors of type T with float64
 // scalars. Returns HM_ERR_NOCONV if maxiter iterations did not converge;
 // stats are filled in either case. With host callbacks (callable A or M, or a
@@ -392,7 +455,8 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
     if (opt.reorth) HM_TRY(rold.alloc(sizeof(T) * n));
     Reducer red{};
     const unsigned vb = vec_blocks(n);
-    HM_TRY(make_reducer(&red, std::max<int>(reducer_blocks(n, A.h(), A.w()), (int)vb), 3));
+    HM_TRY(make_reducer(&red, std::max<int>({reducer_blocks(n, A.h(), A.w()), (int)vb, kUpdBlocks}),
+                        3));
     struct RedGuard { Reducer* r; ~RedGuard() { free_reducer(r); } } rg{&red};
     red.upd_blocks = vb;
     Reducer red_upd = red;  // slot 2: ||r||^2 partials of the CG update
@@ -451,8 +515,16 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
     // iteration); the preconditioner then runs once more after convergence,
     // which changes neither x nor the iteration count.
     const bool defer = !interactive && !opt.reorth;
+    // Stencil operators in that mode also leave the alpha finalize to the
+    // update kernel, which sums the SpMV's partials itself.
+    const bool fused = defer && (A.cf || A.dia);
+    if (fused) red.upd_blocks = kUpdBlocks;
+    red_upd.upd_blocks = red.upd_blocks;
     int enqueued = 0;
-    const DotSink pq{st.dev, red, kDotPQ};
+    DotSink pq{st.dev, red, kDotPQ};
+    pq.defer = fused;
+    const unsigned pq_blocks =
+        fused ? tile_map(OutApply<T>{}, A.h(), A.w(), A.h() / 2, A.w() / 2).ntiles : 0;
     while (enqueued < maxiter) {
         const int k = (maxiter - enqueued < check_every) ? (maxiter - enqueued) : check_every;
         for (int it = 0; it < k; ++it) {
@@ -464,8 +536,12 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
             std::swap(p_cur, p_next);
             {
                 ProfScope prof_u(kProfCgUpdate, n, sizeof(T) == 4, s);
-                k_cg_update<T><<<vb, kBlock, 0, s>>>(x, p_cur, r.as<T>(), q.as<T>(), n,
-                                                     st.dev, red_upd);
+                if (fused)
+                    k_cg_update_pq<T, T><<<kUpdBlocks, kFinBlock, 0, s>>>(
+                        x, p_cur, r.as<T>(), q.as<T>(), n, st.dev, red, pq_blocks, red_upd);
+                else
+                    k_cg_update<T><<<vb, kBlock, 0, s>>>(x, p_cur, r.as<T>(), q.as<T>(), n,
+                                                         st.dev, red_upd);
                 HM_LAUNCH_CHECK();
                 if (!defer) HM_TRY(launch_finalize(1, red_upd, vb, st.dev, kDotUpdate, st.dev, s));
             }
@@ -582,65 +658,6 @@ __global__ void k_note_rr(CgState* st) {
     if (!solve_done(st)) st->aux = st->rr;
 }
 
-// The reliable-update CG step with the alpha finalize folded in: each of
-// kUpdBlocks workgroups sums the SpMV's <p, A p> partials in the same fixed
-// order (so all agree), forms alpha = rz / pq, and updates x (float64) and r
-// (float32) over its grid-stride share, with ||r||^2 partials; workgroup 0
-// records pq and alpha in the state (cg.py:90-100).
-constexpr int kUpdBlocks = 256;
-
-__global__ __launch_bounds__(kFinBlock) void k_update_x64_pq(double* __restrict__ x,
-                                                             const float* __restrict__ p,
-                                                             float* __restrict__ r,
-                                                             const float* __restrict__ q,
-                                                             long long n, CgState* st,
-                                                             Reducer red_pq, unsigned pq_blocks,
-                                                             Reducer red) {
-    if (solve_done(st)) return;
-    __shared__ double s_alpha;
-    {
-        const double* src[1] = {red_pq.partials};
-        const unsigned cnt[1] = {pq_blocks};
-        double sum[1];
-        finalize_sums<1>(src, cnt, sum);
-        if (threadIdx.x == 0) {
-            if (blockIdx.x == 0) dot_finalize(st, kDotPQ, sum[0]);
-            s_alpha = st->rz / sum[0];
-        }
-        __syncthreads();
-    }
-    const double alpha = s_alpha;
-    const float alpha32 = (float)alpha;
-    constexpr int U = 4;
-    const long long stride = (long long)gridDim.x * kFinBlock;
-    double acc = 0.0;
-    for (long long i0 = (long long)blockIdx.x * kFinBlock + threadIdx.x; i0 < n;
-         i0 += U * stride) {
-        double xv[U];
-        float pv[U], rv[U], qv[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const long long i = i0 + u * stride;
-            const bool in = i < n;
-            xv[u] = in ? x[i] : 0.0;
-            pv[u] = in ? p[i] : 0.f;
-            rv[u] = in ? r[i] : 0.f;
-            qv[u] = in ? q[i] : 0.f;
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const long long i = i0 + u * stride;
-            if (i >= n) continue;
-            x[i] = xv[u] + alpha * (double)pv[u];
-            const float ri = rv[u] - alpha32 * qv[u];
-            r[i] = ri;
-            acc += (double)ri * (double)ri;
-        }
-    }
-    double v[1] = {acc};
-    block_partials<kFinBlock, 1>(v, red);
-}
-
 // r32 = (float)r64 with ||r64||^2 partials (replacing the update's; the same
 // kUpdBlocks-workgroup grid, so the partial counts match).
 __global__ __launch_bounds__(kFinBlock) void k_replace_r(const double* __restrict__ r64,
@@ -740,7 +757,7 @@ int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x
             std::swap(p_cur, p_next);
             {
                 ProfScope prof_u(kProfCgUpdateX64, n, 1, s);
-                k_update_x64_pq<<<kUpdBlocks, kFinBlock, 0, s>>>(
+                k_cg_update_pq<double, float><<<kUpdBlocks, kFinBlock, 0, s>>>(
                     x, p_cur, r.as<float>(), q.as<float>(), n, st.dev, red, pq_blocks, red_upd);
                 HM_LAUNCH_CHECK();
             }
