@@ -339,7 +339,9 @@ def bench_c4(args):
 
 def bench_c5(args):
     """Gradient of mean((alpha - target)^2) over 16 images of 512x512 (float32
-    tensors) through the mixed-precision solve and its hand-written adjoint.
+    tensors) through the batched mixed-precision solve and its hand-written
+    adjoint (one batched PCG forward, one batched adjoint PCG backward); the
+    same gradient with the images solved one at a time is timed beside it.
     Checks: central finite differences (step 1e-3, float64 solves to rtol
     1e-13) on 200 random pixel channels of the unknown regions, and the
     float64 adjoint gradient (float64 solves to rtol 1e-12) as the reference."""
@@ -379,6 +381,9 @@ def bench_c5(args):
         tb += b
     tf /= args.steps
     tb /= args.steps
+    # The same gradient with the images solved one at a time.
+    _, g_loop, tf_loop, tb_loop = forward_backward(img32, batched=False)
+    loop_rel = float((g_loop - g32).norm() / g32.norm())
     img64 = torch.from_numpy(images).cuda()
     f64 = dict(precision="f64", rtol=1e-12)
     loss64, g64, _, _ = forward_backward(img64, **f64)
@@ -411,10 +416,14 @@ def bench_c5(args):
         "ms_per_step": round((tf + tb) * 1e3, 3), "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": "16 images 512x512 float32, loss mean((alpha - target)^2), "
-                               "target = true alpha + N(0, 0.05^2); forward: mixed "
-                               "float32/float64 PCG per image, rtol 1e-8; backward: adjoint "
-                               "PCG + image gradient", "batch": B, "height": h, "width": w},
+                               "target = true alpha + N(0, 0.05^2); forward: batched mixed "
+                               "float32/float64 PCG over the 16 images, rtol 1e-8; backward: "
+                               "batched adjoint PCG + image gradient", "batch": B, "height": h,
+                   "width": w},
         "forward_ms": round(tf * 1e3, 3), "backward_ms": round(tb * 1e3, 3),
+        "one_image_at_a_time": {"forward_ms": round(tf_loop * 1e3, 3),
+                                "backward_ms": round(tb_loop * 1e3, 3),
+                                "grad_rel_l2_vs_batched": loop_rel},
         "grad_vs_fp64_adjoint_rel_l2": rel_l2,
         "fd_check": {"pixels": len(fd_err), "step": step,
                      "max_abs_err_over_max_grad": float(max(fd_err)),

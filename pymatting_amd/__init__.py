@@ -14,7 +14,7 @@ All numerical work runs in hand-written HIP kernels of libpymatting_amd.so
 """
 from pymatting_amd.alpha import (LambdaSolver, estimate_alpha, estimate_alpha_cf,
                                  solve_alpha_lambda)
-from pymatting_amd.autograd import AlphaSolve, solve_alpha
+from pymatting_amd.autograd import AlphaSolve, BatchAlphaSolve, solve_alpha
 from pymatting_amd.batch import BatchSolver
 from pymatting_amd.foreground import estimate_foreground_ml
 from pymatting_amd.laplacian import (CfLaplacianMatrix, CfOperator, DiaOperator, cf_laplacian,
@@ -27,5 +27,6 @@ __all__ = [
     "cf_laplacian", "make_linear_system", "cg", "CounterCallback", "ProgressCallback", "jacobi",
     "vcycle", "ichol", "estimate_alpha_cf", "estimate_alpha", "estimate_foreground_ml",
     "trimap_split", "sanity_check_image", "CfOperator", "DiaOperator", "CfLaplacianMatrix",
-    "LambdaSolver", "solve_alpha_lambda", "AlphaSolve", "solve_alpha", "BatchSolver",
+    "LambdaSolver", "solve_alpha_lambda", "AlphaSolve", "BatchAlphaSolve", "solve_alpha",
+    "BatchSolver",
 ]

@@ -12,6 +12,9 @@ with a hand-written adjoint:
   (0 <= x <= 1, as torch.clamp does); lam = A^-1 g by the same PCG (A is
   symmetric); dl/dI = -lam^T (dA/dI) x through the window means and inverse
   covariances of L (hm_cf_adjoint_image). The trimap is not differentiable.
+
+A batch (B, h, w, 3) is solved as one batched PCG (BatchSolver) in the
+forward pass and one batched adjoint PCG in the backward pass.
 """
 import torch
 
@@ -70,11 +73,49 @@ class AlphaSolve(torch.autograd.Function):
         return grad.to(ctx.image_dtype), None, None
 
 
-def solve_alpha(image, trimap, **options):
+class BatchAlphaSolve(torch.autograd.Function):
+    """alpha_i = clip(A(image_i)^-1 b_i, 0, 1) for a (B, h, w, 3) batch: the
+    forward solves and the adjoint solves each run as one batched PCG."""
+
+    @staticmethod
+    def forward(ctx, images, trimaps, opts):
+        from pymatting_amd.batch import BatchSolver
+        B, h, w = int(images.shape[0]), int(images.shape[1]), int(images.shape[2])
+        bs = BatchSolver([images[i].detach().to(torch.float64) for i in range(B)],
+                         [trimaps[i] for i in range(B)], opts.epsilon, opts.lambda_value,
+                         opts.precision, opts.num_pre_iter, opts.num_post_iter,
+                         direct_solve_size=opts.direct_solve_size)
+        xs = bs.solve_x(rtol=opts.rtol, inner_rtol=opts.inner_rtol, maxiter=opts.maxiter)
+        ctx.bs, ctx.xs, ctx.opts = bs, xs, opts
+        ctx.image_dtype = images.dtype
+        ctx.forward_stats = list(bs.stats)
+        return torch.stack([x.clamp(0.0, 1.0).reshape(h, w) for x in xs]).to(images.dtype)
+
+    @staticmethod
+    def backward(ctx, grad_alpha):
+        bs, xs, opts = ctx.bs, ctx.xs, ctx.opts
+        gs = [grad_alpha[i].reshape(-1).to(torch.float64) * ((x >= 0.0) & (x <= 1.0))
+              for i, x in enumerate(xs)]
+        B = len(xs)
+        h, w = bs.solvers[0].h, bs.solvers[0].w
+        grad = torch.zeros((B, h, w, 3), dtype=torch.float64, device=xs[0].device)
+        lams = bs.solve_x(rtol=opts.adjoint_rtol, inner_rtol=opts.inner_rtol,
+                          maxiter=opts.maxiter, rhs=gs)
+        ctx.adjoint_stats = list(bs.stats)
+        for i, s in enumerate(bs.solvers):
+            _ffi.check(_ffi.lib().hm_cf_adjoint_image(s.cf.handle, _ffi.ptr(lams[i]),
+                                                      _ffi.ptr(xs[i]), _ffi.ptr(grad[i]), 0,
+                                                      _ffi.stream()))
+        return grad.to(ctx.image_dtype), None, None
+
+
+def solve_alpha(image, trimap, batched=True, **options):
     """Differentiable alpha of one image (h, w, 3) or a batch (B, h, w, 3).
 
     image: torch tensor on the GPU (float32 or float64); gradients flow to it.
     trimap: (h, w) or (B, h, w) tensor or array (not differentiated).
+    batched: a batch is solved as one batched PCG (forward and adjoint);
+    False solves its images one at a time.
     options: see SolveOptions (precision "mixed" or "f64", rtol, ...).
     """
     opts = SolveOptions(**options)
@@ -82,5 +123,7 @@ def solve_alpha(image, trimap, **options):
         return AlphaSolve.apply(image, trimap, opts)
     if image.dim() != 4:
         raise ValueError("image must have shape (h, w, 3) or (B, h, w, 3)")
+    if batched:
+        return BatchAlphaSolve.apply(image, trimap, opts)
     return torch.stack([AlphaSolve.apply(image[i], trimap[i], opts)
                         for i in range(image.shape[0])])

@@ -77,11 +77,22 @@ class BatchSolver:
     def __del__(self):
         self._destroy()
 
-    def solve_x(self, rtol=1e-7, inner_rtol=1e-3, max_outer=20, maxiter=10000):
-        """The solutions x_i (device float64 vectors), without the clip."""
+    def solve_x(self, rtol=1e-7, inner_rtol=1e-3, max_outer=20, maxiter=10000, rhs=None):
+        """The solutions x_i = A_i^-1 b_i (device float64 vectors), without the
+        clip. b_i is each image's lambda right-hand side, or rhs[i] (device
+        float64 vectors of h_i * w_i values, e.g. adjoint right-hand sides)."""
         h = self._batch()
         xs = [empty(s.h * s.w) for s in self.solvers]
-        bp = (ctypes.c_void_p * self.count)(*[s.b.data_ptr() for s in self.solvers])
+        if rhs is None:
+            rhs = [s.b for s in self.solvers]
+        else:
+            if len(rhs) != self.count:
+                raise ValueError("need one right-hand side per image")
+            rhs = [r.reshape(-1).to(torch.float64).contiguous() for r in rhs]
+            for r, s in zip(rhs, self.solvers):
+                if r.numel() != s.h * s.w or not r.is_cuda:
+                    raise ValueError("right-hand sides must be device vectors of h*w values")
+        bp = (ctypes.c_void_p * self.count)(*[r.data_ptr() for r in rhs])
         xp = (ctypes.c_void_p * self.count)(*[x.data_ptr() for x in xs])
         stats = (_ffi.CgStats * self.count)()
         rc = _ffi.lib().hm_batch_solve(h, bp, xp, float(rtol), float(inner_rtol), int(max_outer),

@@ -89,3 +89,40 @@ def test_batched_float32_mixed_gradient():
     assert grad32.dtype == np.float32 and grad32.shape == image.shape
     assert np.abs(alpha32 - alpha64).max() < 1e-4
     assert np.linalg.norm(grad32 - grad64) <= 1e-3 * np.linalg.norm(grad64)
+
+
+def test_batched_and_per_image_gradients_agree():
+    """The batched forward and adjoint PCG (one BatchSolver for the batch)
+    give the gradient of the image-by-image path; an image whose loss
+    gradient is zero (alpha equal to its target) gets a zero image gradient."""
+    import torch
+    import pymatting_amd as pm
+    from pymatting_amd import synthetic
+    imgs, tris = [], []
+    for i in range(3):
+        image, trimap, _ = synthetic.make_image(48, 72, 5200 + i, strands=4, erode=4)
+        imgs.append(image)
+        tris.append(trimap)
+    image = torch.tensor(np.stack(imgs), device="cuda")
+    trimap = torch.tensor(np.stack(tris), device="cuda")
+    opts = dict(precision="f64", rtol=1e-12)
+    with torch.no_grad():
+        alpha0 = pm.solve_alpha(image, trimap, batched=False, **opts)
+    target = alpha0.clone()
+    target[:2] += 0.05 * torch.randn_like(target[:2])  # image 2 keeps its own alpha
+
+    def grad(batched):
+        I = image.clone().requires_grad_(True)
+        alpha = pm.solve_alpha(I, trimap, batched=batched, **opts)
+        ((alpha - target) ** 2).mean().backward()
+        return alpha.detach(), I.grad
+
+    a_b, g_b = grad(True)
+    a_s, g_s = grad(False)
+    print("alpha", float((a_b - a_s).abs().max()), "grad", float((g_b - g_s).norm() / g_s.norm()),
+          flush=True)
+    assert float((a_b - a_s).abs().max()) < 1e-8
+    assert float((g_b - g_s).norm()) <= 1e-7 * float(g_s.norm())
+    # Image 2's target is its per-image alpha: the batched alpha differs from
+    # it only by rounding, so its gradient is that small.
+    assert float(g_b[2].abs().max()) <= 1e-9 * float(g_b.abs().max())
