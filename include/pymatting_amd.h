@@ -227,6 +227,14 @@ int hm_batch_create(int count, hm_dia* const* ops, hm_vc* const* vcs, int precis
 int hm_batch_solve(hm_batch* b, const double* const* rhs, double* const* x, double rtol,
                    double inner_rtol, int max_outer, int maxiter, int graph_iters,
                    hm_cg_stats* stats, void* stream);
+/* The mixed form with reliable updates (see hm_cg_solve_reliable) for a batch
+ * created with precision HM_F32: one float32 CG per image with its solution in
+ * float64; residual replacement per image after a drop by delta; every
+ * image's convergence is confirmed on its float64 residual. Launches are
+ * direct (no graph). */
+int hm_batch_solve_reliable(hm_batch* b, const double* const* rhs, double* const* x,
+                            double rtol, double delta, int maxiter, hm_cg_stats* stats,
+                            void* stream);
 int hm_batch_destroy(hm_batch* b);
 
 /* ---- adjoint of the alpha solve (no reference counterpart: pymatting is

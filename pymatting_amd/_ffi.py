@@ -101,6 +101,7 @@ _SIGS = {
     "hm_fb_ml": (i32, [P, P, i32, i32, i32, P, P, f64, i32, i32, i32, f64, i32, P, P, P]),
     "hm_batch_create": (i32, [i32, P, P, i32, i32, i32, ctypes.POINTER(P)]),
     "hm_batch_solve": (i32, [P, P, P, f64, f64, i32, i32, i32, P, P]),
+    "hm_batch_solve_reliable": (i32, [P, P, P, f64, f64, i32, P, P]),
     "hm_batch_destroy": (i32, [P]),
 }
 

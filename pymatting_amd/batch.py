@@ -77,10 +77,14 @@ class BatchSolver:
     def __del__(self):
         self._destroy()
 
-    def solve_x(self, rtol=1e-7, inner_rtol=1e-3, max_outer=20, maxiter=10000, rhs=None):
+    def solve_x(self, rtol=1e-7, inner_rtol=1e-3, max_outer=20, maxiter=10000, rhs=None,
+                mixed="refine", delta=0.01):
         """The solutions x_i = A_i^-1 b_i (device float64 vectors), without the
         clip. b_i is each image's lambda right-hand side, or rhs[i] (device
-        float64 vectors of h_i * w_i values, e.g. adjoint right-hand sides)."""
+        float64 vectors of h_i * w_i values, e.g. adjoint right-hand sides).
+        For precision "mixed", ``mixed`` picks restarted refinement
+        (inner_rtol, max_outer) or "reliable" updates (delta), as in
+        LambdaSolver.solve_system."""
         h = self._batch()
         xs = [empty(s.h * s.w) for s in self.solvers]
         if rhs is None:
@@ -95,16 +99,24 @@ class BatchSolver:
         bp = (ctypes.c_void_p * self.count)(*[r.data_ptr() for r in rhs])
         xp = (ctypes.c_void_p * self.count)(*[x.data_ptr() for x in xs])
         stats = (_ffi.CgStats * self.count)()
-        rc = _ffi.lib().hm_batch_solve(h, bp, xp, float(rtol), float(inner_rtol), int(max_outer),
-                                       int(maxiter), int(self.graph_iters), stats, _ffi.stream())
+        if self.precision != "f64" and mixed == "reliable":
+            rc = _ffi.lib().hm_batch_solve_reliable(h, bp, xp, float(rtol), float(delta),
+                                                    int(maxiter), stats, _ffi.stream())
+        elif self.precision == "f64" or mixed == "refine":
+            rc = _ffi.lib().hm_batch_solve(h, bp, xp, float(rtol), float(inner_rtol),
+                                           int(max_outer), int(maxiter), int(self.graph_iters),
+                                           stats, _ffi.stream())
+        else:
+            raise ValueError("mixed must be 'refine' or 'reliable'")
         self.stats = [{"iterations": st.iterations, "converged": bool(st.converged),
                        "norm_b": st.norm_b, "norm_r": st.norm_r} for st in stats]
         _ffi.check(rc)
         return xs
 
-    def solve(self, rtol=1e-7, inner_rtol=1e-3, max_outer=20, maxiter=10000):
+    def solve(self, rtol=1e-7, inner_rtol=1e-3, max_outer=20, maxiter=10000, mixed="refine",
+              delta=0.01):
         """Alpha of every image (device float64 arrays of shape (h, w))."""
-        xs = self.solve_x(rtol, inner_rtol, max_outer, maxiter)
+        xs = self.solve_x(rtol, inner_rtol, max_outer, maxiter, mixed=mixed, delta=delta)
         return [_finalize(x, s.tdev, None, (s.h, s.w)) for x, s in zip(xs, self.solvers)]
 
     def autotune(self, rtol=1e-7, inner_rtol=1e-3, maxiter=10000, repeats=1):

@@ -88,6 +88,87 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_batch(const UpdEntry<T>* _
     block_partials_at<kBlock, 1>(v, red, blockIdx.x);
 }
 
+// Reliable-update forms (float32 CG, float64 solution): x (float64) +=
+// alpha p; r -= alpha q; partial ||r||^2 at the block's slot.
+struct UpdEntryX64 {
+    double* x;
+    const float* p;
+    float* r;
+    const float* q;
+    long long n;
+    CgState* st;
+};
+
+__global__ __launch_bounds__(kBlock) void k_update_x64_batch(const UpdEntryX64* __restrict__ ents,
+                                                             const TileRef* __restrict__ blocks,
+                                                             Reducer red) {
+    const TileRef br = blocks[blockIdx.x];
+    const UpdEntryX64 E = ents[br.img];
+    if (solve_done(E.st)) return;
+    const double alpha = E.st->alpha;
+    const float alpha32 = (float)alpha;
+    double acc = 0.0;
+    const long long base = (long long)br.local * kBlock * kVecPerThread + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kVecPerThread; ++k) {
+        const long long i = base + (long long)k * kBlock;
+        if (i < E.n) {
+            E.x[i] += alpha * (double)E.p[i];
+            const float ri = E.r[i] - alpha32 * E.q[i];
+            E.r[i] = ri;
+            acc += (double)ri * (double)ri;
+        }
+    }
+    double v[1] = {acc};
+    block_partials_at<kBlock, 1>(v, red, blockIdx.x);
+}
+
+// r32 = (float)r64 with partial ||r64||^2 at the block's slot (the update's
+// slot, so the following finalize tests the true residual).
+struct RepEntry {
+    const double* r64;
+    float* r32;
+    long long n;
+    const CgState* gate;  // done: no replacement for this image
+};
+
+__global__ __launch_bounds__(kBlock) void k_replace_batch(const RepEntry* __restrict__ ents,
+                                                          const TileRef* __restrict__ blocks,
+                                                          Reducer red) {
+    const TileRef br = blocks[blockIdx.x];
+    const RepEntry E = ents[br.img];
+    if (solve_done(E.gate)) return;
+    double acc = 0.0;
+    const long long base = (long long)br.local * kBlock * kVecPerThread + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kVecPerThread; ++k) {
+        const long long i = base + (long long)k * kBlock;
+        if (i < E.n) {
+            const double v = E.r64[i];
+            E.r32[i] = (float)v;
+            acc += v * v;
+        }
+    }
+    double v[1] = {acc};
+    block_partials_at<kBlock, 1>(v, red, blockIdx.x);
+}
+
+// gate[i].done = 0 exactly for the images whose replacement is due at
+// iteration k of the unit (at[i], set by the host between units) and that
+// are still iterating.
+__global__ void k_gate_batch(CgState* __restrict__ gate, const CgState* __restrict__ st,
+                             const int* __restrict__ at, int k, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) gate[i].done = (at[i] == k && !solve_done(st + i)) ? 0 : 1;
+}
+
+// Records ||r||^2 of the images replaced in this iteration (the true one).
+__global__ void k_note_rr_batch(CgState* __restrict__ st, const CgState* __restrict__ gate,
+                                int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count && !solve_done(gate + i) && !solve_done(st + i)) st[i].aux = st[i].rr;
+}
+
 template <typename T>
 struct DotEntry {
     const T* a;
@@ -619,6 +700,45 @@ struct BatchPlan {
         return HM_OK;
     }
 
+    // One PCG iteration (p_cur -> p_next) for every image with the solution
+    // in float64 (the reliable-update form, T = float); `after_update` runs
+    // between the update and the preconditioner (residual replacement).
+    int iteration_steps_x64(std::vector<Step>& steps, const std::vector<double*>& x,
+                            const std::vector<T*>& rv, const std::vector<T*>& zv,
+                            const std::vector<T*>& qv, const std::vector<T*>& p_cur,
+                            const std::vector<T*>& p_next, CgState* st,
+                            const std::vector<Step>* after_update) {
+        static_assert(std::is_same<T, float>::value, "reliable updates run float32 CG");
+        using ApplyE = BatchEntry<FineOp, InPupdate<T>, OutApply<T, S0>>;
+        std::vector<ApplyE> ap;
+        std::vector<UpdEntryX64> up;
+        for (int i = 0; i < count; ++i) {
+            const hm_dia* A = B->A[i];
+            OutApply<T, S0> out{qv[i], DotSink{st + i, red, kDotPQ}, p_next[i]};
+            ap.push_back(ApplyE{fine_op(i), InPupdate<T>{zv[i], p_cur[i], st + i}, out, A->h,
+                                A->w, plain_tx[0][i], st + i});
+            up.push_back(UpdEntryX64{x[i], p_next[i], rv[i], qv[i], A->n, st + i});
+        }
+        HM_TRY((add_tile_step<T, FineOp, InPupdate<T>, OutApply<T, S0>>(steps, *ar, ap,
+                                                                         plain[0])));
+        add_finalize_step<1>(steps, red, plain[0].d_lo, nullptr, st, count, kDotPQ, 1);
+        {
+            UpdEntryX64* d = nullptr;
+            HM_TRY(ar->upload(up, &d));
+            const TileRef* refs = vec.d_refs;
+            const unsigned nb = (unsigned)vec.refs.size();
+            const Reducer ru = red_upd();
+            steps.push_back([=](hipStream_t s) {
+                k_update_x64_batch<<<nb, kBlock, 0, s>>>(d, refs, ru);
+                HM_LAUNCH_CHECK();
+                return (int)HM_OK;
+            });
+        }
+        if (after_update) steps.insert(steps.end(), after_update->begin(), after_update->end());
+        std::vector<const T*> rc(rv.begin(), rv.end());
+        return vcycle_steps(steps, rc, zv, st, kDotUpdateRz);
+    }
+
     // One PCG iteration (p_cur -> p_next) for every image.
     int iteration_steps(std::vector<Step>& steps, const std::vector<T*>& x,
                         const std::vector<T*>& rv, const std::vector<T*>& zv,
@@ -819,6 +939,298 @@ int batch_pcg(hm_batch* B, BatchPlan<T, S0, SC>& plan, BatchWork<T>& wk, Arena& 
         stats[i].norm_r = std::sqrt(c.rr);
         stats[i].converged = conv ? 1 : 0;
         ok = ok && conv;
+    }
+    if (!ok) {
+        set_error("Conjugate gradient descent did not converge within %d iterations", maxiter);
+        return HM_ERR_NOCONV;
+    }
+    return HM_OK;
+}
+
+// Mixed precision with reliable updates for a batch (see pcg_reliable): one
+// float32 CG per image with the solution in float64. Units of kUnit
+// iterations run back to back and the states are read after each. Each
+// image's residual is replaced by its float64 residual (b - A x, gated per
+// image) at the iteration of the next unit where its norm is predicted, from
+// the image's rate in the last unit, to have dropped by delta since the last
+// replacement. An image that converges on the recursion is confirmed on its
+// float64 residual and, if that fails, restarted from it (beta = 0) through
+// a V-cycle run on a separate state array, so the other images are not
+// touched.
+template <int S0, int SC>
+int batch_reliable(hm_batch* B, Arena& ar, const double* const* b, double* const* x,
+                   double rtol, double delta, int maxiter, hm_cg_stats* stats, hipStream_t s) {
+    constexpr int kUnit = 8;  // even: the direction buffers alternate
+    const int count = B->count;
+    BatchPlan<float, S0, SC> plan;
+    HM_TRY(plan.init(B, ar));
+    BatchPlan<double, S0, SC> plan64;  // tile tables of the float64 residual
+    HM_TRY(plan64.init(B, ar));
+    BatchWork<float> wk;
+    HM_TRY(wk.init(B, ar));
+    BatchStates& st = wk.st;
+    BatchStates aux, rst;  // true residual norms; states of a restart V-cycle
+    HM_TRY(aux.init(count, ar));
+    HM_TRY(rst.init(count, ar));
+    CgState* gate = nullptr;
+    HM_TRY(ar.alloc<CgState>(count, &gate));
+    int* at_dev = nullptr;  // replacement iteration per image in the next unit, or -1
+    HM_TRY(ar.alloc<int>(count, &at_dev));
+    std::vector<double*> r64(count);
+    for (int i = 0; i < count; ++i) HM_TRY(ar.alloc<double>((size_t)B->A[i]->n, &r64[i]));
+    std::vector<double*> xv(x, x + count);
+    const DotSink nodot{nullptr, Reducer{}, 0};
+    const TileRef* vrefs = plan.vec.d_refs;
+    const unsigned nvb = (unsigned)plan.vec.refs.size();
+    const Reducer red = plan.red, red_upd = plan.red_upd();
+    // Steps: r64 = b - A x (float64, gated), r32 = r64 (gated), and per
+    // iteration k of a unit the gate before them and the note after the
+    // preconditioner.
+    std::vector<Step> resid64, rep32, norms64;
+    std::vector<std::vector<Step>> gate_k(kUnit), note_k(kUnit);
+    {
+        using ResE = BatchEntry<DiaOp<double, false>, InVec<double>, OutResidual<double, S0>>;
+        std::vector<ResE> re;
+        std::vector<RepEntry> rp;
+        std::vector<DotEntry<double>> de;
+        for (int i = 0; i < count; ++i) {
+            const hm_dia* A = B->A[i];
+            re.push_back(ResE{DiaOp<double, false>{A->h, A->w, A->c64}, InVec<double>{x[i]},
+                              OutResidual<double, S0>{b[i], r64[i], nodot}, A->h, A->w,
+                              plan64.plain_tx[0][i], gate + i});
+            rp.push_back(RepEntry{r64[i], wk.r[i], A->n, gate + i});
+            de.push_back(DotEntry<double>{r64[i], r64[i], A->n});
+        }
+        const int n = count;
+        CgState* sd = st.dev;
+        for (int k = 0; k < kUnit; ++k) {
+            gate_k[k].push_back([=](hipStream_t s2) {
+                k_gate_batch<<<div_up(n, 256), 256, 0, s2>>>(gate, sd, at_dev, k, n);
+                HM_LAUNCH_CHECK();
+                return (int)HM_OK;
+            });
+            note_k[k].push_back([=](hipStream_t s2) {
+                k_note_rr_batch<<<div_up(n, 256), 256, 0, s2>>>(sd, gate, n);
+                HM_LAUNCH_CHECK();
+                return (int)HM_OK;
+            });
+        }
+        HM_TRY((add_tile_step<double, DiaOp<double, false>, InVec<double>,
+                              OutResidual<double, S0>>(resid64, ar, re, plan64.plain[0])));
+        RepEntry* dr = nullptr;
+        HM_TRY(ar.upload(rp, &dr));
+        rep32.push_back([=](hipStream_t s2) {
+            k_replace_batch<<<nvb, kBlock, 0, s2>>>(dr, vrefs, red_upd);
+            HM_LAUNCH_CHECK();
+            return (int)HM_OK;
+        });
+        // ||r64_i||^2 of every image into aux (only the gated ones are used).
+        DotEntry<double>* dd = nullptr;
+        HM_TRY(ar.upload(de, &dd));
+        const unsigned* lo = plan.vec.d_lo;
+        CgState* ad = aux.dev;
+        norms64.push_back([=](hipStream_t s2) {
+            k_dot_batch<double><<<nvb, kBlock, 0, s2>>>(dd, vrefs, red);
+            HM_LAUNCH_CHECK();
+            k_finalize_batch<1><<<n, kFinBlock, 0, s2>>>(red, lo, nullptr, ad, kDotAux, 0);
+            HM_LAUNCH_CHECK();
+            return (int)HM_OK;
+        });
+    }
+    // ||b_i|| and the initial states.
+    {
+        std::vector<DotEntry<double>> de;
+        for (int i = 0; i < count; ++i) de.push_back(DotEntry<double>{b[i], b[i], B->A[i]->n});
+        DotEntry<double>* d = nullptr;
+        HM_TRY(ar.upload(de, &d));
+        k_dot_batch<double><<<nvb, kBlock, 0, s>>>(d, vrefs, red);
+        HM_LAUNCH_CHECK();
+        k_finalize_batch<1><<<count, kFinBlock, 0, s>>>(red, plan.vec.d_lo, nullptr, st.dev,
+                                                        kDotAux, 0);
+        HM_LAUNCH_CHECK();
+        HM_TRY(st.pull(s));
+    }
+    std::vector<double> norm_b(count), rr_rep(count);
+    for (int i = 0; i < count; ++i) {
+        CgState& c = st.host[i];
+        norm_b[i] = std::sqrt(c.aux);
+        std::memset(&c, 0, sizeof(CgState));
+        c.rtol = rtol;
+        c.norm_b = norm_b[i];
+        c.rr = norm_b[i] * norm_b[i];
+        c.done = norm_b[i] > 0.0 ? 0 : 1;  // x = 0 solves b = 0
+        rr_rep[i] = c.rr;
+        stats[i] = hm_cg_stats{0, c.done, norm_b[i], norm_b[i]};
+    }
+    HM_TRY(st.push(s));
+    // x = 0, p = 0, r32 = b; z = M r with rz = <r, z>.
+    {
+        std::vector<Step> start;
+        std::vector<EwEntry<double, double>> zx;
+        std::vector<EwEntry<float, float>> zp;
+        std::vector<EwEntry<float, double>> cr;
+        for (int i = 0; i < count; ++i) {
+            const long long n = B->A[i]->n;
+            zx.push_back(EwEntry<double, double>{x[i], nullptr, nullptr, n, nullptr});
+            zp.push_back(EwEntry<float, float>{wk.p[i], nullptr, nullptr, n, nullptr});
+            cr.push_back(EwEntry<float, double>{wk.r[i], b[i], nullptr, n, st.dev + i});
+        }
+        HM_TRY((add_ew_step<double, double, kEwZero>(start, ar, zx, plan.vec)));
+        HM_TRY((add_ew_step<float, float, kEwZero>(start, ar, zp, plan.vec)));
+        HM_TRY((add_ew_step<float, double, kEwCopy>(start, ar, cr, plan.vec)));
+        std::vector<const float*> rc(wk.r.begin(), wk.r.end());
+        HM_TRY(plan.vcycle_steps(start, rc, wk.z, st.dev, kDotRzFirst));
+        HM_TRY(run_steps(start, s));
+    }
+    // A unit of kUnit iterations, without replacements and with gated ones
+    // (gate, float64 residual and r32 after each update; note after each
+    // preconditioner).
+    std::vector<Step> unit, unit_rep;
+    for (int k = 0; k < kUnit; ++k) {
+        const bool even = k % 2 == 0;
+        const std::vector<float*>& pc = even ? wk.p : wk.p2;
+        const std::vector<float*>& pn = even ? wk.p2 : wk.p;
+        HM_TRY(plan.iteration_steps_x64(unit, xv, wk.r, wk.z, wk.q, pc, pn, st.dev, nullptr));
+        std::vector<Step> after = gate_k[k];
+        after.insert(after.end(), resid64.begin(), resid64.end());
+        after.insert(after.end(), rep32.begin(), rep32.end());
+        HM_TRY(plan.iteration_steps_x64(unit_rep, xv, wk.r, wk.z, wk.q, pc, pn, st.dev, &after));
+        unit_rep.insert(unit_rep.end(), note_k[k].begin(), note_k[k].end());
+    }
+    // Restart of a subset: z = M r32 with rz into rst (the others are done there).
+    std::vector<Step> restart_vc;
+    {
+        std::vector<const float*> rc(wk.r.begin(), wk.r.end());
+        HM_TRY(plan.vcycle_steps(restart_vc, rc, wk.z, rst.dev, kDotRzFirst));
+    }
+    int* at_host = nullptr;
+    HM_HIP(hipHostMalloc(reinterpret_cast<void**>(&at_host), sizeof(int) * count,
+                         hipHostMallocDefault));
+    struct AtGuard { int* p; ~AtGuard() { (void)hipHostFree(p); } } ag{at_host};
+    std::vector<char> confirmed(count, 0);
+    std::vector<int> rep_at(count, -1);  // none before a rate is known
+    std::vector<int> iters_before(count, 0), it_prev(count, 0);
+    std::vector<double> rr_prev(count), rho2(count, 0.0);
+    for (int i = 0; i < count; ++i) {
+        confirmed[i] = st.host[i].done;
+        rr_prev[i] = rr_rep[i];
+    }
+    int enqueued = 0;
+    while (enqueued < maxiter) {
+        bool any_rep = false;
+        for (int i = 0; i < count; ++i) {
+            any_rep = any_rep || rep_at[i] >= 0;
+            iters_before[i] = st.host[i].iters;
+            at_host[i] = rep_at[i];
+        }
+        if (any_rep) {
+            HM_HIP(hipMemcpyAsync(at_dev, at_host, sizeof(int) * count, hipMemcpyHostToDevice,
+                                  s));
+            HM_TRY(run_steps(unit_rep, s));
+        } else {
+            HM_TRY(run_steps(unit, s));
+        }
+        enqueued += kUnit;
+        HM_TRY(st.pull(s));
+        std::vector<char> confirm(count, 0);
+        bool any_confirm = false, all = true;
+        for (int i = 0; i < count; ++i) {
+            const CgState& c = st.host[i];
+            if (c.nonfinite) {
+                set_error("non-finite value in the batched solve of image %d", i);
+                return HM_ERR_VALUE;
+            }
+            if (confirmed[i]) continue;
+            const int rep_iter = rep_at[i] >= 0 ? iters_before[i] + rep_at[i] + 1 : -1;
+            if (c.done) {
+                if (c.iters == rep_iter) {
+                    confirmed[i] = 1;  // tested on a true residual
+                } else {
+                    confirm[i] = 1;
+                    any_confirm = true;
+                }
+            } else if (rep_iter >= 0) {
+                rr_rep[i] = c.aux;  // noted in the replacement iteration
+            }
+        }
+        if (any_confirm) {
+            // True residuals of the images that converged on the recursion.
+            // Their gates open for the float64 residual (the gate step
+            // opens only images that still iterate).
+            for (int i = 0; i < count; ++i) {
+                rst.host[i] = CgState{};
+                rst.host[i].done = confirm[i] ? 0 : 1;
+            }
+            HM_TRY(rst.push(s));
+            HM_HIP(hipMemcpyAsync(gate, rst.dev, sizeof(CgState) * count,
+                                  hipMemcpyDeviceToDevice, s));
+            HM_TRY(run_steps(resid64, s));
+            HM_TRY(run_steps(norms64, s));
+            HM_TRY(aux.pull(s));
+            std::vector<char> restart(count, 0);
+            bool any_restart = false;
+            for (int i = 0; i < count; ++i) {
+                if (!confirm[i]) continue;
+                const double rr_true = aux.host[i].aux;
+                if (std::sqrt(rr_true) < rtol * norm_b[i]) {
+                    confirmed[i] = 1;
+                    continue;
+                }
+                restart[i] = 1;
+                any_restart = true;
+                CgState& c = st.host[i];
+                c.done = 0;
+                c.rr = rr_true;
+                c.beta = 0.0;
+                rr_rep[i] = rr_prev[i] = rr_true;
+                it_prev[i] = c.iters;
+            }
+            if (any_restart) {
+                for (int i = 0; i < count; ++i) {
+                    rst.host[i] = CgState{};
+                    rst.host[i].done = restart[i] ? 0 : 1;
+                }
+                HM_TRY(rst.push(s));
+                HM_TRY(st.push(s));
+                // r32 = r64 for the restarted images (their gate is open in rst).
+                HM_HIP(hipMemcpyAsync(gate, rst.dev, sizeof(CgState) * count,
+                                      hipMemcpyDeviceToDevice, s));
+                HM_TRY(run_steps(rep32, s));
+                HM_TRY(run_steps(restart_vc, s));
+                HM_TRY(rst.pull(s));
+                for (int i = 0; i < count; ++i)
+                    if (restart[i]) st.host[i].rz = rst.host[i].rz;
+                HM_TRY(st.push(s));
+            }
+        }
+        for (int i = 0; i < count; ++i) all = all && confirmed[i];
+        if (all) break;
+        // Replacement plan for the next unit: the iteration where the norm
+        // is predicted to have dropped by delta since the last replacement.
+        for (int i = 0; i < count; ++i) {
+            const CgState& c = st.host[i];
+            rep_at[i] = -1;
+            if (confirmed[i] || c.done) continue;
+            if (c.iters > it_prev[i] && c.rr < rr_prev[i])
+                rho2[i] = std::pow(c.rr / rr_prev[i], 1.0 / (c.iters - it_prev[i]));
+            rr_prev[i] = c.rr;
+            it_prev[i] = c.iters;
+            const double target = delta * delta * rr_rep[i];
+            if (c.rr < target) {
+                rep_at[i] = 0;
+            } else if (rho2[i] > 0.0 && rho2[i] < 1.0) {
+                const double j = std::ceil(std::log(target / c.rr) / std::log(rho2[i]));
+                if (j <= kUnit) rep_at[i] = std::max(1, (int)j) - 1;
+            }
+        }
+    }
+    bool ok = true;
+    for (int i = 0; i < count; ++i) {
+        const CgState& c = st.host[i];
+        stats[i].iterations = std::min(c.iters, maxiter);
+        stats[i].norm_r = std::sqrt(c.rr);
+        stats[i].converged = confirmed[i] && c.iters <= maxiter ? 1 : 0;
+        ok = ok && stats[i].converged;
     }
     if (!ok) {
         set_error("Conjugate gradient descent did not converge within %d iterations", maxiter);
@@ -1049,6 +1461,43 @@ int hm_batch_destroy(hm_batch* b) {
     if (b->ev_out) (void)hipEventDestroy(b->ev_out);
     delete b;
     return HM_OK;
+}
+
+int hm_batch_solve_reliable(hm_batch* b, const double* const* rhs, double* const* x,
+                            double rtol, double delta, int maxiter, hm_cg_stats* stats,
+                            void* stream) {
+    if (!b || !rhs || !x || !stats || maxiter < 0 || !(delta > 0.0 && delta < 1.0)) {
+        set_error("hm_batch_solve_reliable: invalid argument");
+        return HM_ERR_ARG;
+    }
+    if (b->prec != HM_F32) {
+        set_error("hm_batch_solve_reliable: the batch must be created for the mixed form");
+        return HM_ERR_ARG;
+    }
+    for (int i = 0; i < b->count; ++i) {
+        if (!rhs[i] || !x[i]) {
+            set_error("hm_batch_solve_reliable: null vector for image %d", i);
+            return HM_ERR_ARG;
+        }
+    }
+    // The batch's own stream, ordered after and before the caller's stream.
+    hipStream_t user = static_cast<hipStream_t>(stream);
+    HM_HIP(hipEventRecord(b->ev_in, user));
+    HM_HIP(hipStreamWaitEvent(b->stream, b->ev_in, 0));
+    Arena ar;  // destroyed after the stream has drained
+    int rc;
+    if (b->s0 == 0 && b->sc == 0)
+        rc = batch_reliable<0, 0>(b, ar, rhs, x, rtol, delta, maxiter, stats, b->stream);
+    else if (b->s0 == 0)
+        rc = batch_reliable<0, 1>(b, ar, rhs, x, rtol, delta, maxiter, stats, b->stream);
+    else if (b->sc == 0)
+        rc = batch_reliable<1, 0>(b, ar, rhs, x, rtol, delta, maxiter, stats, b->stream);
+    else
+        rc = batch_reliable<1, 1>(b, ar, rhs, x, rtol, delta, maxiter, stats, b->stream);
+    (void)hipStreamSynchronize(b->stream);
+    if (hipEventRecord(b->ev_out, b->stream) == hipSuccess)
+        (void)hipStreamWaitEvent(user, b->ev_out, 0);
+    return rc;
 }
 
 int hm_batch_solve(hm_batch* b, const double* const* rhs, double* const* x, double rtol,

@@ -80,6 +80,14 @@ def test_lambda_solve_config1_reliable_updates(level0):
         # The reported residual is the float64 residual of the returned x.
         r = solver.b - solver.op.apply(solver.x)
         assert abs(float(torch.linalg.norm(r)) - st["norm_r"]) <= 1e-3 * st["norm_r"] + 1e-12
+    # Replacements that are never due (delta 1e-12): the recursion converges
+    # on its own, the float64 check fails and the solve restarts from the
+    # true residual (about 126 iterations where 96 suffice), still reaching a
+    # float64 residual below rtol.
+    x = solver.solve_system(solver.b, rtol=1e-11, mixed="reliable", delta=1e-12, maxiter=2000)
+    rel = float(torch.linalg.norm(solver.b - solver.op.apply(x)) / torch.linalg.norm(solver.b))
+    print(level0, "restarts", solver.stats, rel, flush=True)
+    assert rel < 1e-11
     with pytest.raises(ValueError):
         solver.solve(mixed="other")
 

@@ -91,3 +91,75 @@ def test_batch_autotune_and_errors():
                                    num_post_iter=2)
     with pytest.raises(ValueError, match="same V-cycle"):
         a.solve()
+
+
+def _true_rel(solver, x, b):
+    import torch
+    r = b - solver.op.apply(x)
+    return float(torch.linalg.norm(r) / torch.linalg.norm(b))
+
+
+def test_batch_reliable_updates():
+    """Batched mixed solve with reliable updates: every image reaches a
+    float64 residual below rtol in about the float64 iteration count, and its
+    alpha is within 1e-4 of the converged one."""
+    import pymatting_amd as pm
+    data = _images()
+    ref = [pm.LambdaSolver(im, tr, precision="f64", level0="stencil") for im, tr in data]
+    conv = [s.solve(rtol=1e-11) for s in ref]
+    its64 = []
+    for s in ref:
+        s.solve_system(s.b, rtol=1e-9)
+        its64.append(s.stats["iterations"])
+    bs = pm.BatchSolver([d[0] for d in data], [d[1] for d in data], precision="mixed")
+    xs = bs.solve_x(rtol=1e-9, mixed="reliable")
+    for x, s, c, st, it in zip(xs, bs.solvers, conv, bs.stats, its64):
+        print(st, it, flush=True)
+        assert st["converged"]
+        assert _true_rel(s, x, s.b) < 1e-9
+        assert abs(st["iterations"] - it) <= 2
+        assert float((x.clamp(0, 1).reshape(c.shape) - c).abs().max()) < 1e-4
+
+
+def test_batch_reliable_restart_and_zero_rhs():
+    """Fast-converging systems (random colours, 60% known pixels) converge
+    between read-backs: the float64 check fails on some images and they are
+    restarted without disturbing the others; a zero right-hand side is
+    solved by x = 0."""
+    import torch
+    import pymatting_amd as pm
+    rng = np.random.default_rng(7)
+    images, trimaps = [], []
+    for h, w in ((97, 131), (64, 64), (120, 90)):
+        images.append(rng.random((h, w, 3)))
+        trimaps.append(rng.choice([0.0, 0.5, 1.0], size=(h, w), p=[0.3, 0.4, 0.3]))
+    bs = pm.BatchSolver(images, trimaps, precision="mixed")
+    rhs = [s.b.clone() for s in bs.solvers]
+    rhs[1].zero_()
+    for rtol in (1e-9, 1e-11):
+        xs = bs.solve_x(rtol=rtol, mixed="reliable", rhs=rhs)
+        for k, (x, s, st) in enumerate(zip(xs, bs.solvers, bs.stats)):
+            print(rtol, k, st, flush=True)
+            assert st["converged"]
+            if k == 1:
+                assert float(x.abs().max()) == 0.0
+                continue
+            f64 = pm.LambdaSolver(images[k], trimaps[k], precision="f64", level0="stencil")
+            f64.solve_system(f64.b, rtol=rtol)
+            assert _true_rel(s, x, s.b) < rtol
+            assert st["iterations"] <= f64.stats["iterations"] + 2
+    with pytest.raises(ValueError):
+        bs.solve_x(mixed="other")
+    # Replacements that are never due (delta 1e-12) on systems where the
+    # float32 recursion stalls above rtol 1e-11: the float64 check fails, the
+    # images restart one by one without disturbing the others, and each ends
+    # below rtol in float64.
+    from pymatting_amd import synthetic
+    im1, tr1, _ = synthetic.config1()
+    im2, tr2, _ = synthetic.make_image(200, 260, 77, strands=8, erode=6)
+    hard = pm.BatchSolver([im1, im2], [tr1, tr2], precision="mixed")
+    xs = hard.solve_x(rtol=1e-11, mixed="reliable", delta=1e-12, maxiter=2000)
+    for x, s, st in zip(xs, hard.solvers, hard.stats):
+        print("restarts", st, flush=True)
+        assert st["converged"]
+        assert _true_rel(s, x, s.b) < 1e-11
