@@ -20,7 +20,7 @@ from pymatting_amd import _ffi
 from pymatting_amd._device import empty
 from pymatting_amd.alpha import LambdaSolver, _finalize
 
-_TUNED = {}  # signature -> (level0_shape, coarse_shape, graph_iters)
+_TUNED = {}  # (signature, mixed) -> (level0_shape, coarse_shape, graph_iters)
 
 SHAPES = (0, 1)
 GRAPH_ITERS = (0, 2, 4, 8, 16)
@@ -119,26 +119,33 @@ class BatchSolver:
         xs = self.solve_x(rtol, inner_rtol, max_outer, maxiter, mixed=mixed, delta=delta)
         return [_finalize(x, s.tdev, None, (s.h, s.w)) for x, s in zip(xs, self.solvers)]
 
-    def autotune(self, rtol=1e-7, inner_rtol=1e-3, maxiter=10000, repeats=1):
+    def autotune(self, rtol=1e-7, inner_rtol=1e-3, maxiter=10000, repeats=1, mixed="refine",
+                 delta=0.01):
         """Chooses tile shapes, then iterations per graph, by timing full solves
-        of this batch; cached per batch signature. Returns the choice."""
-        sig = self.signature()
-        if sig in _TUNED:
-            self.level0_shape, self.coarse_shape, self.graph_iters = _TUNED[sig]
-            return _TUNED[sig]
+        of this batch with the given solve options; cached per batch signature
+        and form. Reliable updates run without graphs: only the shapes are
+        tuned (graph_iters is reported as 0). Returns the choice."""
+        reliable = self.precision != "f64" and mixed == "reliable"
+        key = (self.signature(), "reliable" if reliable else "refine")
+        if key in _TUNED:
+            self.level0_shape, self.coarse_shape, self.graph_iters = _TUNED[key]
+            return _TUNED[key]
+
+        def run():
+            self.solve_x(rtol, inner_rtol, 20, maxiter, mixed=mixed, delta=delta)
 
         def timed():
             best = float("inf")
             for _ in range(repeats):
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
-                self.solve_x(rtol, inner_rtol, 20, maxiter)
+                run()
                 torch.cuda.synchronize()
                 best = min(best, time.perf_counter() - t0)
             return best
 
-        self.graph_iters = 8
-        self.solve_x(rtol, inner_rtol, 20, maxiter)  # warm the allocator and code
+        self.graph_iters = 0 if reliable else 8
+        run()  # warm the allocator and code
         results = {}
         for s0 in SHAPES:
             for sc in SHAPES:
@@ -146,11 +153,12 @@ class BatchSolver:
                 results[(s0, sc)] = timed()
         self.level0_shape, self.coarse_shape = min(results, key=results.get)
         gtimes = {}
-        for g in GRAPH_ITERS:
-            self.graph_iters = g
-            gtimes[g] = timed()
-        self.graph_iters = min(gtimes, key=gtimes.get)
+        if not reliable:
+            for g in GRAPH_ITERS:
+                self.graph_iters = g
+                gtimes[g] = timed()
+            self.graph_iters = min(gtimes, key=gtimes.get)
         self.tuning = {"shapes_s": {"%d,%d" % k: v for k, v in results.items()},
                        "graph_iters_s": gtimes}
-        _TUNED[sig] = (self.level0_shape, self.coarse_shape, self.graph_iters)
-        return _TUNED[sig]
+        _TUNED[key] = (self.level0_shape, self.coarse_shape, self.graph_iters)
+        return _TUNED[key]

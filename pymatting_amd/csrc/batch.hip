@@ -123,8 +123,9 @@ __global__ __launch_bounds__(kBlock) void k_update_x64_batch(const UpdEntryX64* 
     block_partials_at<kBlock, 1>(v, red, blockIdx.x);
 }
 
-// r32 = (float)r64 with partial ||r64||^2 at the block's slot (the update's
-// slot, so the following finalize tests the true residual).
+// r32 = (float)r64 with partial ||r64||^2 at the update's slot of that
+// vector block (lo[img] + local: the launch may cover a subset of the
+// images), so the following finalize tests the true residual.
 struct RepEntry {
     const double* r64;
     float* r32;
@@ -134,6 +135,7 @@ struct RepEntry {
 
 __global__ __launch_bounds__(kBlock) void k_replace_batch(const RepEntry* __restrict__ ents,
                                                           const TileRef* __restrict__ blocks,
+                                                          const unsigned* __restrict__ lo,
                                                           Reducer red) {
     const TileRef br = blocks[blockIdx.x];
     const RepEntry E = ents[br.img];
@@ -150,7 +152,7 @@ __global__ __launch_bounds__(kBlock) void k_replace_batch(const RepEntry* __rest
         }
     }
     double v[1] = {acc};
-    block_partials_at<kBlock, 1>(v, red, blockIdx.x);
+    block_partials_at<kBlock, 1>(v, red, lo[br.img] + br.local);
 }
 
 // gate[i].done = 0 exactly for the images whose replacement is due at
@@ -701,13 +703,14 @@ struct BatchPlan {
     }
 
     // One PCG iteration (p_cur -> p_next) for every image with the solution
-    // in float64 (the reliable-update form, T = float); `after_update` runs
-    // between the update and the preconditioner (residual replacement).
-    int iteration_steps_x64(std::vector<Step>& steps, const std::vector<double*>& x,
-                            const std::vector<T*>& rv, const std::vector<T*>& zv,
-                            const std::vector<T*>& qv, const std::vector<T*>& p_cur,
-                            const std::vector<T*>& p_next, CgState* st,
-                            const std::vector<Step>* after_update) {
+    // in float64 (the reliable-update form, T = float): the SpMV, its
+    // finalize and the update into `steps`, the preconditioner into `tail`
+    // (residual replacements go between the two).
+    int iteration_steps_x64(std::vector<Step>& steps, std::vector<Step>& tail,
+                            const std::vector<double*>& x, const std::vector<T*>& rv,
+                            const std::vector<T*>& zv, const std::vector<T*>& qv,
+                            const std::vector<T*>& p_cur, const std::vector<T*>& p_next,
+                            CgState* st) {
         static_assert(std::is_same<T, float>::value, "reliable updates run float32 CG");
         using ApplyE = BatchEntry<FineOp, InPupdate<T>, OutApply<T, S0>>;
         std::vector<ApplyE> ap;
@@ -734,9 +737,8 @@ struct BatchPlan {
                 return (int)HM_OK;
             });
         }
-        if (after_update) steps.insert(steps.end(), after_update->begin(), after_update->end());
         std::vector<const T*> rc(rv.begin(), rv.end());
-        return vcycle_steps(steps, rc, zv, st, kDotUpdateRz);
+        return vcycle_steps(tail, rc, zv, st, kDotUpdateRz);
     }
 
     // One PCG iteration (p_cur -> p_next) for every image.
@@ -779,6 +781,10 @@ struct BatchStates {
     CgState* dev = nullptr;
     CgState* host = nullptr;
     int count = 0;
+    BatchStates() = default;
+    // Owns the pinned mirror: never copied (closures take dev/host pointers).
+    BatchStates(const BatchStates&) = delete;
+    BatchStates& operator=(const BatchStates&) = delete;
     int init(int n, Arena& ar) {
         count = n;
         HM_TRY(ar.alloc<CgState>(n, &dev));
@@ -950,13 +956,15 @@ int batch_pcg(hm_batch* B, BatchPlan<T, S0, SC>& plan, BatchWork<T>& wk, Arena& 
 // Mixed precision with reliable updates for a batch (see pcg_reliable): one
 // float32 CG per image with the solution in float64. Units of kUnit
 // iterations run back to back and the states are read after each. Each
-// image's residual is replaced by its float64 residual (b - A x, gated per
-// image) at the iteration of the next unit where its norm is predicted, from
-// the image's rate in the last unit, to have dropped by delta since the last
-// replacement. An image that converges on the recursion is confirmed on its
-// float64 residual and, if that fails, restarted from it (beta = 0) through
-// a V-cycle run on a separate state array, so the other images are not
-// touched.
+// image's residual is replaced by its float64 residual (b - A x) at the
+// iteration of the next unit where its norm is predicted, from the image's
+// rate in the last unit, to have dropped by delta since the last
+// replacement; those launches cover only the images due (tile and block
+// lists compacted per unit and copied up once per unit), with a gate that
+// also skips images that converged earlier in the unit. An image that
+// converges on the recursion is confirmed on its float64 residual and, if
+// that fails, restarted from it (beta = 0) through a V-cycle run on a
+// separate state array, so the other images are not touched.
 template <int S0, int SC>
 int batch_reliable(hm_batch* B, Arena& ar, const double* const* b, double* const* x,
                    double rtol, double delta, int maxiter, hm_cg_stats* stats, hipStream_t s) {
@@ -974,22 +982,22 @@ int batch_reliable(hm_batch* B, Arena& ar, const double* const* b, double* const
     HM_TRY(rst.init(count, ar));
     CgState* gate = nullptr;
     HM_TRY(ar.alloc<CgState>(count, &gate));
-    int* at_dev = nullptr;  // replacement iteration per image in the next unit, or -1
+    int* at_dev = nullptr;  // replacement iteration per image in the running unit, or -1
     HM_TRY(ar.alloc<int>(count, &at_dev));
     std::vector<double*> r64(count);
     for (int i = 0; i < count; ++i) HM_TRY(ar.alloc<double>((size_t)B->A[i]->n, &r64[i]));
     std::vector<double*> xv(x, x + count);
     const DotSink nodot{nullptr, Reducer{}, 0};
     const TileRef* vrefs = plan.vec.d_refs;
+    const unsigned* vlo = plan.vec.d_lo;
     const unsigned nvb = (unsigned)plan.vec.refs.size();
     const Reducer red = plan.red, red_upd = plan.red_upd();
-    // Steps: r64 = b - A x (float64, gated), r32 = r64 (gated), and per
-    // iteration k of a unit the gate before them and the note after the
-    // preconditioner.
-    std::vector<Step> resid64, rep32, norms64;
-    std::vector<std::vector<Step>> gate_k(kUnit), note_k(kUnit);
+    // Per-image entries of the float64 residual, the float32 copy and the norm.
+    using ResE = BatchEntry<DiaOp<double, false>, InVec<double>, OutResidual<double, S0>>;
+    ResE* d_res = nullptr;
+    RepEntry* d_rep = nullptr;
+    DotEntry<double>* d_dot = nullptr;
     {
-        using ResE = BatchEntry<DiaOp<double, false>, InVec<double>, OutResidual<double, S0>>;
         std::vector<ResE> re;
         std::vector<RepEntry> rp;
         std::vector<DotEntry<double>> de;
@@ -1001,42 +1009,106 @@ int batch_reliable(hm_batch* B, Arena& ar, const double* const* b, double* const
             rp.push_back(RepEntry{r64[i], wk.r[i], A->n, gate + i});
             de.push_back(DotEntry<double>{r64[i], r64[i], A->n});
         }
-        const int n = count;
-        CgState* sd = st.dev;
-        for (int k = 0; k < kUnit; ++k) {
-            gate_k[k].push_back([=](hipStream_t s2) {
-                k_gate_batch<<<div_up(n, 256), 256, 0, s2>>>(gate, sd, at_dev, k, n);
-                HM_LAUNCH_CHECK();
-                return (int)HM_OK;
-            });
-            note_k[k].push_back([=](hipStream_t s2) {
-                k_note_rr_batch<<<div_up(n, 256), 256, 0, s2>>>(sd, gate, n);
-                HM_LAUNCH_CHECK();
-                return (int)HM_OK;
-            });
-        }
-        HM_TRY((add_tile_step<double, DiaOp<double, false>, InVec<double>,
-                              OutResidual<double, S0>>(resid64, ar, re, plan64.plain[0])));
-        RepEntry* dr = nullptr;
-        HM_TRY(ar.upload(rp, &dr));
-        rep32.push_back([=](hipStream_t s2) {
-            k_replace_batch<<<nvb, kBlock, 0, s2>>>(dr, vrefs, red_upd);
-            HM_LAUNCH_CHECK();
-            return (int)HM_OK;
-        });
-        // ||r64_i||^2 of every image into aux (only the gated ones are used).
-        DotEntry<double>* dd = nullptr;
-        HM_TRY(ar.upload(de, &dd));
-        const unsigned* lo = plan.vec.d_lo;
-        CgState* ad = aux.dev;
-        norms64.push_back([=](hipStream_t s2) {
-            k_dot_batch<double><<<nvb, kBlock, 0, s2>>>(dd, vrefs, red);
-            HM_LAUNCH_CHECK();
-            k_finalize_batch<1><<<n, kFinBlock, 0, s2>>>(red, lo, nullptr, ad, kDotAux, 0);
-            HM_LAUNCH_CHECK();
-            return (int)HM_OK;
-        });
+        HM_TRY(ar.upload(re, &d_res));
+        HM_TRY(ar.upload(rp, &d_rep));
+        HM_TRY(ar.upload(de, &d_dot));
     }
+    // Compacted tile (float64 residual) and vector-block lists of image
+    // subsets: host staging (pinned) and device copies, filled per unit.
+    const TileTable& t64 = plan64.plain[0];
+    const size_t cap_t = t64.refs.size(), cap_v = plan.vec.refs.size();
+    TileRef *h_t = nullptr, *h_v = nullptr, *d_t = nullptr, *d_v = nullptr;
+    HM_TRY(ar.alloc<TileRef>(cap_t, &d_t));
+    HM_TRY(ar.alloc<TileRef>(cap_v, &d_v));
+    HM_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_t), sizeof(TileRef) * cap_t,
+                         hipHostMallocDefault));
+    struct PinGuard { void* p; ~PinGuard() { if (p) (void)hipHostFree(p); } } g_t{h_t};
+    HM_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_v), sizeof(TileRef) * cap_v,
+                         hipHostMallocDefault));
+    PinGuard g_v{h_v};
+    int* at_host = nullptr;
+    HM_HIP(hipHostMalloc(reinterpret_cast<void**>(&at_host), sizeof(int) * count,
+                         hipHostMallocDefault));
+    PinGuard g_a{at_host};
+    struct Span { size_t t0, nt, v0, nv; };
+    // Appends the lists of the images in `sel` and returns where they are.
+    size_t fill_t = 0, fill_v = 0;
+    auto append = [&](const std::vector<char>& sel) {
+        Span sp{fill_t, 0, fill_v, 0};
+        for (int i = 0; i < count; ++i) {
+            if (!sel[i]) continue;
+            for (unsigned k = t64.lo[i]; k < t64.lo[i + 1]; ++k) h_t[fill_t++] = t64.refs[k];
+            for (unsigned k = plan.vec.lo[i]; k < plan.vec.lo[i + 1]; ++k)
+                h_v[fill_v++] = plan.vec.refs[k];
+        }
+        sp.nt = fill_t - sp.t0;
+        sp.nv = fill_v - sp.v0;
+        return sp;
+    };
+    // Copies what was appended since `from` (one copy per list).
+    auto upload = [&](size_t t_from, size_t v_from) -> int {
+        if (fill_t > t_from)
+            HM_HIP(hipMemcpyAsync(d_t + t_from, h_t + t_from, sizeof(TileRef) * (fill_t - t_from),
+                                  hipMemcpyHostToDevice, s));
+        if (fill_v > v_from)
+            HM_HIP(hipMemcpyAsync(d_v + v_from, h_v + v_from, sizeof(TileRef) * (fill_v - v_from),
+                                  hipMemcpyHostToDevice, s));
+        return HM_OK;
+    };
+    // r64 = b - A x on the images of a span (gated), r32 = r64 on them.
+    auto resid_step = [=](const Span sp) -> Step {
+        const TileRef* refs = d_t + sp.t0;
+        const unsigned nt = (unsigned)sp.nt;
+        return [=](hipStream_t s2) {
+            if (nt == 0) return (int)HM_OK;
+            tile_kernel_batch<double, DiaOp<double, false>, InVec<double>,
+                              OutResidual<double, S0>><<<nt, kBlock, 0, s2>>>(d_res, refs, nt);
+            HM_LAUNCH_CHECK();
+            return (int)HM_OK;
+        };
+    };
+    auto copy_step = [=](const Span sp) -> Step {
+        const TileRef* refs = d_v + sp.v0;
+        const unsigned nv = (unsigned)sp.nv;
+        return [=](hipStream_t s2) {
+            if (nv == 0) return (int)HM_OK;
+            k_replace_batch<<<nv, kBlock, 0, s2>>>(d_rep, refs, vlo, red_upd);
+            HM_LAUNCH_CHECK();
+            return (int)HM_OK;
+        };
+    };
+    // The gate of iteration k (images due at k and still iterating) and the
+    // note of the replaced ||r||^2 after the preconditioner.
+    CgState* const st_dev = st.dev;
+    auto gate_step = [=](int k) -> Step {
+        const int n = count;
+        const CgState* sd = st_dev;
+        return [=](hipStream_t s2) {
+            k_gate_batch<<<div_up(n, 256), 256, 0, s2>>>(gate, sd, at_dev, k, n);
+            HM_LAUNCH_CHECK();
+            return (int)HM_OK;
+        };
+    };
+    auto note_step = [=]() -> Step {
+        const int n = count;
+        CgState* sd = st_dev;
+        return [=](hipStream_t s2) {
+            k_note_rr_batch<<<div_up(n, 256), 256, 0, s2>>>(sd, gate, n);
+            HM_LAUNCH_CHECK();
+            return (int)HM_OK;
+        };
+    };
+    // Opens the gate of exactly the images in `sel` (rst is the staging).
+    auto open_gate = [&](const std::vector<char>& sel) -> int {
+        for (int i = 0; i < count; ++i) {
+            rst.host[i] = CgState{};
+            rst.host[i].done = sel[i] ? 0 : 1;
+        }
+        HM_TRY(rst.push(s));
+        HM_HIP(hipMemcpyAsync(gate, rst.dev, sizeof(CgState) * count, hipMemcpyDeviceToDevice,
+                              s));
+        return HM_OK;
+    };
     // ||b_i|| and the initial states.
     {
         std::vector<DotEntry<double>> de;
@@ -1045,8 +1117,7 @@ int batch_reliable(hm_batch* B, Arena& ar, const double* const* b, double* const
         HM_TRY(ar.upload(de, &d));
         k_dot_batch<double><<<nvb, kBlock, 0, s>>>(d, vrefs, red);
         HM_LAUNCH_CHECK();
-        k_finalize_batch<1><<<count, kFinBlock, 0, s>>>(red, plan.vec.d_lo, nullptr, st.dev,
-                                                        kDotAux, 0);
+        k_finalize_batch<1><<<count, kFinBlock, 0, s>>>(red, vlo, nullptr, st.dev, kDotAux, 0);
         HM_LAUNCH_CHECK();
         HM_TRY(st.pull(s));
     }
@@ -1082,20 +1153,14 @@ int batch_reliable(hm_batch* B, Arena& ar, const double* const* b, double* const
         HM_TRY(plan.vcycle_steps(start, rc, wk.z, st.dev, kDotRzFirst));
         HM_TRY(run_steps(start, s));
     }
-    // A unit of kUnit iterations, without replacements and with gated ones
-    // (gate, float64 residual and r32 after each update; note after each
-    // preconditioner).
-    std::vector<Step> unit, unit_rep;
+    // Iteration k of a unit: head (SpMV, finalize, update) and preconditioner.
+    std::vector<std::vector<Step>> head(kUnit), tail(kUnit);
     for (int k = 0; k < kUnit; ++k) {
         const bool even = k % 2 == 0;
         const std::vector<float*>& pc = even ? wk.p : wk.p2;
         const std::vector<float*>& pn = even ? wk.p2 : wk.p;
-        HM_TRY(plan.iteration_steps_x64(unit, xv, wk.r, wk.z, wk.q, pc, pn, st.dev, nullptr));
-        std::vector<Step> after = gate_k[k];
-        after.insert(after.end(), resid64.begin(), resid64.end());
-        after.insert(after.end(), rep32.begin(), rep32.end());
-        HM_TRY(plan.iteration_steps_x64(unit_rep, xv, wk.r, wk.z, wk.q, pc, pn, st.dev, &after));
-        unit_rep.insert(unit_rep.end(), note_k[k].begin(), note_k[k].end());
+        HM_TRY(plan.iteration_steps_x64(head[k], tail[k], xv, wk.r, wk.z, wk.q, pc, pn,
+                                        st.dev));
     }
     // Restart of a subset: z = M r32 with rz into rst (the others are done there).
     std::vector<Step> restart_vc;
@@ -1103,10 +1168,6 @@ int batch_reliable(hm_batch* B, Arena& ar, const double* const* b, double* const
         std::vector<const float*> rc(wk.r.begin(), wk.r.end());
         HM_TRY(plan.vcycle_steps(restart_vc, rc, wk.z, rst.dev, kDotRzFirst));
     }
-    int* at_host = nullptr;
-    HM_HIP(hipHostMalloc(reinterpret_cast<void**>(&at_host), sizeof(int) * count,
-                         hipHostMallocDefault));
-    struct AtGuard { int* p; ~AtGuard() { (void)hipHostFree(p); } } ag{at_host};
     std::vector<char> confirmed(count, 0);
     std::vector<int> rep_at(count, -1);  // none before a rate is known
     std::vector<int> iters_before(count, 0), it_prev(count, 0);
@@ -1117,19 +1178,37 @@ int batch_reliable(hm_batch* B, Arena& ar, const double* const* b, double* const
     }
     int enqueued = 0;
     while (enqueued < maxiter) {
+        // This unit's steps, with the replacements of the images due.
+        fill_t = fill_v = 0;
+        std::vector<Step> unit;
         bool any_rep = false;
         for (int i = 0; i < count; ++i) {
-            any_rep = any_rep || rep_at[i] >= 0;
             iters_before[i] = st.host[i].iters;
             at_host[i] = rep_at[i];
+            any_rep = any_rep || rep_at[i] >= 0;
         }
-        if (any_rep) {
+        if (any_rep)
             HM_HIP(hipMemcpyAsync(at_dev, at_host, sizeof(int) * count, hipMemcpyHostToDevice,
                                   s));
-            HM_TRY(run_steps(unit_rep, s));
-        } else {
-            HM_TRY(run_steps(unit, s));
+        for (int k = 0; k < kUnit; ++k) {
+            std::vector<char> due(count, 0);
+            bool any = false;
+            for (int i = 0; i < count; ++i) {
+                due[i] = rep_at[i] == k;
+                any = any || due[i];
+            }
+            unit.insert(unit.end(), head[k].begin(), head[k].end());
+            if (any) {
+                const Span sp = append(due);
+                unit.push_back(gate_step(k));
+                unit.push_back(resid_step(sp));
+                unit.push_back(copy_step(sp));
+            }
+            unit.insert(unit.end(), tail[k].begin(), tail[k].end());
+            if (any) unit.push_back(note_step());
         }
+        HM_TRY(upload(0, 0));
+        HM_TRY(run_steps(unit, s));
         enqueued += kUnit;
         HM_TRY(st.pull(s));
         std::vector<char> confirm(count, 0);
@@ -1155,17 +1234,29 @@ int batch_reliable(hm_batch* B, Arena& ar, const double* const* b, double* const
         }
         if (any_confirm) {
             // True residuals of the images that converged on the recursion.
-            // Their gates open for the float64 residual (the gate step
-            // opens only images that still iterate).
-            for (int i = 0; i < count; ++i) {
-                rst.host[i] = CgState{};
-                rst.host[i].done = confirm[i] ? 0 : 1;
+            fill_t = fill_v = 0;
+            const Span sp = append(confirm);
+            HM_TRY(upload(0, 0));
+            HM_TRY(open_gate(confirm));
+            HM_TRY(resid_step(sp)(s));
+            k_dot_batch<double><<<(unsigned)sp.nv, kBlock, 0, s>>>(d_dot, d_v + sp.v0, red);
+            HM_LAUNCH_CHECK();
+            // The dot wrote partials at its compacted slots: give every image
+            // its range within them.
+            std::vector<unsigned> lo(count + 1, 0);
+            {
+                unsigned acc = 0;
+                for (int i = 0; i < count; ++i) {
+                    lo[i] = acc;
+                    if (confirm[i]) acc += plan.vec.lo[i + 1] - plan.vec.lo[i];
+                }
+                lo[count] = acc;
             }
-            HM_TRY(rst.push(s));
-            HM_HIP(hipMemcpyAsync(gate, rst.dev, sizeof(CgState) * count,
-                                  hipMemcpyDeviceToDevice, s));
-            HM_TRY(run_steps(resid64, s));
-            HM_TRY(run_steps(norms64, s));
+            unsigned* d_lo = nullptr;
+            HM_TRY(ar.upload(lo, &d_lo));
+            k_finalize_batch<1><<<count, kFinBlock, 0, s>>>(red, d_lo, nullptr, aux.dev,
+                                                            kDotAux, 0);
+            HM_LAUNCH_CHECK();
             HM_TRY(aux.pull(s));
             std::vector<char> restart(count, 0);
             bool any_restart = false;
@@ -1186,16 +1277,18 @@ int batch_reliable(hm_batch* B, Arena& ar, const double* const* b, double* const
                 it_prev[i] = c.iters;
             }
             if (any_restart) {
+                HM_TRY(st.push(s));
+                // r32 = r64 on the restarted images, then z = M r32 for them.
+                fill_t = fill_v = 0;
+                const Span rs = append(restart);
+                HM_TRY(upload(0, 0));
+                HM_TRY(open_gate(restart));
+                HM_TRY(copy_step(rs)(s));
                 for (int i = 0; i < count; ++i) {
                     rst.host[i] = CgState{};
                     rst.host[i].done = restart[i] ? 0 : 1;
                 }
                 HM_TRY(rst.push(s));
-                HM_TRY(st.push(s));
-                // r32 = r64 for the restarted images (their gate is open in rst).
-                HM_HIP(hipMemcpyAsync(gate, rst.dev, sizeof(CgState) * count,
-                                      hipMemcpyDeviceToDevice, s));
-                HM_TRY(run_steps(rep32, s));
                 HM_TRY(run_steps(restart_vc, s));
                 HM_TRY(rst.pull(s));
                 for (int i = 0; i < count; ++i)

@@ -83,6 +83,13 @@ def test_batch_autotune_and_errors():
     assert choice[0] in (0, 1) and choice[1] in (0, 1) and choice[2] in (0, 2, 4, 8, 16)
     assert bs.autotune(rtol=1e-7) == choice  # cached
     bs.solve(rtol=1e-7)
+    # Reliable updates tune the shapes only (no graphs), cached per form.
+    m = pm.BatchSolver([d[0] for d in data], [d[1] for d in data], precision="mixed")
+    rel = m.autotune(rtol=1e-9, mixed="reliable")
+    assert rel[0] in (0, 1) and rel[1] in (0, 1) and rel[2] == 0
+    assert m.autotune(rtol=1e-9, mixed="reliable") == rel
+    for st in m.stats:
+        assert st["converged"]
     with pytest.raises(ValueError):
         pm.BatchSolver([data[0][0]], [data[0][1], data[1][1]])
     # V-cycles with different sweep counts cannot share a batch.
