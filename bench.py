@@ -132,17 +132,33 @@ def pmc_probe():
                       "copy_bytes": a.numel() * 4, "copy_launches": 5}))
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline():
     """The oracle (a CPU restatement of the reference: numpy/scipy CSR assembly,
     vcycle-preconditioned CG at the reference's default rtol 1e-7, float64) on
-    the same 1920x1080 workload, once, in one thread."""
+    the same 1920x1080 workload, once, in one thread. Beside it, the
+    reference's own default path on the same image: estimate_alpha_cf (the
+    unknown-pixel system with the ichol preconditioner, rtol 1e-7)."""
     code = (
         "import sys, time, json; sys.path.insert(0, %r)\n"
-        "from oracle.cf import solve_alpha_lambda\n"
+        "from oracle.cf import solve_alpha_lambda, estimate_alpha_cf\n"
         "from pymatting_amd import synthetic\n"
         "image, trimap, _ = synthetic.config2()\n"
         "t = time.perf_counter(); solve_alpha_lambda(image, trimap); t = time.perf_counter() - t\n"
-        "print(json.dumps({'t': t, 'n': image.shape[0] * image.shape[1]}))\n" % ROOT)
+        "t1 = time.perf_counter(); estimate_alpha_cf(image, trimap)\n"
+        "t1 = time.perf_counter() - t1\n"
+        "print(json.dumps({'t': t, 't_a1': t1, 'n': image.shape[0] * image.shape[1]}))\n"
+        % ROOT)
     env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1",
                HIP_VISIBLE_DEVICES="", CUDA_VISIBLE_DEVICES="")
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env,
@@ -153,7 +169,11 @@ def cpu_baseline():
     return {"value": round(r["n"] / r["t"] / 1e6, 4), "unit": "Mpix/s", "cores": 1,
             "kind": "port",
             "sample": "one 1920x1080 lambda-system solve (cf_laplacian, make_linear_system, "
-                      "vcycle-PCG rtol 1e-7, float64; numpy/scipy CSR), %.1f s" % r["t"]}
+                      "vcycle-PCG rtol 1e-7, float64; numpy/scipy CSR), %.1f s" % r["t"],
+            "estimate_alpha_cf_ichol": {"value": round(r["n"] / r["t_a1"] / 1e6, 4),
+                                        "unit": "Mpix/s", "seconds": round(r["t_a1"], 2)},
+            "affinity_cores": len(os.sched_getaffinity(0)), "host_cpus": os.cpu_count(),
+            "cpu_model": _cpu_model()}
 
 
 def bench_c3(args):
@@ -537,6 +557,29 @@ def main():
     alpha_f, st_f, t_mf = timed(MIXED_MF, args.steps)
     alpha_x, st_x, t_rf = timed(MIXED_REFINE, args.steps)
 
+    # The value form from host buffers (numpy in, numpy out), as the reference's
+    # callers hand them over: adds the image/trimap upload and alpha download.
+    def host_step():
+        kw = dict(MIXED)
+        solver = pm.LambdaSolver(image_h, trimap_h, precision=kw.pop("precision"),
+                                 level0=kw.pop("level0"))
+        return solver.solve(**kw).cpu().numpy()
+
+    def timed_fn(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            r = fn()
+        torch.cuda.synchronize()
+        return r, (time.perf_counter() - t0) / args.steps
+
+    alpha_host, t_host = timed_fn(host_step)
+    # The reference's own entry point on the same image: estimate_alpha_cf (the
+    # unknown-pixel system, V-cycle PCG at its default rtol 1e-7, float64).
+    alpha_a1, t_a1 = timed_fn(lambda: pm.estimate_alpha_cf(image, trimap))
+
     # Converged float64 solution (outside the timed regions).
     conv = pm.LambdaSolver(image, trimap, precision="f64", level0="stencil")
     alpha_c = conv.solve(rtol=1e-11, maxiter=100000)
@@ -592,6 +635,12 @@ def main():
                            "rms_dalpha": e_r[1]},
         "mixed_matrix_free": {"value": round(n / t_mf / 1e6, 3), "unit": "Mpix/s",
                               "iterations": st_f["iterations"], "max_abs_dalpha": e_f[0]},
+        "mixed_host_buffers": {"value": round(n / t_host / 1e6, 3), "unit": "Mpix/s",
+                               "ms_per_step": round(t_host * 1e3, 3),
+                               "max_abs_dalpha_vs_device": float(np.abs(
+                                   alpha_host - alpha_m.cpu().numpy()).max())},
+        "estimate_alpha_cf_fp64": {"value": round(n / t_a1 / 1e6, 3), "unit": "Mpix/s",
+                                   "ms_per_step": round(t_a1 * 1e3, 3), "rtol": 1e-7},
         "mixed_restarted_refinement": {"value": round(n / t_rf / 1e6, 3), "unit": "Mpix/s",
                                        "rtol": MIXED_REFINE["rtol"],
                                        "inner_rtol": MIXED_REFINE["inner_rtol"],

@@ -14,24 +14,45 @@ from pymatting_amd import _ffi
 from pymatting_amd._device import as_device
 
 
+def _min_max(a):
+    """(min, max) of a numpy array, or of a torch tensor reduced where it lives
+    (a device tensor is not copied to the host: two scalars are read back)."""
+    if isinstance(a, torch.Tensor):
+        lo, hi = torch.aminmax(a.detach())
+        return lo.item(), hi.item()
+    return a.min(), a.max()
+
+
+def _dtype_name(a):
+    """(is float32/float64, dtype name as numpy spells it)."""
+    if isinstance(a, torch.Tensor):
+        return a.dtype in (torch.float32, torch.float64), str(a.dtype).replace("torch.", "")
+    return a.dtype in [np.float32, np.float64], a.dtype
+
+
+def check_trimap(trimap, stacklevel=4):
+    """The warnings of util.py:490-507 (range and dtype of a trimap)."""
+    lo, hi = _min_max(trimap)
+    if lo < 0.0:
+        warnings.warn("Trimap values should be in [0, 1], but trimap.min() is %s." % lo,
+                      stacklevel=stacklevel)
+    if hi > 1.0:
+        # util.py:498 reports the minimum in this message; kept for parity.
+        warnings.warn("Trimap values should be in [0, 1], but trimap.max() is %s." % lo,
+                      stacklevel=stacklevel)
+    ok, name = _dtype_name(trimap)
+    if not ok:
+        warnings.warn("Unexpected trimap.dtype %s. Are you sure that you do not want to use "
+                      "np.float32 or np.float64 instead?" % name, stacklevel=stacklevel)
+
+
 def trimap_split(trimap, flatten=True, bg_threshold=0.1, fg_threshold=0.9):
     """util.py:439-525: boolean fg/bg/known/unknown masks of a trimap."""
     if isinstance(trimap, torch.Tensor):
         trimap = trimap.detach().cpu().numpy()
     if flatten:
         trimap = trimap.flatten()
-    lo = trimap.min()
-    hi = trimap.max()
-    if lo < 0.0:
-        warnings.warn("Trimap values should be in [0, 1], but trimap.min() is %s." % lo,
-                      stacklevel=3)
-    if hi > 1.0:
-        # util.py:498 reports the minimum in this message; kept for parity.
-        warnings.warn("Trimap values should be in [0, 1], but trimap.max() is %s." % lo,
-                      stacklevel=3)
-    if trimap.dtype not in [np.float32, np.float64]:
-        warnings.warn("Unexpected trimap.dtype %s. Are you sure that you do not want to use "
-                      "np.float32 or np.float64 instead?" % trimap.dtype, stacklevel=3)
+    check_trimap(trimap)
     is_fg = trimap >= fg_threshold
     is_bg = trimap <= bg_threshold
     if is_bg.sum() == 0:
@@ -43,21 +64,22 @@ def trimap_split(trimap, flatten=True, bg_threshold=0.1, fg_threshold=0.9):
 
 
 def sanity_check_image(image):
-    """util.py:528-578: warnings for shape, range and dtype."""
-    if isinstance(image, torch.Tensor):
-        image = image.detach().cpu().numpy()
+    """util.py:528-578: warnings for shape, range and dtype (a device tensor is
+    checked on the device)."""
     if len(image.shape) != 3 or image.shape[2] != 3:
         warnings.warn("Expected RGB image of shape (?, ?, 3), but image.shape is %s."
-                      % str(image.shape), stacklevel=3)
-    if image.min() < 0.0:
-        warnings.warn("Image values should be in [0, 1], but image.min() is %s." % image.min(),
+                      % str(tuple(image.shape)), stacklevel=3)
+    lo, hi = _min_max(image)
+    if lo < 0.0:
+        warnings.warn("Image values should be in [0, 1], but image.min() is %s." % lo,
                       stacklevel=3)
-    if image.max() > 1.0:
-        warnings.warn("Image values should be in [0, 1], but image.max() is %s." % image.max(),
+    if hi > 1.0:
+        warnings.warn("Image values should be in [0, 1], but image.max() is %s." % hi,
                       stacklevel=3)
-    if image.dtype not in [np.float32, np.float64]:
+    ok, name = _dtype_name(image)
+    if not ok:
         warnings.warn("Unexpected image.dtype %s. Are you sure that you do not want to use "
-                      "np.float32 or np.float64 instead?" % image.dtype, stacklevel=3)
+                      "np.float32 or np.float64 instead?" % name, stacklevel=3)
 
 
 def device_classes(trimap, fg_threshold=0.9, bg_threshold=0.1):

@@ -39,6 +39,50 @@ def test_estimate_alpha_cf_errors():
         pm.cf_laplacian(image.astype(np.float32))
 
 
+def test_device_input_checks():
+    """util.py:439-578 on device tensors: the range checks run on the device
+    and give the messages and errors of the numpy path, also through
+    estimate_alpha_cf; the stored-stencil solve there matches the matrix-free
+    one given the same V-cycle factory."""
+    import warnings
+    import torch
+    import pymatting_amd as pm
+    d = load_golden("lemur_0p1")
+    image = torch.from_numpy(d["image"]).cuda()
+    trimap = torch.from_numpy(d["trimap"]).cuda()
+
+    def caught(fn, *args):
+        with warnings.catch_warnings(record=True) as w:
+            warnings.simplefilter("always")
+            out = fn(*args)
+        return out, [str(x.message) for x in w]
+
+    img = image.clone()
+    img[0, 0, 0], img[1, 1, 1] = -0.25, 1.5
+    _, msgs = caught(pm.sanity_check_image, img)
+    assert msgs == ["Image values should be in [0, 1], but image.min() is -0.25.",
+                    "Image values should be in [0, 1], but image.max() is 1.5."]
+    _, msgs = caught(pm.sanity_check_image, image.half())
+    assert len(msgs) == 1 and msgs[0].startswith("Unexpected image.dtype float16.")
+    t = trimap.clone()
+    t[0, 0] = 1.5  # still foreground; the warning prints the minimum (util.py:498)
+    alpha, msgs = caught(pm.estimate_alpha_cf, image, t)
+    assert msgs == ["Trimap values should be in [0, 1], but trimap.max() is 0.0."]
+    assert float(alpha[0, 0]) == 1.0
+    _, msgs = caught(pm.estimate_alpha_cf, image, trimap)
+    assert msgs == []
+    with pytest.raises(ValueError, match="did not contain background"):
+        pm.estimate_alpha_cf(image, torch.where(trimap <= 0.1, 0.5, trimap))
+    with pytest.raises(ValueError, match="did not contain foreground"):
+        pm.estimate_alpha_cf(image, torch.where(trimap >= 0.9, 0.5, trimap))
+    # Default (stored stencil) against the matrix-free operator with the same
+    # V-cycle, both tight.
+    a = pm.estimate_alpha_cf(image, trimap, cg_kwargs={"rtol": 1e-12})
+    b = pm.estimate_alpha_cf(image, trimap, preconditioner=lambda A: pm.vcycle(A, A.grid_shape),
+                             cg_kwargs={"rtol": 1e-12})
+    assert float((a - b).abs().max()) < 1e-9
+
+
 def test_lambda_solve_config1_fp64_and_mixed():
     """320x240 workload. The float64 GPU solve follows the oracle's iterates
     (same rtol 1e-7, alpha within 1e-6). An rtol-1e-7 solution of this system
