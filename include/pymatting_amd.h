@@ -115,7 +115,11 @@ int hm_csr_apply(hm_csr* A, const double* x, double* y, void* stream);
 /* ---- vcycle: pymatting/preconditioner/vcycle.py:6-153 ----
  * Geometric multigrid on the (h, w) grid of the operator: full-weighting
  * restriction, Galerkin coarse operators R A P, damped Jacobi smoothing and a
- * dense direct solve once h*w <= direct_solve_size. Exactly one of cf / dia is
+ * dense direct solve once h*w <= direct_solve_size. A grid above that size
+ * whose coarse grid (h/2, w/2) would be empty (a 1-row grid, say) ends the
+ * hierarchy: its cycle is num_pre_iter + num_post_iter damped Jacobi sweeps
+ * from zero, which is what the reference's recursion into the empty grid
+ * computes (that sum must then be positive). Exactly one of cf / dia is
  * non-null. prec: HM_F64, or HM_F32 for float32 coefficients and smoothing. */
 int hm_vcycle_create(hm_cf* cf, hm_dia* dia, int h, int w, int num_pre_iter,
                      int num_post_iter, double omega, int direct_solve_size, int prec,

@@ -58,6 +58,26 @@ int vcycle_apply_t(hm_vc* vc, const T* r, T* z, CgState* st, const Reducer* red,
     const Reducer& R = red ? *red : vc->red;
     const DotSink nodot{nullptr, Reducer{}, 0};
     const DotSink dot{st, R, kind};
+    if (L == 1 && vc->coarse_sweeps > 0) {
+        // The finest grid cannot be coarsened (see vc_create): its pre- and
+        // post-smoothing, coarse_sweeps damped Jacobi sweeps from zero on r,
+        // ping-ponging through .x and .t so that the last one lands in z and
+        // forms <r, z>.
+        const LevelOp<T> op = vc_level_op<T>(vc, 0);
+        T* bufs[2] = {vptr<T>(vc->levels[0].x), vptr<T>(vc->levels[0].t)};
+        const int S = vc->coarse_sweeps;
+        const T* prev = nullptr;
+        for (int k = 1; k <= S; ++k) {
+            T* outp = (k == S) ? z : bufs[(S - 1 - k) % 2];
+            const OutSmooth<T> sm{r, op.iD, omega, outp, k == S ? dot : nodot};
+            if (k == 1)
+                HM_TRY(run_level<T>(op, InZero<T>{}, sm, guard, s));
+            else
+                HM_TRY(run_level<T>(op, InVec<T>{prev}, sm, guard, s));
+            prev = outp;
+        }
+        return HM_OK;
+    }
     if (L == 1) {
         k_dense_apply<T><<<1, kBlock, 0, s>>>(vc_inv<T>(vc), r, z, vc->n_direct, st, R, guard);
         HM_LAUNCH_CHECK();
@@ -908,6 +928,13 @@ int vc_create(hm_cf* cf, hm_dia* dia, int h, int w, int pre, int post, double om
     int rc = HM_OK;
     hm_dia* fine = dia;
     hm_dia* tmp0 = nullptr;
+    // A grid above the direct-solve size whose coarse grid (h/2, w/2) would
+    // be empty ends the hierarchy as a smoothing-only level. The reference
+    // recurses into the empty grid (vcycle.py:59-60 solves a 0 x 0 system),
+    // so that level's cycle is its pre-smoothing from zero, a zero
+    // correction and its post-smoothing: pre + post damped Jacobi sweeps
+    // from zero.
+    bool smooth_only = (long long)h * w > direct && (h / 2 == 0 || w / 2 == 0);
     // Level 0.
     VcLevel L0;
     L0.h = h;
@@ -931,7 +958,7 @@ int vc_create(hm_cf* cf, hm_dia* dia, int h, int w, int pre, int post, double om
         vc->cf = cf;
         L0.A = dia;  // null when matrix-free
         if (cf) rc = cf_prepare(cf, kCfDiag | (prec == HM_F32 ? kCfF32 : 0), s);
-        if (rc == HM_OK && (pre >= 2 || post >= 2)) {
+        if (rc == HM_OK && (pre >= 2 || post >= 2 || smooth_only)) {
             rc = dev_alloc(&L0.x, es * n0);
             if (rc == HM_OK) rc = dev_alloc(&L0.t, es * n0);
             if (rc == HM_OK) rc = dev_alloc(&L0.u, es * n0);
@@ -944,7 +971,7 @@ int vc_create(hm_cf* cf, hm_dia* dia, int h, int w, int pre, int post, double om
             fine = tmp0;
         }
         int ch = h, cw = w;
-        while (true) {
+        while (!smooth_only) {
             ch /= 2;
             cw /= 2;
             VcLevel Lc;
@@ -966,6 +993,10 @@ int vc_create(hm_cf* cf, hm_dia* dia, int h, int w, int pre, int post, double om
             fine = Lc.A;
             if (Lc.n <= direct) break;
             if (max_levels > 0 && (int)vc->levels.size() >= max_levels) break;
+            if (ch / 2 == 0 || cw / 2 == 0) {
+                smooth_only = true;
+                break;
+            }
         }
         if (tmp0) {
             HM_HIP(hipStreamSynchronize(s));
@@ -973,6 +1004,21 @@ int vc_create(hm_cf* cf, hm_dia* dia, int h, int w, int pre, int post, double om
             tmp0 = nullptr;
         }
         if (rc != HM_OK) { vc_free(vc); return rc; }
+    }
+    if (smooth_only) {
+        if (pre + post == 0) {
+            set_error("the (%d x %d) grid cannot be coarsened and V(0,0) smooths nothing on it; "
+                      "give num_pre_iter + num_post_iter > 0",
+                      vc->levels.back().h, vc->levels.back().w);
+            vc_free(vc);
+            return HM_ERR_ARG;
+        }
+        vc->coarse_sweeps = pre + post;
+        rc = make_reducer(&vc->red, reducer_blocks(n0, h, w), 2);
+        if (rc != HM_OK) { vc_free(vc); return rc; }
+        HM_HIP(hipStreamSynchronize(s));
+        *out = vc;
+        return HM_OK;
     }
     const hm_dia* last = vc->levels.back().A;
     if (vc->levels.size() > 1 && last->n > kMaxDirect) {

@@ -152,3 +152,75 @@ def test_direct_coarse_inverse(n):
     S[-1, :] = 0.0
     with pytest.raises(ValueError):
         VCycle(S, (1, n), direct_solve_size=n)
+
+
+def _thin_case(shape):
+    rng = np.random.default_rng(shape[0] * 1000 + shape[1])
+    image = rng.random(shape + (3,))
+    trimap = rng.choice([0.0, 0.5, 1.0], size=shape, p=[0.3, 0.4, 0.3])
+    trimap.flat[0], trimap.flat[-1] = 0.0, 1.0
+    return image, trimap, rng
+
+
+@pytest.mark.parametrize("shape", [(3, 200), (200, 3), (12, 700)])
+def test_thin_images_reach_an_empty_grid(shape):
+    """Coarsening that reaches an empty grid above the direct-solve size
+    (3x200 -> 1x100 -> 0x50; 12x700 -> ... -> 1x87 -> 0x43). The reference
+    recurses into the empty grid (vcycle.py:59-60 solves the 0x0 system), so
+    the last non-empty grid is smoothed with a zero coarse correction. The
+    V-cycle, PCG with it, the device solvers and estimate_alpha_cf match the
+    oracle."""
+    import pymatting_amd as pm
+    image, trimap, rng = _thin_case(shape)
+    print("shape", shape, flush=True)
+    A, b = pm.make_linear_system(pm.cf_laplacian(image), trimap)
+    Ar, br = ocf.make_linear_system(ocf.cf_laplacian(image), trimap)
+    for pre, post in [(1, 1), (2, 1), (0, 2)]:
+        M = pm.vcycle(A, shape, num_pre_iter=pre, num_post_iter=post)
+        Mr = osol.vcycle(Ar, shape, num_pre_iter=pre, num_post_iter=post)
+        r = rng.standard_normal(b.size)
+        z, zr = M(r), Mr(r)
+        assert np.abs(z - zr).max() <= 1e-8 * np.abs(zr).max(), (pre, post)
+    cb, cbr = pm.CounterCallback(), osol.CounterCallback()
+    x = pm.cg(A, b, M=pm.vcycle(A, shape), rtol=1e-10, callback=cb)
+    xr = osol.cg(Ar, br, M=osol.vcycle(Ar, shape), rtol=1e-10, callback=cbr)
+    assert abs(cb.n - cbr.n) <= 1, (cb.n, cbr.n)
+    assert np.abs(x - xr).max() < 1e-6
+    ref = np.clip(xr, 0, 1).reshape(shape)
+    for prec, level0, kw in [("f64", "stencil", {}), ("f64", "matrix_free", {}),
+                             ("mixed", "stencil", {"mixed": "reliable"}),
+                             ("mixed", "stencil", {"mixed": "refine", "inner_rtol": 1e-3})]:
+        s = pm.LambdaSolver(image, trimap, precision=prec, level0=level0)
+        alpha = s.solve(rtol=1e-10, **kw).cpu().numpy()
+        assert np.abs(alpha - ref).max() < 1e-6, (prec, level0, kw)
+    alpha = pm.estimate_alpha_cf(image, trimap, cg_kwargs={"rtol": 1e-12})
+    ref_a1 = ocf.estimate_alpha_cf(image, trimap, cg_kwargs={"rtol": 1e-12})
+    assert np.abs(alpha - ref_a1).max() < 1e-6
+
+
+@pytest.mark.parametrize("shape", [(1, 200), (200, 1)])
+def test_single_grid_that_cannot_be_coarsened(shape):
+    """A 200-point grid of one row (or column) has an empty coarse grid: the
+    V-cycle is the pre- and post-smoothing alone, pre + post damped Jacobi
+    sweeps from zero, as in the oracle's recursion. A SPD 1-D operator of
+    reach 2 along the grid."""
+    import pymatting_amd as pm
+    n = 200
+    rng = np.random.default_rng(9)
+    A = scipy.sparse.diags([6.0 + rng.random(n), -1.5 * np.ones(n - 1), -1.5 * np.ones(n - 1),
+                            -0.5 * np.ones(n - 2), -0.5 * np.ones(n - 2)],
+                           [0, 1, -1, 2, -2], format="csr")
+    for pre, post in [(1, 1), (2, 2), (0, 1), (3, 0)]:
+        M = pm.vcycle(A, shape, num_pre_iter=pre, num_post_iter=post)
+        Mr = osol.vcycle(A, shape, num_pre_iter=pre, num_post_iter=post)
+        r = rng.standard_normal(n)
+        z, zr = M(r), Mr(r)
+        assert np.abs(z - zr).max() <= 1e-13 * np.abs(zr).max(), (pre, post)
+    b = rng.standard_normal(n)
+    cb, cbr = pm.CounterCallback(), osol.CounterCallback()
+    x = pm.cg(A, b, M=pm.vcycle(A, shape), rtol=1e-12, callback=cb)
+    xr = osol.cg(A, b, M=osol.vcycle(A, shape), rtol=1e-12, callback=cbr)
+    assert abs(cb.n - cbr.n) <= 1, (cb.n, cbr.n)
+    assert np.abs(x - xr).max() < 1e-10 * np.abs(xr).max()
+    with pytest.raises(ValueError, match="num_pre_iter"):
+        pm.vcycle(A, shape, num_pre_iter=0, num_post_iter=0)
