@@ -1,0 +1,104 @@
+"""Properties at the full workload sizes: 1920x1080 and 3840x2160.
+
+The oracle takes minutes at these sizes, so these tests check what holds at
+any size. L 1 = 0 for the closed-form Laplacian (the centred colours of a
+window sum to zero), so the lambda system maps the ones vector to
+lambda * is_known. The stored stencil is symmetric and agrees with the
+matrix-free operator. The mixed solves reach the converged float64 alpha
+within 1e-4, with a float64 residual at their tolerance and the float64
+iteration count.
+"""
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+LAMBDA = 100.0
+
+
+def _device(image, trimap):
+    import torch
+    return torch.from_numpy(image).cuda(), torch.from_numpy(trimap).cuda()
+
+
+def _abs_row_sums(op):
+    """sum_j |A_ij| from the stored [25][n] stencil."""
+    return op.coef.abs().sum(dim=0)
+
+
+def test_1080p_operator_properties():
+    import torch
+    import pymatting_amd as pm
+    from pymatting_amd import synthetic
+    from pymatting_amd.laplacian import DiaOperator
+    image_h, trimap_h, _ = synthetic.config2()
+    image, trimap = _device(image_h, trimap_h)
+    print("1080p operator", image.shape, flush=True)
+    s = pm.LambdaSolver(image, trimap, precision="f64", level0="stencil")
+    op, cf = s.op, s.cf
+    assert isinstance(op, DiaOperator)
+    n = op.n
+    known = torch.from_numpy(((trimap_h <= 0.1) | (trimap_h >= 0.9)).ravel()).cuda()
+    row_abs = _abs_row_sums(op)
+    ones = torch.ones(n, dtype=torch.float64, device="cuda")
+    # A 1 = lambda * known, to rounding of each row's sum.
+    d = (op.apply(ones) - LAMBDA * known.double()).abs()
+    assert bool((d <= 1e-12 * row_abs).all()), float((d / row_abs).max())
+    # Symmetry: <A x, y> = <x, A y>.
+    g = torch.Generator(device="cuda").manual_seed(7)
+    x = torch.randn(n, dtype=torch.float64, device="cuda", generator=g)
+    y = torch.randn(n, dtype=torch.float64, device="cuda", generator=g)
+    lhs, rhs = float(op.apply(x) @ y), float(x @ op.apply(y))
+    bound = float(row_abs @ (x.abs() * y.abs().max()))
+    assert abs(lhs - rhs) <= 1e-12 * bound
+    # Matrix-free against stored: rounding of |A| |x|.
+    diff = (cf.apply(x) - op.apply(x)).abs()
+    assert float(diff.max()) <= 1e-10 * float((row_abs * x.abs().max()).max())
+    # The right-hand side is lambda * is_fg.
+    fg = torch.from_numpy((trimap_h >= 0.9).ravel()).cuda()
+    assert torch.equal(s.b, LAMBDA * fg.double())
+
+
+def test_1080p_mixed_reliable_accuracy():
+    import torch
+    import pymatting_amd as pm
+    from pymatting_amd import synthetic
+    image_h, trimap_h, _ = synthetic.config2()
+    image, trimap = _device(image_h, trimap_h)
+    print("1080p solves", flush=True)
+    conv = pm.LambdaSolver(image, trimap, precision="f64", level0="stencil")
+    alpha_c = conv.solve(rtol=1e-11, maxiter=100000)
+    f64 = pm.LambdaSolver(image, trimap, precision="f64", level0="stencil")
+    f64.solve(rtol=1e-9)
+    m = pm.LambdaSolver(image, trimap, precision="mixed", level0="stencil")
+    alpha_m = m.solve(rtol=1e-9, mixed="reliable")
+    assert float((alpha_m - alpha_c).abs().max()) <= 1e-4
+    assert abs(m.stats["iterations"] - f64.stats["iterations"]) <= 1
+    # The float64 residual of the mixed solution meets the tolerance.
+    r = conv.b - conv.op.apply(m.x)
+    # (1% for the norms' own rounding: the solver stops just below the bound.)
+    assert float(torch.linalg.norm(r)) <= 1.01e-9 * float(torch.linalg.norm(conv.b))
+
+
+def test_4k_five_level_mixed_accuracy():
+    """C4's preconditioner: 5 grids, V(2,2), the 240x135 coarsest grid by two
+    damped Jacobi sweeps; reliable updates at rtol 4e-10 and restarted
+    refinement at rtol 1e-8 both land within 1e-4 of the converged alpha."""
+    import torch
+    import pymatting_amd as pm
+    from pymatting_amd import synthetic
+    image_h, trimap_h, _ = synthetic.config4()
+    image, trimap = _device(image_h, trimap_h)
+    print("4K solves", image.shape, flush=True)
+    base = dict(precision="mixed", level0="stencil", num_pre_iter=2, num_post_iter=2,
+                max_levels=5, coarse_sweeps=2)
+    conv = pm.LambdaSolver(image, trimap, precision="f64", level0="stencil")
+    alpha_c = conv.solve(rtol=1e-11, maxiter=100000)
+    rel = pm.LambdaSolver(image, trimap, **base)
+    alpha_r = rel.solve(rtol=4e-10, mixed="reliable")
+    assert rel.M.levels == 5
+    assert float((alpha_r - alpha_c).abs().max()) <= 1e-4
+    r = conv.b - conv.op.apply(rel.x)
+    assert float(torch.linalg.norm(r)) <= 1.01 * 4e-10 * float(torch.linalg.norm(conv.b))
+    ref = pm.LambdaSolver(image, trimap, **base)
+    alpha_f = ref.solve(rtol=1e-8, inner_rtol=1e-3)
+    assert float((alpha_f - alpha_c).abs().max()) <= 1e-4
