@@ -32,6 +32,25 @@ def test_bit_exact_float32(shape):
     assert np.array_equal(B, Br), np.abs(B - Br).max()
 
 
+@pytest.mark.parametrize("shape", [(1, 37), (37, 1), (2, 150), (150, 2), (3, 300), (1, 2),
+                                   (70, 3)])
+def test_bit_exact_float32_thin(shape):
+    """Images one to three pixels thin: every level of the pyramid is a strip
+    of one or two rows or columns (the sweep's strips and halos at their
+    narrowest)."""
+    import pymatting_amd as pm
+    rng = np.random.default_rng(shape[0] * 1000 + shape[1])
+    image = rng.random(shape + (3,))
+    alpha = np.clip(rng.random(shape) * 1.4 - 0.2, 0, 1)
+    print("shape", shape, flush=True)
+    F, B = pm.estimate_foreground_ml(image, alpha, return_background=True)
+    Fr, Br = ofg(image, alpha, return_background=True, mean_mode="f64")
+    assert np.array_equal(F, Fr), np.abs(F - Fr).max()
+    assert np.array_equal(B, Br), np.abs(B - Br).max()
+    F64 = pm.estimate_foreground_ml(image, alpha, precision="f64")
+    assert np.abs(F64 - ofg(image, alpha, precision="f64")).max() < 1e-12
+
+
 @pytest.mark.parametrize("shape", [(9, 13), (130, 201)])
 def test_float64(shape):
     import pymatting_amd as pm
