@@ -271,9 +271,11 @@ def bench_c3(args):
 def bench_c4(args):
     """3840x2160 (unknown band 40 px): mixed form with a 5-level V-cycle, two
     pre- and two post-smoothing float32 Jacobi sweeps and an approximate
-    coarsest solve (240x135 grid, 2 damped Jacobi sweeps), float64 outer
-    refinement; stored float32 stencil against the matrix-free stencil; then
-    estimate_foreground_ml in float32 against float64 on the solved alpha."""
+    coarsest solve (240x135 grid, 2 damped Jacobi sweeps). The value: stored
+    float32 stencil with reliable updates at rtol 4e-10; beside it restarted
+    refinement (rtol 1e-8, inner 1e-3) on the stored and on the matrix-free
+    stencil. Then estimate_foreground_ml in float32 against float64 on the
+    converged alpha."""
     import numpy as np
     import torch
     import pymatting_amd as pm
@@ -328,33 +330,42 @@ def bench_c4(args):
     # against float64.
     a_np = alpha_c.clamp(0, 1)
     img32, a32 = image.float(), a_np.float()
+    for _ in range(args.warmup):
+        pm.estimate_foreground_ml(img32, a32, return_background=True, precision="f32")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    F32, B32 = pm.estimate_foreground_ml(img32, a32, return_background=True, precision="f32")
+    for _ in range(args.steps):
+        F32, B32 = pm.estimate_foreground_ml(img32, a32, return_background=True,
+                                             precision="f32")
     torch.cuda.synchronize()
-    t_fg = time.perf_counter() - t0
+    t_fg = (time.perf_counter() - t0) / args.steps
     F64, B64 = pm.estimate_foreground_ml(image, a_np, return_background=True, precision="f64")
     dF = float((F32.double() - F64).abs().max())
     dB = float((B32.double() - B64).abs().max())
     line = {
         "metric": "alpha Mpix/s at 4K: mixed 5-level V(2,2) PCG, stored FP32 stencil",
-        "value": round(n / t_s / 1e6, 3), "unit": "Mpix/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_s * 1e3, 3),
+        "value": round(n / t_r / 1e6, 3), "unit": "Mpix/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_r * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": "3840x2160 lambda-system alpha solve, unknown band 40 px, mixed "
                                "float32/float64, 5-level V-cycle (2 pre, 2 post sweeps, coarsest "
-                               "240x135 by 2 Jacobi sweeps), rtol 1e-8, inner 1e-3",
+                               "240x135 by 2 Jacobi sweeps), reliable updates (float32 CG, "
+                               "float64 solution and residual replacement), rtol 4e-10",
                    "height": h, "width": w},
-        "stored_fp32": {"iterations": st_s["iterations"], "max_abs_dalpha": err(alpha_s)[0],
-                        "rms_dalpha": err(alpha_s)[1]},
-        "matrix_free_fp32": {"value": round(n / t_f / 1e6, 3), "ms_per_step": round(t_f * 1e3, 3),
-                             "iterations": st_f["iterations"],
-                             "max_abs_dalpha": err(alpha_f)[0]},
-        "stored_fp32_reliable_updates": {"value": round(n / t_r / 1e6, 3),
-                                         "ms_per_step": round(t_r * 1e3, 3), "rtol": 4e-10,
+        "stored_fp32_reliable_updates": {"rtol": 4e-10, "delta": 0.01,
                                          "iterations": st_r["iterations"],
-                                         "max_abs_dalpha": err(alpha_r)[0]},
+                                         "max_abs_dalpha": err(alpha_r)[0],
+                                         "rms_dalpha": err(alpha_r)[1]},
+        "stored_fp32_refinement": {"value": round(n / t_s / 1e6, 3),
+                                   "ms_per_step": round(t_s * 1e3, 3), "rtol": RTOL,
+                                   "inner_rtol": INNER, "iterations": st_s["iterations"],
+                                   "max_abs_dalpha": err(alpha_s)[0],
+                                   "rms_dalpha": err(alpha_s)[1]},
+        "matrix_free_fp32_refinement": {"value": round(n / t_f / 1e6, 3),
+                                        "ms_per_step": round(t_f * 1e3, 3), "rtol": RTOL,
+                                        "iterations": st_f["iterations"],
+                                        "max_abs_dalpha": err(alpha_f)[0]},
         "foreground_f32": {"ms": round(t_fg * 1e3, 3), "Mpix_per_s": round(n / t_fg / 1e6, 2),
                            "max_abs_dF_vs_f64": dF, "max_abs_dB_vs_f64": dB},
     }
