@@ -74,12 +74,14 @@ def bytes_per_point(kind, s):
         "coarse_smooth": 13 * s + 2 * s + s / 4 + s,
         "coarse_residual": 13 * s + 3 * s,
         "cg_update_x64": 8 + s + 2 * s + s + 8,            # x (f64), p, r, q -> x, r
+        "cg_update_r": 3 * s,                               # r, q -> r (x beside the tail)
     }[kind]
 
 
 KINDS = ["cf_apply", "cf_restrict", "cf_smooth", "cf_residual", "dia_apply", "dia_restrict",
          "dia_smooth", "dia_residual", "cg_update", "cg_pupdate", "tail", "fg_sweep",
-         "coarse_apply", "coarse_restrict", "coarse_smooth", "coarse_residual", "cg_update_x64"]
+         "coarse_apply", "coarse_restrict", "coarse_smooth", "coarse_residual", "cg_update_x64",
+         "cg_update_r"]
 
 
 # Algorithmic bytes per grid point of the stored-stencil SpMV y = A x:

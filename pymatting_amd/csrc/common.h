@@ -137,6 +137,7 @@ enum ProfKind {
     kProfCgUpdate, kProfCgPupdate, kProfDense, kProfFgSweep,
     kProfCoarseApply, kProfCoarseRestrict, kProfCoarseSmooth, kProfCoarseResidual,
     kProfCgUpdateX64,  // float32 CG update with the solution in float64
+    kProfCgUpdateR,    // CG update of r alone (x updated beside the V-cycle tail)
     kProfCount
 };
 

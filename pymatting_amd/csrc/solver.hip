@@ -49,11 +49,16 @@ __global__ void k_axpy_f32_to_f64(double* __restrict__ x, double scale,
 
 // z = V-cycle(r) (vcycle.py:46-100). If st is given, the final kernel also
 // forms <r, z> and finalises it as `kind`, through `red` (the caller's
-// reducer, which a deferred CG update shares) or the V-cycle's own.
+// reducer, which a deferred CG update shares) or the V-cycle's own. `side`
+// (only when vc_has_tail) is work for the tail's launch to carry.
 template <typename T>
 int vcycle_apply_t(hm_vc* vc, const T* r, T* z, CgState* st, const Reducer* red, int kind,
-                   const CgState* guard, hipStream_t s) {
+                   const CgState* guard, hipStream_t s, const TailSide* side) {
     const size_t L = vc->levels.size();
+    if (side && side->x && !vc_has_tail(vc)) {
+        set_error("internal: tail side work for a V-cycle without a tail");
+        return HM_ERR_ARG;
+    }
     const T omega = (T)vc->omega;
     const Reducer& R = red ? *red : vc->red;
     const DotSink nodot{nullptr, Reducer{}, 0};
@@ -137,7 +142,7 @@ int vcycle_apply_t(hm_vc* vc, const T* r, T* z, CgState* st, const Reducer* red,
             }
         }
     } else if (tail < L - 1) {
-        HM_TRY(launch_tail<T>(vc, tail, guard, s));
+        HM_TRY(launch_tail<T>(vc, tail, guard, s, side));
     } else {
         const VcLevel& cl = vc->levels[L - 1];
         k_dense_apply<T><<<1, kBlock, 0, s>>>(vc_inv<T>(vc), vptr<T>(cl.b), vptr<T>(cl.x),
@@ -185,12 +190,12 @@ int vcycle_apply_t(hm_vc* vc, const T* r, T* z, CgState* st, const Reducer* red,
 }
 
 int vcycle_apply(hm_vc* vc, const void* r, void* z, CgState* st, const Reducer* red, int kind,
-                 const CgState* guard, hipStream_t s) {
+                 const CgState* guard, hipStream_t s, const TailSide* side = nullptr) {
     if (vc->prec == HM_F64)
         return vcycle_apply_t<double>(vc, static_cast<const double*>(r), static_cast<double*>(z),
-                                      st, red, kind, guard, s);
+                                      st, red, kind, guard, s, side);
     return vcycle_apply_t<float>(vc, static_cast<const float*>(r), static_cast<float*>(z), st,
-                                 red, kind, guard, s);
+                                 red, kind, guard, s, side);
 }
 
 }  // namespace hm
@@ -318,8 +323,12 @@ int op_residual(const OpHandle& A, const T* b, const T* x, T* r, hipStream_t s) 
 
 template <typename T>
 int pc_apply(const PcHandle& M, const T* r, T* z, CgState* st, const Reducer& red, int kind,
-             const CgState* guard, long long n, hipStream_t s) {
-    if (M.kind == HM_PC_VCYCLE) return vcycle_apply(M.vc, r, z, st, &red, kind, guard, s);
+             const CgState* guard, long long n, hipStream_t s, const TailSide* side = nullptr) {
+    if (M.kind == HM_PC_VCYCLE) return vcycle_apply(M.vc, r, z, st, &red, kind, guard, s, side);
+    if (side && side->x) {
+        set_error("internal: tail side work without a V-cycle");
+        return HM_ERR_ARG;
+    }
     if (M.kind == HM_PC_CSR || M.kind == HM_PC_HOST) {
         if constexpr (std::is_same<T, double>::value) {
             if (M.kind == HM_PC_CSR)
@@ -400,9 +409,11 @@ struct PcgOptions {
 // alpha = rz / pq, and updates x and r over its grid-stride share with
 // ||r||^2 partials; workgroup 0 records pq and alpha in the state. x is
 // float64 in the reliable-update solve (TX = double, T = float), else T.
+// With kUpdX false only r is updated: x += alpha p then runs in the
+// V-cycle tail's launch (TailSide), with the alpha recorded here.
 constexpr int kUpdBlocks = 256;
 
-template <typename TX, typename T>
+template <typename TX, typename T, bool kUpdX = true>
 __global__ __launch_bounds__(kFinBlock) void k_cg_update_pq(TX* __restrict__ x,
                                                             const T* __restrict__ p,
                                                             T* __restrict__ r,
@@ -435,8 +446,10 @@ __global__ __launch_bounds__(kFinBlock) void k_cg_update_pq(TX* __restrict__ x,
         for (int u = 0; u < U; ++u) {
             const long long i = i0 + u * stride;
             const bool in = i < n;
-            xv[u] = in ? x[i] : TX(0);
-            pv[u] = in ? p[i] : T(0);
+            if constexpr (kUpdX) {
+                xv[u] = in ? x[i] : TX(0);
+                pv[u] = in ? p[i] : T(0);
+            }
             rv[u] = in ? r[i] : T(0);
             qv[u] = in ? q[i] : T(0);
         }
@@ -444,10 +457,12 @@ __global__ __launch_bounds__(kFinBlock) void k_cg_update_pq(TX* __restrict__ x,
         for (int u = 0; u < U; ++u) {
             const long long i = i0 + u * stride;
             if (i >= n) continue;
-            if constexpr (std::is_same<TX, T>::value)
+            if constexpr (!kUpdX) {
+            } else if constexpr (std::is_same<TX, T>::value) {
                 x[i] = xv[u] + alphaT * pv[u];
-            else
+            } else {
                 x[i] = xv[u] + alpha * (TX)pv[u];
+            }
             const T ri = rv[u] - alphaT * qv[u];
             r[i] = ri;
             acc += (double)ri * (double)ri;
@@ -539,6 +554,10 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
     // update kernel, which sums the SpMV's partials itself.
     const bool fused = defer && (A.cf || A.dia);
     if (fused) red.upd_blocks = kUpdBlocks;
+    // In that mode, with an LDS tail in the V-cycle, x += alpha p runs in the
+    // tail's launch on the CUs the tail leaves idle; the update kernel only
+    // updates r.
+    const bool x_in_tail = fused && M.kind == HM_PC_VCYCLE && vc_has_tail(M.vc);
     red_upd.upd_blocks = red.upd_blocks;
     int enqueued = 0;
     DotSink pq{st.dev, red, kDotPQ};
@@ -555,8 +574,12 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
             HM_TRY(op_apply_pupdate<T>(A, z.as<T>(), p_cur, p_next, q.as<T>(), pq, st.dev, s));
             std::swap(p_cur, p_next);
             {
-                ProfScope prof_u(kProfCgUpdate, n, sizeof(T) == 4, s);
-                if (fused)
+                ProfScope prof_u(x_in_tail ? kProfCgUpdateR : kProfCgUpdate, n, sizeof(T) == 4,
+                                 s);
+                if (x_in_tail)
+                    k_cg_update_pq<T, T, false><<<kUpdBlocks, kFinBlock, 0, s>>>(
+                        nullptr, p_cur, r.as<T>(), q.as<T>(), n, st.dev, red, pq_blocks, red_upd);
+                else if (fused)
                     k_cg_update_pq<T, T><<<kUpdBlocks, kFinBlock, 0, s>>>(
                         x, p_cur, r.as<T>(), q.as<T>(), n, st.dev, red, pq_blocks, red_upd);
                 else
@@ -580,8 +603,10 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
                 HM_TRY(pc_apply<T>(M, r.as<T>(), z.as<T>(), nullptr, red, 0, st.dev, n, s));
                 HM_TRY(launch_dot<T>(r.as<T>(), z.as<T>(), rold.as<T>(), n, st.dev, red, kDotRzUpdate, s));
             } else {
+                const TailSide side{x, p_cur, n, st.dev, std::is_same<T, double>::value ? 1 : 0};
                 HM_TRY(pc_apply<T>(M, r.as<T>(), z.as<T>(), st.dev, red,
-                                   defer ? kDotUpdateRz : kDotRzUpdate, st.dev, n, s));
+                                   defer ? kDotUpdateRz : kDotRzUpdate, st.dev, n, s,
+                                   x_in_tail ? &side : nullptr));
             }
         }
         enqueued += k;
@@ -750,6 +775,10 @@ int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x
                            s));
     float* p_cur = p.as<float>();
     float* p_next = p2.as<float>();
+    // With an LDS tail in the V-cycle, x += alpha p runs in the tail's launch
+    // on the CUs the tail leaves idle, and the update kernel only updates r;
+    // replacement iterations need x first and update both.
+    const bool x_beside_tail = vc_has_tail(vc32);
     // The state is read back every K iterations. A replacement is placed at
     // the iteration where the residual norm is predicted (from the rate of
     // the last batch) to have dropped by delta since the last replacement;
@@ -775,7 +804,14 @@ int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x
             HM_TRY(op_apply_pupdate<float>(A, z.as<float>(), p_cur, p_next, q.as<float>(), pq,
                                            st.dev, s));
             std::swap(p_cur, p_next);
-            {
+            const bool x_in_tail = x_beside_tail && !replace;
+            if (x_in_tail) {
+                ProfScope prof_u(kProfCgUpdateR, n, 1, s);
+                k_cg_update_pq<double, float, false><<<kUpdBlocks, kFinBlock, 0, s>>>(
+                    nullptr, p_cur, r.as<float>(), q.as<float>(), n, st.dev, red, pq_blocks,
+                    red_upd);
+                HM_LAUNCH_CHECK();
+            } else {
                 ProfScope prof_u(kProfCgUpdateX64, n, 1, s);
                 k_cg_update_pq<double, float><<<kUpdBlocks, kFinBlock, 0, s>>>(
                     x, p_cur, r.as<float>(), q.as<float>(), n, st.dev, red, pq_blocks, red_upd);
@@ -788,8 +824,9 @@ int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x
                                                   red_upd);
                 HM_LAUNCH_CHECK();
             }
+            const TailSide side{x, p_cur, n, st.dev, 1};
             HM_TRY(pc_apply<float>(M, r.as<float>(), z.as<float>(), st.dev, red, kDotUpdateRz,
-                                   st.dev, n, s));
+                                   st.dev, n, s, x_in_tail ? &side : nullptr));
             if (replace) {
                 k_note_rr<<<1, 1, 0, s>>>(st.dev);
                 HM_LAUNCH_CHECK();

@@ -2,6 +2,8 @@
 // the single-system drivers (solver.hip) and the batched drivers (batch.hip).
 #pragma once
 
+#include <type_traits>
+
 #include <algorithm>
 #include <vector>
 
@@ -580,9 +582,67 @@ __device__ __forceinline__ void vc_tail_body(const TailParams<T>& P) {
     }
 }
 
-template <typename T>
-__global__ __launch_bounds__(kTailBlock) void k_vc_tail(TailParams<T> P, const CgState* guard) {
+// Work that rides in the tail's launch on the CUs the tail leaves idle: the
+// CG solution update x += alpha p (p in the tail's type, x in float64 or in
+// the tail's type, alpha of the current iteration from the solve state),
+// streamed by workgroups 1.. of the grid while workgroup 0 runs the tail.
+// The V-cycle never reads x, and the solve's freeze flag stops both alike.
+struct TailSide {
+    void* x = nullptr;
+    const void* p = nullptr;
+    long long n = 0;
+    const CgState* st = nullptr;
+    int x_f64 = 1;  // x is float64 (else the tail's type)
+};
+constexpr unsigned kTailSideBlocks = 255;
+
+// x += alpha p over workgroup b of nb (the arithmetic of k_cg_update_pq).
+template <typename TX, typename T>
+__device__ __forceinline__ void tail_side_update(TX* __restrict__ x, const T* __restrict__ p,
+                                                 long long n, double alpha, unsigned b,
+                                                 unsigned nb) {
+    const T alphaT = (T)alpha;
+    constexpr int U = 4;
+    const long long stride = (long long)nb * kTailBlock;
+    for (long long i0 = (long long)b * kTailBlock + threadIdx.x; i0 < n; i0 += U * stride) {
+        TX xv[U];
+        T pv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long i = i0 + u * stride;
+            const bool in = i < n;
+            xv[u] = in ? x[i] : TX(0);
+            pv[u] = in ? p[i] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long i = i0 + u * stride;
+            if (i >= n) continue;
+            if constexpr (std::is_same<TX, T>::value)
+                x[i] = xv[u] + alphaT * pv[u];
+            else
+                x[i] = xv[u] + alpha * (TX)pv[u];
+        }
+    }
+}
+
+template <typename T, bool kSide>
+__global__ __launch_bounds__(kTailBlock) void k_vc_tail(TailParams<T> P, const CgState* guard,
+                                                        TailSide side) {
     if (solve_done(guard)) return;
+    if constexpr (kSide) {
+        if (blockIdx.x > 0) {
+            const double alpha = side.st->alpha;
+            const T* p = static_cast<const T*>(side.p);
+            if (side.x_f64)
+                tail_side_update<double, T>(static_cast<double*>(side.x), p, side.n, alpha,
+                                            blockIdx.x - 1, gridDim.x - 1);
+            else
+                tail_side_update<T, T>(static_cast<T*>(side.x), p, side.n, alpha,
+                                       blockIdx.x - 1, gridDim.x - 1);
+            return;
+        }
+    }
     vc_tail_body<T>(P);
 }
 
@@ -631,6 +691,13 @@ inline size_t vc_tail_start(const hm_vc* vc) {
     bool tc, is;
     while (t > 1 && tail_fits(vc, t - 1, &tc, &is)) --t;
     return t;
+}
+
+// Whether a V-cycle of this hierarchy runs an LDS tail (which can carry
+// TailSide work).
+inline bool vc_has_tail(const hm_vc* vc) {
+    const size_t L = vc->levels.size();
+    return L > 1 && vc->coarse_sweeps == 0 && vc_tail_start(vc) < L - 1;
 }
 
 // Tail parameters of levels [start, L) of a hierarchy; returns the LDS bytes.
@@ -702,7 +769,8 @@ int build_tail_pack(hm_vc* vc, size_t start, hipStream_t s) {
 }
 
 template <typename T>
-int launch_tail(hm_vc* vc, size_t start, const CgState* guard, hipStream_t s) {
+int launch_tail(hm_vc* vc, size_t start, const CgState* guard, hipStream_t s,
+                const TailSide* side = nullptr) {
     TailParams<T> P;
     const size_t lds = make_tail_params<T>(vc, start, P);
     if (start != vc->tail_start || (P.pack_count > 0 && !P.pack)) {
@@ -711,12 +779,22 @@ int launch_tail(hm_vc* vc, size_t start, const CgState* guard, hipStream_t s) {
     }
     static bool attr_set[2] = {false, false};
     if (!attr_set[sizeof(T) == 4]) {
-        HM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vc_tail<T>),
+        HM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vc_tail<T, false>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes));
+        HM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vc_tail<T, true>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes));
         attr_set[sizeof(T) == 4] = true;
     }
     ProfScope prof(kProfDense, vc->levels[start].n, sizeof(T) == 4, s);
-    k_vc_tail<T><<<1, kTailBlock, lds, s>>>(P, guard);
+    if (side && side->x) {
+        if (!side->p || !side->st || side->n < 0) {
+            set_error("internal: incomplete V-cycle tail side work");
+            return HM_ERR_ARG;
+        }
+        k_vc_tail<T, true><<<1 + kTailSideBlocks, kTailBlock, lds, s>>>(P, guard, *side);
+    } else {
+        k_vc_tail<T, false><<<1, kTailBlock, lds, s>>>(P, guard, TailSide{});
+    }
     HM_LAUNCH_CHECK();
     return HM_OK;
 }
