@@ -20,7 +20,10 @@
 //     strip s writes a column of its first row only after it has read that
 //     column of strip s-1's last row, which strip s-1 writes after reading.
 // Waits are bounded: a strip that waits too long raises a failure flag and
-// finishes, so every wave exits.
+// finishes, so every wave exits. A level whose colours and coefficients fit
+// in LDS (every level up to 32x32, where the reference makes its many small
+// sweeps) runs instead in one workgroup with all its sweeps pipelined
+// (k_fg_small).
 //
 // Arithmetic follows the types Numba gives the reference: with float32
 // arrays, a0, a00, da and the b vector are float32 while a1 = 1.0 - a0 and
@@ -75,6 +78,32 @@ __device__ __forceinline__ void pixel_coef(S a0, const S (&an)[4], const S* I, S
     pc.b00 = inv_det * a11;
     pc.b01 = inv_det * -a01;
     pc.b11 = inv_det * (double)a00;
+}
+
+// The pixel's new F (res[0..2]) and B (res[3..5]) from its coefficients and
+// the current values of its left, right, upper and lower neighbours
+// (estimate_foreground_ml.py:158-175); shared by the sweep kernels so that
+// they round alike.
+template <typename S>
+__device__ __forceinline__ void fg_solve_pixel(const S (&da)[4], const S* b0c, const S* b1c,
+                                               double b00, double b01, double b11,
+                                               const S (&left)[6], const S (&right)[6],
+                                               const S (&up)[6], const S (&down)[6],
+                                               S (&res)[6]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        S b0 = b0c[c], b1 = b1c[c];
+        b0 += da[0] * left[c];
+        b1 += da[0] * left[3 + c];
+        b0 += da[1] * right[c];
+        b1 += da[1] * right[3 + c];
+        b0 += da[2] * up[c];
+        b1 += da[2] * up[3 + c];
+        b0 += da[3] * down[c];
+        b1 += da[3] * down[3 + c];
+        res[c] = clamp01_store<S>(b00 * (double)b0 + b01 * (double)b1);
+        res[3 + c] = clamp01_store<S>(b01 * (double)b0 + b11 * (double)b1);
+    }
 }
 
 template <typename S>
@@ -155,7 +184,7 @@ __global__ void k_fg_coef(const S* __restrict__ img, const S* __restrict__ alpha
 // coefficients and the old F/B one column to the right; wave 1 also stages
 // the new row above the strip (for lane 0, from strip s-1's tagged halo) and
 // the old row below it (for the strip's last lane).
-constexpr int kFgThreads = 512;
+constexpr int kFgThreads = 1024;
 constexpr int kFgLoaders = kFgThreads - kWave;
 constexpr int kFgS = 16;  // S fields: da 4, b0 3, b1 3, old F/B to the right 6
 constexpr int kFgD = 3;   // double fields: b00, b01, b11
@@ -378,21 +407,14 @@ __global__ __launch_bounds__(kFgThreads) void k_fg_sweep(const S* __restrict__ c
                 const double b00 = st.dv[bf][j][0][lane];
                 const double b01 = st.dv[bf][j][1][lane];
                 const double b11 = st.dv[bf][j][2][lane];
-                S res[6];
+                S b0c[3], b1c[3];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    S b0 = sv[(4 + c) * kFgPad], b1 = sv[(7 + c) * kFgPad];
-                    b0 += da[0] * left[c];
-                    b1 += da[0] * left[3 + c];
-                    b0 += da[1] * right[c];
-                    b1 += da[1] * right[3 + c];
-                    b0 += da[2] * up[c];
-                    b1 += da[2] * up[3 + c];
-                    b0 += da[3] * down[c];
-                    b1 += da[3] * down[3 + c];
-                    res[c] = clamp01_store<S>(b00 * (double)b0 + b01 * (double)b1);
-                    res[3 + c] = clamp01_store<S>(b01 * (double)b0 + b11 * (double)b1);
+                    b0c[c] = sv[(4 + c) * kFgPad];
+                    b1c[c] = sv[(7 + c) * kFgPad];
                 }
+                S res[6];
+                fg_solve_pixel<S>(da, b0c, b1c, b00, b01, b11, left, right, up, down, res);
 #pragma unroll
                 for (int q = 0; q < 6; ++q) st.out[bf][j][q][lane] = res[q];
                 if (active) {
@@ -421,6 +443,103 @@ __global__ __launch_bounds__(kFgThreads) void k_fg_sweep(const S* __restrict__ c
     }
     if (!computer) fg_writeback<S>(st, (nchunks - 1) & 1, (nchunks - 1) * C, s, lt, F, B, h, w);
     if (!ok && lt == 0) atomicExch(fail, 1);
+}
+
+// A whole level in one workgroup, when its colours and per-pixel
+// coefficients fit in LDS: the level's coefficients (as k_fg_coef) and all of
+// its n_iter sweeps in one launch, the sweeps pipelined. Sweep k updates
+// pixel (y, x) at step x + y + 2k. Its left and upper neighbours then hold
+// sweep k's values and its right and lower neighbours sweep k-1's, each
+// written at the step before, which is what the reference's in-place raster
+// sweeps read. A neighbour lies on the next or previous anti-diagonal, whose
+// pixels are written only at steps of the other parity, so no value is read
+// and written in the same step: one barrier per step orders everything.
+constexpr int kFgSmallThreads = 1024;
+constexpr size_t kFgSmallLds = 156 * 1024;
+
+template <typename S>
+constexpr size_t fg_small_bytes_per_px() {
+    return sizeof(double) * kFgD + sizeof(S) * (10 + 6);  // b00..b11, da/b0/b1, F/B
+}
+
+template <typename S>
+__global__ __launch_bounds__(kFgSmallThreads) void k_fg_small(const S* __restrict__ img,
+                                                              const S* __restrict__ alpha,
+                                                              int h, int w, S reg, S gw, S* F,
+                                                              S* B, int n_iter) {
+    extern __shared__ __align__(16) unsigned char fg_lds[];
+    const int n = h * w;
+    double* cd = reinterpret_cast<double*>(fg_lds);  // [3][n]: b00, b01, b11
+    S* cs = reinterpret_cast<S*>(cd + kFgD * n);     // [10][n]: da l/r/u/d, b0 rgb, b1 rgb
+    S* fb = cs + 10 * n;                             // [6][n]: F rgb, B rgb
+    for (int i = threadIdx.x; i < n; i += kFgSmallThreads) {
+        const int y = i / w, x = i - y * w;
+        const int row = y * w;
+        S an[4];
+        an[0] = alpha[row + (x > 0 ? x - 1 : 0)];
+        an[1] = alpha[row + (x + 1 < w ? x + 1 : x)];
+        an[2] = alpha[(y > 0 ? y - 1 : 0) * w + x];
+        an[3] = alpha[(y + 1 < h ? y + 1 : y) * w + x];
+        const S I[3] = {img[i * 3], img[i * 3 + 1], img[i * 3 + 2]};
+        PixelCoef<S> pc;
+        pixel_coef<S>(alpha[i], an, I, reg, gw, pc);
+#pragma unroll
+        for (int d = 0; d < 4; ++d) cs[d * n + i] = pc.da[d];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            cs[(4 + c) * n + i] = pc.b0[c];
+            cs[(7 + c) * n + i] = pc.b1[c];
+            fb[c * n + i] = F[i * 3 + c];
+            fb[(3 + c) * n + i] = B[i * 3 + c];
+        }
+        cd[i] = pc.b00;
+        cd[n + i] = pc.b01;
+        cd[2 * n + i] = pc.b11;
+    }
+    __syncthreads();
+    const int items = n_iter * h;  // (sweep, row) pairs
+    const int steps = (w - 1) + (h - 1) + 2 * (n_iter - 1) + 1;
+    for (int t = 0; t < steps; ++t) {
+        for (int j = threadIdx.x; j < items; j += kFgSmallThreads) {
+            const int k = j / h, y = j - k * h;
+            const int x = t - 2 * k - y;
+            if (x < 0 || x >= w) continue;
+            const int i = y * w + x;
+            // Neighbours clamped to the pixel itself at the borders
+            // (estimate_foreground_ml.py:144-157): its value before this sweep.
+            const int il = x > 0 ? i - 1 : i, ir = x + 1 < w ? i + 1 : i;
+            const int iu = y > 0 ? i - w : i, idn = y + 1 < h ? i + w : i;
+            S left[6], right[6], up[6], down[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                left[q] = fb[q * n + il];
+                right[q] = fb[q * n + ir];
+                up[q] = fb[q * n + iu];
+                down[q] = fb[q * n + idn];
+            }
+            S da[4], b0c[3], b1c[3];
+#pragma unroll
+            for (int d = 0; d < 4; ++d) da[d] = cs[d * n + i];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                b0c[c] = cs[(4 + c) * n + i];
+                b1c[c] = cs[(7 + c) * n + i];
+            }
+            S res[6];
+            fg_solve_pixel<S>(da, b0c, b1c, cd[i], cd[n + i], cd[2 * n + i], left, right, up,
+                              down, res);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) fb[q * n + i] = res[q];
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < n; i += kFgSmallThreads) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            F[i * 3 + c] = fb[c * n + i];
+            B[i * 3 + c] = fb[(3 + c) * n + i];
+        }
+    }
 }
 
 // Nearest-neighbour resizes of one level (estimate_foreground_ml.py:5-55,
@@ -580,14 +699,32 @@ int fb_ml_t(const S* img0, const S* alpha0, int h0, int w0, int n_levels, const 
             img0, alpha0, h0, w0, Fp, Bp, hp, wp, static_cast<S*>(imgb), static_cast<S*>(alphab),
             Fc, Bc, h, w, full ? 0 : 1);
         const int n_iter = (w <= small_size && h <= small_size) ? n_small : n_big;
-        if (n_iter > 0) {
+        const size_t small_lds = (size_t)h * w * fg_small_bytes_per_px<S>();
+        if (n_iter > 0 && small_lds <= kFgSmallLds) {
+            // The whole level in one workgroup (k_fg_small).
+            static bool attr_set = false;
+            if (!attr_set) {
+                e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fg_small<S>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)kFgSmallLds);
+                if (e != hipSuccess) {
+                    cleanup();
+                    set_error("foreground: %s", hipGetErrorString(e));
+                    return HM_ERR_HIP;
+                }
+                attr_set = true;
+            }
+            ProfScope prof(kProfFgSweep, (long long)h * w * n_iter, sizeof(S) == 4, s);
+            k_fg_small<S><<<1, kFgSmallThreads, small_lds, s>>>(img, alpha, h, w, (S)reg, (S)gw,
+                                                               Fc, Bc, n_iter);
+        } else if (n_iter > 0) {
             k_fg_coef<S><<<div_up((long long)h * w, 256), 256, 0, s>>>(
                 img, alpha, h, w, (S)reg, (S)gw, static_cast<S*>(cs), static_cast<double*>(cd));
             // Halo words of this level start untagged; sweeps cycle tags 1..3.
             const long long nh = 6LL * div_up(h, kWave) * w;
             k_fill_words<W><<<div_up(nh, 256), 256, 0, s>>>(halo, nh);
         }
-        for (int it = 0; it < n_iter; ++it) {
+        for (int it = 0; it < n_iter && small_lds > kFgSmallLds; ++it) {
             ProfScope prof(kProfFgSweep, (long long)h * w, sizeof(S) == 4, s);
             k_fg_sweep<S><<<div_up(h, kWave), kFgThreads, 0, s>>>(
                 static_cast<const S*>(cs), static_cast<const double*>(cd), Fc, Bc, h, w, halo,
