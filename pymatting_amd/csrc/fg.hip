@@ -456,6 +456,7 @@ __global__ __launch_bounds__(kFgThreads) void k_fg_sweep(const S* __restrict__ c
 // and written in the same step: one barrier per step orders everything.
 constexpr int kFgSmallThreads = 1024;
 constexpr size_t kFgSmallLds = 156 * 1024;
+constexpr int kFgSmallMaxIter = 4096;  // sweeps per launch (more go to the strip kernel)
 
 template <typename S>
 constexpr size_t fg_small_bytes_per_px() {
@@ -700,7 +701,8 @@ int fb_ml_t(const S* img0, const S* alpha0, int h0, int w0, int n_levels, const 
             Fc, Bc, h, w, full ? 0 : 1);
         const int n_iter = (w <= small_size && h <= small_size) ? n_small : n_big;
         const size_t small_lds = (size_t)h * w * fg_small_bytes_per_px<S>();
-        if (n_iter > 0 && small_lds <= kFgSmallLds) {
+        const bool one_wg = small_lds <= kFgSmallLds && n_iter <= kFgSmallMaxIter;
+        if (n_iter > 0 && one_wg) {
             // The whole level in one workgroup (k_fg_small).
             static bool attr_set = false;
             if (!attr_set) {
@@ -724,7 +726,7 @@ int fb_ml_t(const S* img0, const S* alpha0, int h0, int w0, int n_levels, const 
             const long long nh = 6LL * div_up(h, kWave) * w;
             k_fill_words<W><<<div_up(nh, 256), 256, 0, s>>>(halo, nh);
         }
-        for (int it = 0; it < n_iter && small_lds > kFgSmallLds; ++it) {
+        for (int it = 0; it < n_iter && !one_wg; ++it) {
             ProfScope prof(kProfFgSweep, (long long)h * w, sizeof(S) == 4, s);
             k_fg_sweep<S><<<div_up(h, kWave), kFgThreads, 0, s>>>(
                 static_cast<const S*>(cs), static_cast<const double*>(cd), Fc, Bc, h, w, halo,

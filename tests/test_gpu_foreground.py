@@ -90,6 +90,21 @@ def test_lemur_known_answer_and_options():
     assert np.array_equal(F2, F2r)
 
 
+def test_many_sweeps_both_level_kernels():
+    """Levels that fit in LDS run all their sweeps in one workgroup, up to
+    4096 sweeps per launch; more go to the strip kernel, one launch per
+    sweep. Both are bit-exact against the oracle."""
+    import pymatting_amd as pm
+    rng = np.random.default_rng(12)
+    image = rng.random((12, 9, 3))
+    alpha = np.clip(rng.random((12, 9)) * 1.4 - 0.2, 0, 1)
+    for n_small in (37, 4100):
+        print("n_small_iterations", n_small, flush=True)
+        F = pm.estimate_foreground_ml(image, alpha, n_small_iterations=n_small)
+        Fr = ofg(image, alpha, n_small_iterations=n_small, mean_mode="f64")
+        assert np.array_equal(F, Fr), (n_small, np.abs(F - Fr).max())
+
+
 def test_single_pixel_raises():
     import pymatting_amd as pm
     with pytest.raises(ZeroDivisionError):
