@@ -608,6 +608,7 @@ def main():
     achieved = bytes_launch / t_spmv32 / 1e9
     achieved64 = spmv_bytes_per_point(8) * n / t_spmv64 / 1e9
     breakdown = {}
+    step_bytes = 0.0
     for (k, p), v in sorted(prof.items(), key=lambda kv: -kv[1][0]):
         ms, cnt, pts, s = v
         entry = {"ms_total": round(ms, 3), "launches": cnt,
@@ -615,6 +616,7 @@ def main():
         bpp = bytes_per_point(k, s)
         if bpp is not None:
             entry["GBps_with_events"] = round(bpp * pts / (ms * 1e-3) / 1e9, 1)
+            step_bytes += bpp * pts
         breakdown["%s_%s" % (k, "f32" if p else "f64")] = entry
 
     cpu = None if args.no_cpu else cpu_baseline()
@@ -670,6 +672,11 @@ def main():
                              "frac": round(achieved64 / HBM_PEAK_GBS, 4),
                              "avg_launch_us": round(t_spmv64 * 1e6, 2)}},
         "kernel_breakdown_timed_mixed": breakdown,
+        # Algorithmic bytes of one value-form step (the classes above; the
+        # LDS tail's launches, with the x update they carry, are not
+        # counted) over the step's time.
+        "step_bytes_model": {"GB": round(step_bytes / 1e9, 3),
+                             "GBps_over_step": round(step_bytes / t_mixed / 1e9, 1)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
