@@ -1,4 +1,5 @@
-"""Properties at the full workload sizes: 1920x1080 and 3840x2160.
+"""Properties at the full workload sizes: 1920x1080, 3840x2160 and the
+64-image batch.
 
 The oracle takes minutes at these sizes, so these tests check what holds at
 any size. L 1 = 0 for the closed-form Laplacian (the centred colours of a
@@ -102,3 +103,32 @@ def test_4k_five_level_mixed_accuracy():
     ref = pm.LambdaSolver(image, trimap, **base)
     alpha_f = ref.solve(rtol=1e-8, inner_rtol=1e-3)
     assert float((alpha_f - alpha_c).abs().max()) <= 1e-4
+
+
+def test_c3_batch_of_64_images():
+    """The 64-image batch (h, w uniform in 256..1024, bands 5-40 px): batched
+    reliable updates at rtol 1e-9 land within 1e-4 of every image's converged
+    float64 alpha, with each image's float64 residual at the tolerance and
+    the iteration count of its one-image solve (+-1)."""
+    import torch
+    import pymatting_amd as pm
+    from pymatting_amd import synthetic
+    data = synthetic.config3()
+    images = [torch.from_numpy(d[0]).cuda() for d in data]
+    trimaps = [torch.from_numpy(d[1]).cuda() for d in data]
+    print("C3 batch", len(images), flush=True)
+    bs = pm.BatchSolver(images, trimaps, precision="mixed")
+    xs = bs.solve_x(rtol=1e-9, mixed="reliable")
+    worst = 0.0
+    for i, (im, tr) in enumerate(zip(images, trimaps)):
+        conv = pm.LambdaSolver(im, tr, precision="f64", level0="stencil")
+        alpha_c = conv.solve(rtol=1e-11, maxiter=100000)
+        alpha_b = xs[i].clamp(0.0, 1.0).reshape(alpha_c.shape)
+        worst = max(worst, float((alpha_b - alpha_c).abs().max()))
+        r = conv.b - conv.op.apply(xs[i])
+        assert float(torch.linalg.norm(r)) <= 1.01e-9 * float(torch.linalg.norm(conv.b)), i
+        if i % 8 == 0:  # one-image solves of every eighth image
+            one = pm.LambdaSolver(im, tr, precision="mixed", level0="stencil")
+            one.solve(rtol=1e-9, mixed="reliable")
+            assert abs(one.stats["iterations"] - bs.stats[i]["iterations"]) <= 1, i
+    assert worst <= 1e-4, worst
