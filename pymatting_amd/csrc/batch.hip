@@ -45,10 +45,13 @@ __global__ __launch_bounds__(kBlock) void tile_kernel_batch(
     const unsigned t = xcd_tile(blockIdx.x, ntiles);
     const TileRef tr = tiles[t];
     const BatchEntry<Op, In, Out> E = ents[tr.img];
-    const bool done = solve_done(E.guard);
+    // A finished image's tiles return before loading anything (tile_body
+    // would issue its loads first): in a batch, images finish tens of
+    // iterations apart and the launches keep covering them.
+    if (solve_done(E.guard)) return;
     const int by = (int)(tr.local / E.tiles_x);
     const int bx = (int)(tr.local - (unsigned)by * E.tiles_x);
-    tile_body<T, Op, In, Out>(E.op, E.in, E.out, E.h, E.w, done, bx, by, t);
+    tile_body<T, Op, In, Out>(E.op, E.in, E.out, E.h, E.w, false, bx, by, t);
 }
 
 // ---- vector kernels over the images' vector blocks (kBlock*kVecPerThread) --
