@@ -84,6 +84,25 @@ __device__ __forceinline__ void pixel_coef(S a0, const S (&an)[4], const S* I, S
 // the current values of its left, right, upper and lower neighbours
 // (estimate_foreground_ml.py:158-175); shared by the sweep kernels so that
 // they round alike.
+// One colour channel: the channels' 2x2 systems share only the
+// coefficients (v[0] is F, v[1] is B of the channel).
+template <typename S>
+__device__ __forceinline__ void fg_solve_channel(const S (&da)[4], S b0, S b1, double b00,
+                                                 double b01, double b11, const S (&left)[2],
+                                                 const S (&right)[2], const S (&up)[2],
+                                                 const S (&down)[2], S (&res)[2]) {
+    b0 += da[0] * left[0];
+    b1 += da[0] * left[1];
+    b0 += da[1] * right[0];
+    b1 += da[1] * right[1];
+    b0 += da[2] * up[0];
+    b1 += da[2] * up[1];
+    b0 += da[3] * down[0];
+    b1 += da[3] * down[1];
+    res[0] = clamp01_store<S>(b00 * (double)b0 + b01 * (double)b1);
+    res[1] = clamp01_store<S>(b01 * (double)b0 + b11 * (double)b1);
+}
+
 template <typename S>
 __device__ __forceinline__ void fg_solve_pixel(const S (&da)[4], const S* b0c, const S* b1c,
                                                double b00, double b01, double b11,
@@ -92,17 +111,12 @@ __device__ __forceinline__ void fg_solve_pixel(const S (&da)[4], const S* b0c, c
                                                S (&res)[6]) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        S b0 = b0c[c], b1 = b1c[c];
-        b0 += da[0] * left[c];
-        b1 += da[0] * left[3 + c];
-        b0 += da[1] * right[c];
-        b1 += da[1] * right[3 + c];
-        b0 += da[2] * up[c];
-        b1 += da[2] * up[3 + c];
-        b0 += da[3] * down[c];
-        b1 += da[3] * down[3 + c];
-        res[c] = clamp01_store<S>(b00 * (double)b0 + b01 * (double)b1);
-        res[3 + c] = clamp01_store<S>(b01 * (double)b0 + b11 * (double)b1);
+        const S l[2] = {left[c], left[3 + c]}, r[2] = {right[c], right[3 + c]};
+        const S u[2] = {up[c], up[3 + c]}, d[2] = {down[c], down[3 + c]};
+        S out[2];
+        fg_solve_channel<S>(da, b0c[c], b1c[c], b00, b01, b11, l, r, u, d, out);
+        res[c] = out[0];
+        res[3 + c] = out[1];
     }
 }
 
@@ -175,17 +189,21 @@ __global__ void k_fg_coef(const S* __restrict__ img, const S* __restrict__ alpha
 // coefficient planes (k_fg_coef); F/B (h,w,3). halo[s][x][6] carries strip
 // s's last row, tagged with this sweep's tag (1..3), to strip s+1.
 //
-// A workgroup of kFgThreads threads owns one strip. Wave 0 computes; the
-// other waves stage the inputs of the next chunk of C steps in LDS while wave
-// 0 computes the current one (double buffering), and write the previous
-// chunk's results from LDS to F/B with row-contiguous stores, so wave 0 only
-// touches LDS (its lanes own different rows: its own stores would each touch
-// 64 cache lines). Per lane and step the stage holds the pixel's
-// coefficients and the old F/B one column to the right; wave 1 also stages
-// the new row above the strip (for lane 0, from strip s-1's tagged halo) and
-// the old row below it (for the strip's last lane).
+// A workgroup of kFgThreads threads owns one strip. Waves 0-2 sweep it, one
+// colour channel each (a channel's F and B; the channels' 2x2 systems share
+// only their coefficients, so a step's work splits three ways with the same
+// arithmetic). The other waves stage the inputs of the next chunk of C steps
+// in LDS while the sweeping waves compute the current one (double
+// buffering), and write the previous chunk's results from LDS to F/B with
+// row-contiguous stores, so the sweeping waves only touch LDS (their lanes
+// own different rows: their own stores would each touch 64 cache lines). Per
+// lane and step the stage holds the pixel's coefficients and the old F/B one
+// column to the right; the first loader wave also stages the new row above
+// the strip (for lane 0, from strip s-1's tagged halo) and the old row below
+// it (for the strip's last lane).
 constexpr int kFgThreads = 1024;
-constexpr int kFgLoaders = kFgThreads - kWave;
+constexpr int kFgSweepers = 3 * kWave;  // one wave per colour channel
+constexpr int kFgLoaders = kFgThreads - kFgSweepers;
 constexpr int kFgS = 16;  // S fields: da 4, b0 3, b1 3, old F/B to the right 6
 constexpr int kFgD = 3;   // double fields: b00, b01, b11
 constexpr int kFgPad = kWave + 1;  // lane stride (conflict-free staging stores)
@@ -327,9 +345,10 @@ __global__ __launch_bounds__(kFgThreads) void k_fg_sweep(const S* __restrict__ c
     constexpr int C = FgChunk<S>::C;
     __shared__ FgStage<S> st;
     const int s = blockIdx.x;
-    const bool computer = threadIdx.x < kWave;
-    const int lane = threadIdx.x;             // compute wave
-    const int lt = threadIdx.x - kWave;       // loaders
+    const bool computer = threadIdx.x < kFgSweepers;
+    const int ch = threadIdx.x / kWave;         // sweeping waves: colour channel
+    const int lane = threadIdx.x % kWave;       // sweeping waves: row in the strip
+    const int lt = threadIdx.x - kFgSweepers;   // loaders
     const int last_lane = min(kWave, h - s * kWave) - 1;  // lane owning the strip's last row
     const int steps = w + kWave - 1;
     const int nchunks = (steps + C - 1) / C;
@@ -340,16 +359,15 @@ __global__ __launch_bounds__(kFgThreads) void k_fg_sweep(const S* __restrict__ c
     const bool row_ok = r < h;
     const int rc = row_ok ? r : h - 1;
     const long long row = (long long)rc * w;
-    S cur_old[6], prev_res[6];
+    // The channel's F and B (index 0 and 1) of the current pixel before
+    // this sweep and of the previous pixel after it.
+    S cur_old[2], prev_res[2];
     if (computer) {
         // Old values at column 0 (the first step's "current" pixel).
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            cur_old[c] = F[row * 3 + c];
-            cur_old[3 + c] = B[row * 3 + c];
-            prev_res[c] = cur_old[c];
-            prev_res[3 + c] = cur_old[3 + c];
-        }
+        cur_old[0] = F[row * 3 + ch];
+        cur_old[1] = B[row * 3 + ch];
+        prev_res[0] = cur_old[0];
+        prev_res[1] = cur_old[1];
     } else {
         ok = fg_stage<S>(st, 0, 0, s, lt, cs, cd, F, B, h, w, last_lane, halo, tag, waited);
     }
@@ -365,40 +383,40 @@ __global__ __launch_bounds__(kFgThreads) void k_fg_sweep(const S* __restrict__ c
                 const bool active = row_ok && x >= 0 && x < w;
                 const int xc = x < 0 ? 0 : (x >= w ? w - 1 : x);
                 const S* sv = &st.sv[bf][j][0][lane];
-                S next_old[6];
+                // This channel's fields: F at q = ch, B at q = 3 + ch.
+                const int qf = ch, qb = 3 + ch;
+                S next_old[2] = {sv[(10 + qf) * kFgPad], sv[(10 + qb) * kFgPad]};
+                S left[2], right[2], up[2], down[2];
+                S from_above[2], from_below[2];
 #pragma unroll
-                for (int c = 0; c < 6; ++c) next_old[c] = sv[(10 + c) * kFgPad];
-                S left[6], right[6], up[6], down[6];
-                S from_above[6], from_below[6];
-#pragma unroll
-                for (int q = 0; q < 6; ++q) {
-                    from_above[q] = shfl_up1(prev_res[q]);    // lane-1's result at this column
-                    from_below[q] = shfl_down1(next_old[q]);  // lane+1's old value at this column
+                for (int v = 0; v < 2; ++v) {
+                    from_above[v] = shfl_up1(prev_res[v]);    // lane-1's result at this column
+                    from_below[v] = shfl_down1(next_old[v]);  // lane+1's old value at this column
                 }
 #pragma unroll
-                for (int q = 0; q < 6; ++q) {
-                    left[q] = x > 0 ? prev_res[q] : cur_old[q];
-                    right[q] = x + 1 < w ? next_old[q] : cur_old[q];
+                for (int v = 0; v < 2; ++v) {
+                    left[v] = x > 0 ? prev_res[v] : cur_old[v];
+                    right[v] = x + 1 < w ? next_old[v] : cur_old[v];
                 }
                 if (r == 0) {
-#pragma unroll
-                    for (int q = 0; q < 6; ++q) up[q] = cur_old[q];
+                    up[0] = cur_old[0];
+                    up[1] = cur_old[1];
                 } else if (lane > 0) {
-#pragma unroll
-                    for (int q = 0; q < 6; ++q) up[q] = from_above[q];
+                    up[0] = from_above[0];
+                    up[1] = from_above[1];
                 } else {
-#pragma unroll
-                    for (int q = 0; q < 6; ++q) up[q] = st.up[bf][j][q];
+                    up[0] = st.up[bf][j][qf];
+                    up[1] = st.up[bf][j][qb];
                 }
                 if (r >= h - 1) {
-#pragma unroll
-                    for (int q = 0; q < 6; ++q) down[q] = cur_old[q];
+                    down[0] = cur_old[0];
+                    down[1] = cur_old[1];
                 } else if (lane < last_lane) {
-#pragma unroll
-                    for (int q = 0; q < 6; ++q) down[q] = from_below[q];
+                    down[0] = from_below[0];
+                    down[1] = from_below[1];
                 } else {
-#pragma unroll
-                    for (int q = 0; q < 6; ++q) down[q] = st.dn[bf][j][q];
+                    down[0] = st.dn[bf][j][qf];
+                    down[1] = st.dn[bf][j][qb];
                 }
                 // The 2x2 solve (estimate_foreground_ml.py:158-175).
                 S da[4];
@@ -407,30 +425,25 @@ __global__ __launch_bounds__(kFgThreads) void k_fg_sweep(const S* __restrict__ c
                 const double b00 = st.dv[bf][j][0][lane];
                 const double b01 = st.dv[bf][j][1][lane];
                 const double b11 = st.dv[bf][j][2][lane];
-                S b0c[3], b1c[3];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    b0c[c] = sv[(4 + c) * kFgPad];
-                    b1c[c] = sv[(7 + c) * kFgPad];
-                }
-                S res[6];
-                fg_solve_pixel<S>(da, b0c, b1c, b00, b01, b11, left, right, up, down, res);
-#pragma unroll
-                for (int q = 0; q < 6; ++q) st.out[bf][j][q][lane] = res[q];
+                S res[2];
+                fg_solve_channel<S>(da, sv[(4 + ch) * kFgPad], sv[(7 + ch) * kFgPad], b00, b01,
+                                    b11, left, right, up, down, res);
+                st.out[bf][j][qf][lane] = res[0];
+                st.out[bf][j][qb][lane] = res[1];
                 if (active) {
                     if (lane == last_lane && r + 1 < h) {
                         // The strip's last row, tagged, for strip s+1.
                         auto* dst = halo + ((long long)s * w + xc) * 6;
-#pragma unroll
-                        for (int q = 0; q < 6; ++q)
-                            __hip_atomic_store(dst + q, fg_tag<S>(res[q], tag), __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_store(dst + qf, fg_tag<S>(res[0], tag), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_store(dst + qb, fg_tag<S>(res[1], tag), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
                     }
                 }
 #pragma unroll
-                for (int q = 0; q < 6; ++q) {
-                    prev_res[q] = res[q];
-                    cur_old[q] = next_old[q];
+                for (int v = 0; v < 2; ++v) {
+                    prev_res[v] = res[v];
+                    cur_old[v] = next_old[v];
                 }
             }
         } else {
