@@ -5,12 +5,14 @@
 // new value of (y, x) uses the NEW values of (y, x-1) and (y-1, x) and the
 // OLD values of (y, x+1) and (y+1, x). This file reproduces that order
 // exactly with a strip-pipelined wavefront:
-//   - one 64-lane wavefront owns a strip of 64 rows; lane y owns row y and at
-//     step t updates column x = t - y, so lane y runs one column behind lane
-//     y-1. The new value of the pixel above is lane y-1's result of the
-//     previous step, and the old value of the pixel below is the "next old"
-//     register lane y+1 loaded for its right neighbour; both move by
-//     cross-lane shuffles, never through memory.
+//   - a strip of 64 rows is swept by three 64-lane waves, one per colour
+//     channel (each channel's F and B form their own 2x2 systems; only the
+//     coefficients are shared). Lane y owns row y and at step t updates
+//     column x = t - y, so lane y runs one column behind lane y-1. The new
+//     value of the pixel above is lane y-1's result of the previous step,
+//     and the old value of the pixel below is the "next old" register lane
+//     y+1 loaded for its right neighbour; both move by cross-lane shuffles,
+//     never through memory.
 //   - strip s gets the new last row of strip s-1 through a halo buffer: each
 //     value is stored with a 2-bit sweep tag in its two top bits (F and B are
 //     clamped to [0, 1], whose float and double encodings leave those bits
