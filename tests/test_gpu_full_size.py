@@ -80,6 +80,23 @@ def test_1080p_mixed_reliable_accuracy():
     assert float(torch.linalg.norm(r)) <= 1.01e-9 * float(torch.linalg.norm(conv.b))
 
 
+def test_1080p_configured_comparison():
+    """The metric's comparison: both forms to rtol 1e-7, the mixed form's
+    alpha within 1e-4 of the float64 PCG's (it is ~1e-6 off: reliable
+    updates follow the float64 iterates), in the same iteration count."""
+    import pymatting_amd as pm
+    from pymatting_amd import synthetic
+    image_h, trimap_h, _ = synthetic.config2()
+    image, trimap = _device(image_h, trimap_h)
+    print("1080p rtol 1e-7", flush=True)
+    f64 = pm.LambdaSolver(image, trimap, precision="f64", level0="stencil")
+    a64 = f64.solve(rtol=1e-7)
+    m = pm.LambdaSolver(image, trimap, precision="mixed", level0="stencil")
+    am = m.solve(rtol=1e-7, mixed="reliable")
+    assert float((am - a64).abs().max()) <= 1e-4
+    assert abs(m.stats["iterations"] - f64.stats["iterations"]) <= 1
+
+
 def test_4k_five_level_mixed_accuracy():
     """C4's preconditioner: 5 grids, V(2,2), the 240x135 coarsest grid by two
     damped Jacobi sweeps; reliable updates at rtol 4e-10 and restarted
