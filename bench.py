@@ -681,6 +681,7 @@ def main():
                    "unknown_fraction": round(float((trimap_h == 0.5).mean()), 4),
                    "rtol": MIXED["rtol"], "mixed_delta": MIXED["delta"]},
         "mixed": {"iterations": st_m["iterations"],
+                  "within_1e-4_of_fp64": bool(v_m[0] <= 1e-4),
                   "max_abs_dalpha_vs_fp64": v_m[0], "rms_dalpha_vs_fp64": v_m[1],
                   "max_abs_dalpha_vs_converged": e_m[0], "rms_dalpha_vs_converged": e_m[1]},
         "fp64": {"value": round(n / t_f64 / 1e6, 3), "unit": "Mpix/s",
