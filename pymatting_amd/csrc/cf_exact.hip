@@ -222,7 +222,7 @@ __global__ void k_assemble_exact(const double* __restrict__ I, const double* __r
 // L_ij = sum_k [delta_ij - (1 + c_k . (I_j - mu_k)) / 9] over the windows
 // containing both pixels in raster order of the centre, with the operations
 // of entry_exact, so every entry is the same double.
-constexpr int kAsmX = 64, kAsmY = 4;
+constexpr int kAsmX = 32, kAsmY = 8;
 constexpr int kAsmWX = kAsmX + 2, kAsmWY = kAsmY + 2;  // window centres
 constexpr int kAsmIX = kAsmX + 4, kAsmIY = kAsmY + 4;  // pixels
 
@@ -258,27 +258,39 @@ __global__ __launch_bounds__(kAsmX * kAsmY) void k_assemble_sym(
     const long long i = (long long)yi * w + xi;
     const int ib = (ty + 2) * kAsmIX + tx + 2;
     const double Ii0 = si[0][ib], Ii1 = si[1][ib], Ii2 = si[2][ib];
-    // Window (a, b) is centred at (yi + a - 1, xi + b - 1). Windows outside
-    // rows/columns 1..h-2 / 1..w-2 have flag 0 in wc.
-    double cw[9][3], mw[9][3];
-    bool vw[9];
+    // Window (a, b) is centred at (yi + a - 1, xi + b - 1); windows outside
+    // rows/columns 1..h-2 / 1..w-2 have flag 0 in wc. The windows are visited
+    // in raster order of the centre, each adding its term to every stored
+    // slot whose pixel j it also contains, so each slot's sum keeps the
+    // order of entry_exact; slot (dy, dx) lies in windows
+    // a = max(dy, 0) .. min(dy, 0) + 2 (b likewise).
+    double acc[13];  // stored slots 12..24
+#pragma unroll
+    for (int s = 12; s < 25; ++s) acc[s - 12] = 0.0;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
-            const int q = a * 3 + b;
             const int wi = (ty + a) * kAsmWX + tx + b;
-            vw[q] = sw[0][wi] != 0.0;
+            if (sw[0][wi] == 0.0) continue;
             const double mu0 = sw[1][wi], mu1 = sw[2][wi], mu2 = sw[3][wi];
             const double m00 = sw[4][wi], m01 = sw[5][wi], m02 = sw[6][wi];
             const double m11 = sw[7][wi], m12 = sw[8][wi], m22 = sw[9][wi];
-            const double s = Ii0 - mu0, tt = Ii1 - mu1, u = Ii2 - mu2;
-            cw[q][0] = m00 * s + m01 * tt + m02 * u;
-            cw[q][1] = m01 * s + m11 * tt + m12 * u;
-            cw[q][2] = m02 * s + m12 * tt + m22 * u;
-            mw[q][0] = mu0;
-            mw[q][1] = mu1;
-            mw[q][2] = mu2;
+            const double s0 = Ii0 - mu0, t0 = Ii1 - mu1, u0 = Ii2 - mu2;
+            const double c0 = m00 * s0 + m01 * t0 + m02 * u0;
+            const double c1 = m01 * s0 + m11 * t0 + m12 * u0;
+            const double c2 = m02 * s0 + m12 * t0 + m22 * u0;
+#pragma unroll
+            for (int s = 12; s < 25; ++s) {
+                const int dy = s / 5 - 2, dx = s % 5 - 2;
+                if (a < (dy > 0 ? dy : 0) || a > (dy < 0 ? dy : 0) + 2) continue;
+                if (b < (dx > 0 ? dx : 0) || b > (dx < 0 ? dx : 0) + 2) continue;
+                const int jb = (ty + 2 + dy) * kAsmIX + tx + 2 + dx;
+                const double temp = c0 * (si[0][jb] - mu0) + c1 * (si[1][jb] - mu1) +
+                                    c2 * (si[2][jb] - mu2);
+                const double delta = (s == 12) ? 1.0 : 0.0;
+                acc[s - 12] += delta - (1 + temp) / 9;
+            }
         }
     }
     const bool known_i = cls && cls[i] != 0;
@@ -295,23 +307,7 @@ __global__ __launch_bounds__(kAsmX * kAsmY) void k_assemble_sym(
             } else if (mode == HM_OP_MASKED && known_j) {
                 v = 0.0;
             } else {
-                const int jb = (ty + 2 + dy) * kAsmIX + tx + 2 + dx;
-                const double Ij0 = si[0][jb], Ij1 = si[1][jb], Ij2 = si[2][jb];
-                const double delta = (s == 12) ? 1.0 : 0.0;
-                double value = 0.0;
-#pragma unroll
-                for (int a = (dy > 0 ? dy : 0); a <= (dy < 0 ? dy : 0) + 2; ++a) {
-#pragma unroll
-                    for (int b = (dx > 0 ? dx : 0); b <= (dx < 0 ? dx : 0) + 2; ++b) {
-                        const int q = a * 3 + b;
-                        if (!vw[q]) continue;
-                        const double temp = cw[q][0] * (Ij0 - mw[q][0]) +
-                                            cw[q][1] * (Ij1 - mw[q][1]) +
-                                            cw[q][2] * (Ij2 - mw[q][2]);
-                        value += delta - (1 + temp) / 9;
-                    }
-                }
-                v = value;
+                v = acc[s - 12];
                 if (mode == HM_OP_LAMBDA && s == 12 && known_i) v = v + lambda;
             }
         }
