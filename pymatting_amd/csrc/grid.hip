@@ -152,9 +152,13 @@ __device__ __forceinline__ double galerkin_entry(const double* __restrict__ Af, 
 }
 
 // One thread per (coarse point, stored coarse slot s >= 12); both operators
-// use the symmetric storage of engine.h.
+// use the symmetric storage of engine.h. The same store writes the float32
+// planes, and slot 12 the diagonal and inverse diagonal (float64 and
+// float32), with the operations of k_dia_finish.
 __global__ void k_galerkin(const double* __restrict__ Af, int hf, int wf,
-                           double* __restrict__ Ac, int hc, int wc) {
+                           double* __restrict__ Ac, float* __restrict__ Ac32,
+                           double* __restrict__ d64, float* __restrict__ d32,
+                           double* __restrict__ id64, float* __restrict__ id32, int hc, int wc) {
     const long long nc = (long long)hc * wc, nf = (long long)hf * wf;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= kSymSlots * nc) return;
@@ -174,12 +178,21 @@ __global__ void k_galerkin(const double* __restrict__ Af, int hf, int wf,
         }
     }
     Ac[t] = sum;
+    Ac32[t] = (float)sum;
+    if (s == 12) {
+        d64[I] = sum;
+        d32[I] = (float)sum;
+        id64[I] = 1.0 / sum;
+        id32[I] = (float)(1.0 / sum);
+    }
 }
 
 int dia_galerkin(const hm_dia* fine, hm_dia* coarse, hipStream_t s) {
     const long long total = kSymSlots * coarse->n;
     k_galerkin<<<div_up(total, 256), 256, 0, s>>>(fine->c64, fine->h, fine->w, coarse->c64,
-                                                  coarse->h, coarse->w);
+                                                  coarse->c32, coarse->diag64, coarse->diag32,
+                                                  coarse->idiag64, coarse->idiag32, coarse->h,
+                                                  coarse->w);
     HM_LAUNCH_CHECK();
     return HM_OK;
 }

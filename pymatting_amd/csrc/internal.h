@@ -113,7 +113,7 @@ int trimap_classes(const double* trimap, long long n, double fg, double bg, uint
                    int64_t* counts, hipStream_t s);
 
 // grid.hip
-int dia_galerkin(const hm_dia* fine, hm_dia* coarse, hipStream_t s);
+int dia_galerkin(const hm_dia* fine, hm_dia* coarse, hipStream_t s);  // + dia_finish's outputs
 int dia_finish(hm_dia* A, hipStream_t s);  // diagonal + float32 copies
 int dia_pack(const double* full, hm_dia* A, hipStream_t s);     // [25][n] -> [13][n]
 int dia_unpack(const hm_dia* A, double* full, hipStream_t s);   // [13][n] -> [25][n]

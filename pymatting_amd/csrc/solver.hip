@@ -1018,9 +1018,8 @@ int vc_create(hm_cf* cf, hm_dia* dia, int h, int w, int pre, int post, double om
             rc = dia_alloc(ch, cw, &Lc.A);
             if (rc == HM_OK) {
                 Lc.owns_A = true;
-                rc = dia_galerkin(fine, Lc.A, s);
+                rc = dia_galerkin(fine, Lc.A, s);  // also the float32 copies and diagonals
             }
-            if (rc == HM_OK) rc = dia_finish(Lc.A, s);
             if (rc == HM_OK) rc = dev_alloc(&Lc.b, es * Lc.n);
             if (rc == HM_OK) rc = dev_alloc(&Lc.x, es * Lc.n);
             if (rc == HM_OK) rc = dev_alloc(&Lc.t, es * Lc.n);
