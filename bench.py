@@ -47,8 +47,8 @@ HBM_PEAK_GBS = 8000.0
 # two rocprofv3 --pmc passes over `python bench.py --pmc-probe` (FETCH_SIZE,
 # then WRITE_SIZE, KiB per dispatch), each scaled by what the same counter
 # reads for a 256 MiB copy (FETCH_SIZE x 1.9998, WRITE_SIZE x 1.0):
-# 57550.3 KiB x 1.9998 + 8100.0 KiB = 126.1 MB (algorithmic: 124.4 MB).
-TRAFFIC_SPMV_F32 = 126145628
+# 57668.3 KiB x 1.9998 + 8100.0 KiB = 126.4 MB (algorithmic: 124.4 MB).
+TRAFFIC_SPMV_F32 = 126384910
 
 # Algorithmic bytes per grid point and launch, by kernel class, for vector
 # element size s (8 float64, 4 float32). Matrix-free passes read the image
