@@ -151,3 +151,29 @@ def stream():
 
 def ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def numel(a):
+    """Element count of a numpy array, torch tensor or sequence, without copying it."""
+    if hasattr(a, "shape"):
+        n = 1
+        for d in tuple(a.shape):
+            n *= int(d)
+        return n
+    import numpy as np
+    return int(np.asarray(a).size)
+
+
+def release(obj, destroy):
+    """Destroys obj.handle with the library function `destroy`, once. Safe at
+    interpreter shutdown, when module globals and ctypes may already be gone:
+    the handle is then left to the process exit."""
+    h = getattr(obj, "handle", None)
+    if h is None:
+        return
+    obj.handle = None
+    try:
+        if h.value and _lib is not None:
+            getattr(_lib, destroy)(h)
+    except Exception:
+        pass

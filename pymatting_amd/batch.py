@@ -70,9 +70,14 @@ class BatchSolver:
 
     def _destroy(self):
         h = getattr(self, "_handle", None)
-        if h is not None and h.value and _ffi is not None and _ffi._lib is not None:
-            _ffi._lib.hm_batch_destroy(h)
         self._handle = None
+        if h is None:
+            return
+        try:  # at interpreter shutdown the handle is left to the process exit
+            if h.value and _ffi._lib is not None:
+                _ffi._lib.hm_batch_destroy(h)
+        except Exception:
+            pass
 
     def __del__(self):
         self._destroy()

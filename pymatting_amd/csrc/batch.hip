@@ -520,8 +520,7 @@ struct BatchPlan {
             unsigned* dimg = nullptr;
             HM_TRY(ar->upload(P, &dP));
             HM_TRY(ar->upload(img, &dimg));
-            HM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vc_tail_batch<T>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+            HM_TRY(set_max_dynamic_lds(reinterpret_cast<const void*>(&k_vc_tail_batch<T>),
                                        kTailLdsBytes));
             const int n = count;
             steps.push_back([=](hipStream_t s) {

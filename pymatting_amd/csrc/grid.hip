@@ -16,6 +16,24 @@
 
 namespace hm {
 
+// ------------------------------------------------------- kernel attributes
+
+// Raises a kernel's dynamic LDS limit once per (kernel, device); threads that
+// race here wait for the first one's call.
+int set_max_dynamic_lds(const void* kernel, int bytes) {
+    static std::mutex mu;
+    static std::unordered_map<const void*, std::vector<int>> done;
+    int dev = 0;
+    HM_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    std::vector<int>& devs = done[kernel];
+    for (int d : devs)
+        if (d == dev) return HM_OK;
+    HM_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    devs.push_back(dev);
+    return HM_OK;
+}
+
 // ------------------------------------------------------------ device memory
 
 // Device allocations go through a caching pool: a solve allocates dozens of
@@ -407,13 +425,8 @@ static int gauss_jordan(int n, double* M, double* inv64, float* inv32, hipStream
     HM_TRY(dev_alloc(reinterpret_cast<void**>(&status), sizeof(int)));
     HM_HIP(hipMemsetAsync(status, 0, sizeof(int), s));
     if (n <= kGjLds) {
-        static bool attr_set = false;
-        if (!attr_set) {
-            HM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gauss_jordan_lds),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)sizeof(double) * 2 * kGjLds * kGjLds));
-            attr_set = true;
-        }
+        HM_TRY(set_max_dynamic_lds(reinterpret_cast<const void*>(&k_gauss_jordan_lds),
+                                   (int)sizeof(double) * 2 * kGjLds * kGjLds));
         k_gauss_jordan_lds<<<1, 256, sizeof(double) * 2 * n * n, s>>>(n, M, inv64, inv32, status);
     } else {
         k_gauss_jordan<<<1, 1024, 0, s>>>(n, M, inv64, inv32, status);

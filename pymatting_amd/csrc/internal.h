@@ -127,6 +127,10 @@ int csr_apply(const hm_csr* A, const double* x, double* y, CgState* st, Reducer*
 int launch_finalize(int nvals, const Reducer& red, unsigned nblocks, CgState* st, int kind,
                     const CgState* guard, hipStream_t s);
 
+// Raises a kernel's dynamic LDS limit on the current device (once per
+// kernel and device, thread-safe).
+int set_max_dynamic_lds(const void* kernel, int bytes);
+
 // Device memory helpers (hipMalloc-backed).
 int dev_alloc(void** p, size_t bytes);
 void dev_free(void* p);

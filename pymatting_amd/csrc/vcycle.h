@@ -777,14 +777,10 @@ int launch_tail(hm_vc* vc, size_t start, const CgState* guard, hipStream_t s,
         set_error("V-cycle tail at level %zu has no packed contents", start);
         return HM_ERR_ARG;
     }
-    static bool attr_set[2] = {false, false};
-    if (!attr_set[sizeof(T) == 4]) {
-        HM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vc_tail<T, false>),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes));
-        HM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vc_tail<T, true>),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes));
-        attr_set[sizeof(T) == 4] = true;
-    }
+    HM_TRY(set_max_dynamic_lds(reinterpret_cast<const void*>(&k_vc_tail<T, false>),
+                               kTailLdsBytes));
+    HM_TRY(set_max_dynamic_lds(reinterpret_cast<const void*>(&k_vc_tail<T, true>),
+                               kTailLdsBytes));
     ProfScope prof(kProfDense, vc->levels[start].n, sizeof(T) == 4, s);
     if (side && side->x) {
         if (!side->p || !side->st || side->n < 0) {

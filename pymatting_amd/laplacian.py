@@ -15,6 +15,7 @@ import ctypes
 import numpy as np
 import scipy.sparse
 import torch
+import xxhash
 
 from pymatting_amd import _ffi
 from pymatting_amd._device import as_device, empty, is_torch, output_like
@@ -61,10 +62,7 @@ class CfOperator:
         self.handle = handle
 
     def __del__(self):
-        h = getattr(self, "handle", None)
-        if h is not None and h.value and _ffi is not None and _ffi._lib is not None:
-            _ffi._lib.hm_cf_destroy(h)
-            self.handle = None
+        _ffi.release(self, "hm_cf_destroy")
 
     # -- matrix-like interface -------------------------------------------
     def apply(self, x, out=None, precision="f64"):
@@ -190,10 +188,7 @@ class DiaOperator:
         return cls(coef, h, w)
 
     def __del__(self):
-        h = getattr(self, "handle", None)
-        if h is not None and h.value and _ffi is not None and _ffi._lib is not None:
-            _ffi._lib.hm_dia_destroy(h)
-            self.handle = None
+        _ffi.release(self, "hm_dia_destroy")
 
     def apply(self, x, out=None):
         out = out if out is not None else empty(self.n)
@@ -214,14 +209,34 @@ class DiaOperator:
 
 
 class CfLaplacianMatrix(scipy.sparse.csr_matrix):
-    """scipy CSR matrix carrying the device operator it was assembled from."""
+    """scipy CSR matrix carrying the device operator it was assembled from.
+
+    The operator stands for the matrix only while the matrix is unchanged:
+    a fingerprint of its arrays (their addresses and sizes, and a hash of the
+    values) is taken when the operator is attached, and ``device_operator``
+    drops the operator once an in-place edit (``L *= 2``, ``L.data[k] = v``,
+    ``setdiag``) has changed it. The edited matrix is then solved as the CSR
+    matrix it now is.
+    """
 
     operator = None
+    _fingerprint = None
+
+    def matches_operator(self):
+        return self.operator is not None and self._fingerprint == _fingerprint(self)
+
+
+def _fingerprint(csr):
+    arrays = (csr.data, csr.indices, csr.indptr)
+    return (tuple(a.__array_interface__["data"][0] for a in arrays),
+            tuple(a.size for a in arrays), tuple(csr.shape),
+            xxhash.xxh3_64_intdigest(np.ascontiguousarray(csr.data)))
 
 
 def _attach(csr, operator):
     out = CfLaplacianMatrix(csr)
     out.operator = operator
+    out._fingerprint = _fingerprint(out)
     return out
 
 
@@ -275,10 +290,11 @@ def make_linear_system(L, trimap, lambda_value=100.0, return_c=False):
 
 
 def device_operator(A):
-    """The device operator behind A (CfOperator/DiaOperator), or None."""
+    """The device operator behind A (CfOperator/DiaOperator), or None: a
+    matrix from cf_laplacian/make_linear_system yields its operator unless it
+    has been edited since."""
     if isinstance(A, (CfOperator, DiaOperator)):
         return A
-    op = getattr(A, "operator", None)
-    if isinstance(op, (CfOperator, DiaOperator)):
-        return op
+    if isinstance(A, CfLaplacianMatrix):
+        return A.operator if A.matches_operator() else None
     return None

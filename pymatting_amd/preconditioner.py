@@ -36,6 +36,9 @@ class Jacobi:
         self.n = self.inv_diag.numel()
 
     def __call__(self, x):
+        if _ffi.numel(x) != self.n:
+            raise ValueError("dimension mismatch: Jacobi preconditioner of size %d, vector of "
+                             "size %d" % (self.n, _ffi.numel(x)))
         xd = as_device(x).reshape(-1)
         out = empty(self.n)
         _ffi.check(_ffi.lib().hm_diag_apply(_ffi.ptr(self.inv_diag), _ffi.ptr(xd), _ffi.ptr(out),
@@ -114,10 +117,7 @@ class VCycle:
         return dense
 
     def __del__(self):
-        h = getattr(self, "handle", None)
-        if h is not None and h.value and _ffi is not None and _ffi._lib is not None:
-            _ffi._lib.hm_vcycle_destroy(h)
-            self.handle = None
+        _ffi.release(self, "hm_vcycle_destroy")
 
     def apply(self, r, out=None):
         dt = torch.float64 if self.precision == "f64" else torch.float32

@@ -58,10 +58,7 @@ class DeviceCSR:
         self.handle = handle
 
     def __del__(self):
-        h = getattr(self, "handle", None)
-        if h is not None and h.value and _ffi is not None and _ffi._lib is not None:
-            _ffi._lib.hm_csr_destroy(h)
-            self.handle = None
+        _ffi.release(self, "hm_csr_destroy")
 
 
 def _to_host(ptr, n):
@@ -109,6 +106,26 @@ class _HostIter:
             return 1
 
 
+def _check_sizes(A, op, M, x0, n):
+    """ValueError (dimension mismatch) unless A, M and x0 all match len(b)."""
+    def mismatch(what):
+        raise ValueError("dimension mismatch: %s, but b has size %d" % (what, n))
+    if op is not None:
+        if op.n != n:
+            mismatch("operator of size %d" % op.n)
+    elif scipy.sparse.issparse(A) or isinstance(A, np.ndarray):
+        if A.ndim != 2 or tuple(A.shape) != (n, n):
+            mismatch("matrix of shape %s" % (tuple(A.shape),))
+    if x0 is not None and _ffi.numel(x0) != n:
+        mismatch("x0 of size %d" % _ffi.numel(x0))
+    if isinstance(M, (Jacobi, VCycle)):
+        if M.n != n:
+            mismatch("preconditioner of size %d" % M.n)
+    elif scipy.sparse.issparse(M) or isinstance(M, np.ndarray):
+        if M.ndim != 2 or tuple(M.shape) != (n, n):
+            mismatch("preconditioner matrix of shape %s" % (tuple(M.shape),))
+
+
 def cg(A, b, x0=None, atol=0.0, rtol=1e-7, maxiter=10000, callback=None, M=None,
        reorthogonalize=False, check_every=8, return_stats=False):
     """cg.py:4-117: solves A x = b by preconditioned conjugate gradients.
@@ -120,11 +137,15 @@ def cg(A, b, x0=None, atol=0.0, rtol=1e-7, maxiter=10000, callback=None, M=None,
     Raises ValueError if the iteration does not converge within maxiter.
     """
     L = _ffi.lib()
+    n = _ffi.numel(b)
+    # Every size is checked before anything is uploaded or launched: the
+    # library runs over the operator's n and trusts the vectors' lengths
+    # (the reference raises ValueError from A.dot on a mismatch).
+    op = device_operator(A)
+    _check_sizes(A, op, M, x0, n)
     bd = as_device(b).reshape(-1)
-    n = bd.numel()
     args = _ffi.CgArgs()
     keep = []
-    op = device_operator(A)
     if isinstance(op, CfOperator):
         args.cf = op.handle
     elif isinstance(op, DiaOperator):
@@ -140,8 +161,6 @@ def cg(A, b, x0=None, atol=0.0, rtol=1e-7, maxiter=10000, callback=None, M=None,
         args.n = n
     else:
         raise TypeError("A must be a matrix or a callable")
-    if op is not None and op.n != n:
-        raise ValueError("dimension mismatch: operator of size %d, b of size %d" % (op.n, n))
 
     if M is None:
         args.pc_kind = _ffi.HM_PC_NONE
