@@ -259,6 +259,21 @@ int hm_fb_ml(const void* image, const void* alpha, int h0, int w0, int n_levels,
              int n_big, int small_size, double gradient_weight, int prec, void* F, void* B,
              void* stream);
 
+/* ---- allocation-free foreground (no counterpart in the reference) ----
+ * hm_fb_ml without any device allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, kernels and results as hm_fb_ml; the
+ * workspace is the library's static device array of 64 MiB per device
+ * (kept until the process ends), which serves float32 images up to about
+ * 0.8 Mpix (e.g. 1024x768) and float64 images up to about half that; larger
+ * ones are rejected with HM_ERR_ARG before any device work. The workspace
+ * is shared by every call on a device: calls that overlap on one device
+ * race. */
+int hm_fb_ml_noalloc(const void* image, const void* alpha, int h0, int w0, int n_levels,
+                     const int* level_h, const int* level_w, double regularization,
+                     int n_small, int n_big, int small_size, double gradient_weight, int prec,
+                     void* F, void* B, void* stream);
+
 /* ---- measurement (no reference counterpart) ----
  * hm_prof_set(1) starts recording HIP events around every launch of the
  * library's kernel classes (on the stream each kernel runs on) and discards

@@ -99,6 +99,8 @@ _SIGS = {
     "hm_prof_set": (i32, [i32]),
     "hm_prof_query": (i32, [i32, i32, ctypes.POINTER(f64), ctypes.POINTER(i64), ctypes.POINTER(i64)]),
     "hm_fb_ml": (i32, [P, P, i32, i32, i32, P, P, f64, i32, i32, i32, f64, i32, P, P, P]),
+    "hm_fb_ml_noalloc": (i32, [P, P, i32, i32, i32, P, P, f64, i32, i32, i32, f64, i32, P, P,
+                               P]),
     "hm_batch_create": (i32, [i32, P, P, i32, i32, i32, ctypes.POINTER(P)]),
     "hm_batch_solve": (i32, [P, P, P, f64, f64, i32, i32, i32, P, P]),
     "hm_batch_solve_reliable": (i32, [P, P, P, f64, f64, i32, P, P]),

@@ -30,6 +30,8 @@
 // Arithmetic follows the types Numba gives the reference: with float32
 // arrays, a0, a00, da and the b vector are float32 while a1 = 1.0 - a0 and
 // everything derived from it are float64. Compiled with -ffp-contract=off.
+#include <mutex>
+
 #include "common.h"
 #include "internal.h"
 
@@ -657,86 +659,114 @@ __global__ void k_fill_i32(int* p, long long n) {
 
 }  // namespace
 
+// Device buffers of one foreground estimate, carved out of one workspace in
+// a fixed order (each region 256-byte aligned). Sizes follow the pyramid:
+// the ping-pong colour buffers and the resized inputs hold the levels below
+// the finest, the coefficient planes and halo words the levels swept by the
+// strip kernel.
+struct FgLayout {
+    enum { kImg, kAlpha, kFa, kBa, kFb, kBb, kMeans, kHalo, kFail, kCs, kCd, kRed, kCount };
+    size_t off[kCount];
+    size_t total;
+    unsigned mean_blocks;
+};
+
+inline size_t fg_align(size_t b) { return (b + 255) / 256 * 256; }
+
 template <typename S>
-int fb_ml_t(const S* img0, const S* alpha0, int h0, int w0, int n_levels, const int* lh,
-            const int* lw, double reg, int n_small, int n_big, int small_size, double gw,
-            S* F_out, S* B_out, hipStream_t s) {
+FgLayout fg_layout(int h0, int w0, int n_levels, const int* lh, const int* lw, int n_small,
+                   int n_big, int small_size) {
     const long long n0 = (long long)h0 * w0;
-    void *imgb = nullptr, *alphab = nullptr, *Fa = nullptr, *Ba = nullptr, *Fb = nullptr,
-         *Bb = nullptr, *means = nullptr, *prog = nullptr, *fail = nullptr, *cs = nullptr,
-         *cd = nullptr;
-    Reducer red{};
-    int rc = HM_OK;
-    const int max_strips = (int)div_up(h0, kWave);
-    auto cleanup = [&]() {
-        dev_free(imgb); dev_free(alphab); dev_free(Fa); dev_free(Ba); dev_free(Fb); dev_free(Bb);
-        dev_free(means); dev_free(prog); dev_free(fail); dev_free(cs); dev_free(cd);
-        free_reducer(&red);
-    };
-    const unsigned mean_blocks = div_up(n0, (long long)kBlockMean * 16) > 4096
-                                     ? 4096u
-                                     : div_up(n0, (long long)kBlockMean * 16);
-    if ((rc = dev_alloc(&imgb, sizeof(S) * n0 * 3)) != HM_OK ||
-        (rc = dev_alloc(&alphab, sizeof(S) * n0)) != HM_OK ||
-        (rc = dev_alloc(&Fa, sizeof(S) * n0 * 3)) != HM_OK ||
-        (rc = dev_alloc(&Ba, sizeof(S) * n0 * 3)) != HM_OK ||
-        (rc = dev_alloc(&Fb, sizeof(S) * n0 * 3)) != HM_OK ||
-        (rc = dev_alloc(&Bb, sizeof(S) * n0 * 3)) != HM_OK ||
-        (rc = dev_alloc(&means, sizeof(S) * 8)) != HM_OK ||
-        (rc = dev_alloc(&prog, sizeof(typename FgWord<S>::U) * 6 * (long long)max_strips * w0)) !=
-            HM_OK ||
-        (rc = dev_alloc(&fail, sizeof(int))) != HM_OK ||
-        (rc = dev_alloc(&cs, sizeof(S) * n0 * 10)) != HM_OK ||
-        (rc = dev_alloc(&cd, sizeof(double) * n0 * 3)) != HM_OK ||
-        (rc = make_reducer(&red, (int)mean_blocks + 1, 8)) != HM_OK) {
-        cleanup();
-        return rc;
+    long long below = 1, strip = 0, halo = 0;  // points of the largest such level
+    for (int l = 0; l <= n_levels; ++l) {
+        const long long np = (long long)lh[l] * lw[l];
+        if (l < n_levels && np > below) below = np;
+        const int n_iter = (lw[l] <= small_size && lh[l] <= small_size) ? n_small : n_big;
+        const bool one_wg = (size_t)np * fg_small_bytes_per_px<S>() <= kFgSmallLds &&
+                            n_iter <= kFgSmallMaxIter;
+        if (n_iter > 0 && !one_wg) {
+            if (np > strip) strip = np;
+            const long long nh = 6LL * div_up(lh[l], kWave) * lw[l];
+            if (nh > halo) halo = nh;
+        }
     }
-    k_fill_i32<<<1, 64, 0, s>>>(static_cast<int*>(fail), 1);
+    FgLayout L{};
+    L.mean_blocks = div_up(n0, (long long)kBlockMean * 16) > 4096
+                        ? 4096u
+                        : div_up(n0, (long long)kBlockMean * 16);
+    const size_t bytes[FgLayout::kCount] = {
+        sizeof(S) * below * 3, sizeof(S) * below, sizeof(S) * below * 3, sizeof(S) * below * 3,
+        sizeof(S) * below * 3, sizeof(S) * below * 3, sizeof(S) * 8,
+        sizeof(typename FgWord<S>::U) * (size_t)(halo > 0 ? halo : 1), sizeof(int),
+        sizeof(S) * (size_t)(strip > 0 ? strip : 1) * 10,
+        sizeof(double) * (size_t)(strip > 0 ? strip : 1) * 3,
+        sizeof(double) * 8 * ((size_t)L.mean_blocks + 1)};
+    size_t at = 0;
+    for (int k = 0; k < FgLayout::kCount; ++k) {
+        L.off[k] = at;
+        at += fg_align(bytes[k]);
+    }
+    L.total = at;
+    return L;
+}
+
+// The pyramid of estimate_foreground_ml.py:98-183 on the buffers of `work`
+// (laid out by fg_layout; its contents on entry do not matter: everything
+// that must start at zero is cleared by a kernel).
+template <typename S>
+int fb_ml_run(const S* img0, const S* alpha0, int h0, int w0, int n_levels, const int* lh,
+              const int* lw, double reg, int n_small, int n_big, int small_size, double gw,
+              S* F_out, S* B_out, const FgLayout& lay, unsigned char* work, hipStream_t s) {
+    const long long n0 = (long long)h0 * w0;
+    auto at = [&](int k) { return static_cast<void*>(work + lay.off[k]); };
+    S* imgb = static_cast<S*>(at(FgLayout::kImg));
+    S* alphab = static_cast<S*>(at(FgLayout::kAlpha));
+    S* Fa = static_cast<S*>(at(FgLayout::kFa));
+    S* Ba = static_cast<S*>(at(FgLayout::kBa));
+    S* Fb = static_cast<S*>(at(FgLayout::kFb));
+    S* Bb = static_cast<S*>(at(FgLayout::kBb));
+    S* means = static_cast<S*>(at(FgLayout::kMeans));
+    int* fail = static_cast<int*>(at(FgLayout::kFail));
+    S* cs = static_cast<S*>(at(FgLayout::kCs));
+    double* cd = static_cast<double*>(at(FgLayout::kCd));
+    Reducer red{static_cast<double*>(at(FgLayout::kRed)), (int)lay.mean_blocks + 1, 0};
+    const unsigned mean_blocks = lay.mean_blocks;
+    int rc = HM_OK;
+    k_fill_i32<<<1, 64, 0, s>>>(fail, 1);
     hipError_t e = hipSuccess;
-    k_fg_means<S><<<mean_blocks, kBlockMean, 0, s>>>(img0, alpha0, n0, red, static_cast<S*>(means));
-    k_fg_means_final<S><<<1, kBlockMean, 0, s>>>(red, mean_blocks, static_cast<S*>(means));
+    k_fg_means<S><<<mean_blocks, kBlockMean, 0, s>>>(img0, alpha0, n0, red, means);
+    k_fg_means_final<S><<<1, kBlockMean, 0, s>>>(red, mean_blocks, means);
     // Level "-1": 1x1 F/B initialised with the mean colours.
-    S* Fp = static_cast<S*>(Fa);
-    S* Bp = static_cast<S*>(Ba);
-    k_fg_init<S><<<1, 64, 0, s>>>(static_cast<S*>(means), Fp, Bp);
+    S* Fp = Fa;
+    S* Bp = Ba;
+    k_fg_init<S><<<1, 64, 0, s>>>(means, Fp, Bp);
     int hp = 1, wp = 1;
     using W = typename FgWord<S>::U;
-    W* halo = static_cast<W*>(prog);
+    W* halo = static_cast<W*>(at(FgLayout::kHalo));
     for (int l = 0; l <= n_levels; ++l) {
         const int h = lh[l], w = lw[l];
         const bool full = (h == h0 && w == w0);
         const bool last = (l == n_levels);
-        S* Fc = last ? F_out : (Fp == static_cast<S*>(Fa) ? static_cast<S*>(Fb) : static_cast<S*>(Fa));
-        S* Bc = last ? B_out : (Bp == static_cast<S*>(Ba) ? static_cast<S*>(Bb) : static_cast<S*>(Ba));
-        const S* img = full ? img0 : static_cast<S*>(imgb);
-        const S* alpha = full ? alpha0 : static_cast<S*>(alphab);
+        S* Fc = last ? F_out : (Fp == Fa ? Fb : Fa);
+        S* Bc = last ? B_out : (Bp == Ba ? Bb : Ba);
+        const S* img = full ? img0 : imgb;
+        const S* alpha = full ? alpha0 : alphab;
         k_fg_resize<S><<<div_up((long long)h * w, 256), 256, 0, s>>>(
-            img0, alpha0, h0, w0, Fp, Bp, hp, wp, static_cast<S*>(imgb), static_cast<S*>(alphab),
-            Fc, Bc, h, w, full ? 0 : 1);
+            img0, alpha0, h0, w0, Fp, Bp, hp, wp, imgb, alphab, Fc, Bc, h, w, full ? 0 : 1);
         const int n_iter = (w <= small_size && h <= small_size) ? n_small : n_big;
         const size_t small_lds = (size_t)h * w * fg_small_bytes_per_px<S>();
         const bool one_wg = small_lds <= kFgSmallLds && n_iter <= kFgSmallMaxIter;
         if (n_iter > 0 && one_wg) {
             // The whole level in one workgroup (k_fg_small).
-            static bool attr_set = false;
-            if (!attr_set) {
-                e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fg_small<S>),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)kFgSmallLds);
-                if (e != hipSuccess) {
-                    cleanup();
-                    set_error("foreground: %s", hipGetErrorString(e));
-                    return HM_ERR_HIP;
-                }
-                attr_set = true;
-            }
+            rc = set_max_dynamic_lds(reinterpret_cast<const void*>(&k_fg_small<S>),
+                                     (int)kFgSmallLds);
+            if (rc != HM_OK) return rc;
             ProfScope prof(kProfFgSweep, (long long)h * w * n_iter, sizeof(S) == 4, s);
             k_fg_small<S><<<1, kFgSmallThreads, small_lds, s>>>(img, alpha, h, w, (S)reg, (S)gw,
                                                                Fc, Bc, n_iter);
         } else if (n_iter > 0) {
-            k_fg_coef<S><<<div_up((long long)h * w, 256), 256, 0, s>>>(
-                img, alpha, h, w, (S)reg, (S)gw, static_cast<S*>(cs), static_cast<double*>(cd));
+            k_fg_coef<S><<<div_up((long long)h * w, 256), 256, 0, s>>>(img, alpha, h, w, (S)reg,
+                                                                      (S)gw, cs, cd);
             // Halo words of this level start untagged; sweeps cycle tags 1..3.
             const long long nh = 6LL * div_up(h, kWave) * w;
             k_fill_words<W><<<div_up(nh, 256), 256, 0, s>>>(halo, nh);
@@ -744,11 +774,13 @@ int fb_ml_t(const S* img0, const S* alpha0, int h0, int w0, int n_levels, const 
         for (int it = 0; it < n_iter && !one_wg; ++it) {
             ProfScope prof(kProfFgSweep, (long long)h * w, sizeof(S) == 4, s);
             k_fg_sweep<S><<<div_up(h, kWave), kFgThreads, 0, s>>>(
-                static_cast<const S*>(cs), static_cast<const double*>(cd), Fc, Bc, h, w, halo,
-                1u + (unsigned)(it % 3), static_cast<int*>(fail));
+                cs, cd, Fc, Bc, h, w, halo, 1u + (unsigned)(it % 3), fail);
         }
         e = hipGetLastError();
-        if (e != hipSuccess) { cleanup(); set_error("foreground launch: %s", hipGetErrorString(e)); return HM_ERR_HIP; }
+        if (e != hipSuccess) {
+            set_error("foreground launch: %s", hipGetErrorString(e));
+            return HM_ERR_HIP;
+        }
         Fp = Fc;
         Bp = Bc;
         hp = h;
@@ -757,7 +789,6 @@ int fb_ml_t(const S* img0, const S* alpha0, int h0, int w0, int n_levels, const 
     int host_fail = 0;
     e = hipMemcpyAsync(&host_fail, fail, sizeof(int), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    cleanup();
     if (e != hipSuccess) {
         set_error("foreground: %s", hipGetErrorString(e));
         return HM_ERR_HIP;
@@ -769,28 +800,105 @@ int fb_ml_t(const S* img0, const S* alpha0, int h0, int w0, int n_levels, const 
     return HM_OK;
 }
 
+template <typename S>
+int fb_ml_t(const S* img0, const S* alpha0, int h0, int w0, int n_levels, const int* lh,
+            const int* lw, double reg, int n_small, int n_big, int small_size, double gw,
+            S* F_out, S* B_out, hipStream_t s) {
+    const FgLayout lay = fg_layout<S>(h0, w0, n_levels, lh, lw, n_small, n_big, small_size);
+    void* work = nullptr;
+    HM_TRY(dev_alloc(&work, lay.total));
+    const int rc = fb_ml_run<S>(img0, alpha0, h0, w0, n_levels, lh, lw, reg, n_small, n_big,
+                                small_size, gw, F_out, B_out, lay,
+                                static_cast<unsigned char*>(work), s);
+    dev_free(work);
+    return rc;
+}
+
+// ------------------------------------------- the allocation-free variant
+//
+// hm_fb_ml_noalloc runs the same kernels on a workspace that is a file-scope
+// device array, so it makes no allocation (for callers that cannot allocate:
+// code under graph capture, serving loops, harnesses that forbid hipMalloc).
+// Each device holds its own copy of the array for the life of the process;
+// its address on a device is looked up once, on that device's first call.
+// Every call on one device shares that device's array: calls that overlap
+// on one device (on different streams or threads) race and must not be made.
+constexpr size_t kFgScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kFgScratchBytes];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+}  // namespace
+
+// The current device's copy of `scratch` (a device address usable in kernel
+// arguments; the array's name itself is a host-side stand-in).
+static int scratch_address(unsigned char** out) {
+    int dev = 0;
+    HM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices) {
+        set_error("hm_fb_ml_noalloc: device %d is beyond the %d devices served", dev,
+                  kMaxScratchDevices);
+        return HM_ERR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = static_cast<unsigned char*>(g_scratch_ptr[dev]);
+    return HM_OK;
+}
+
+template <typename S>
+int fb_ml_noalloc_t(const S* img0, const S* alpha0, int h0, int w0, int n_levels, const int* lh,
+                    const int* lw, double reg, int n_small, int n_big, int small_size, double gw,
+                    S* F_out, S* B_out, hipStream_t s) {
+    const FgLayout lay = fg_layout<S>(h0, w0, n_levels, lh, lw, n_small, n_big, small_size);
+    if (lay.total > kFgScratchBytes) {
+        set_error("hm_fb_ml_noalloc: a %dx%d %s image needs %zu bytes of workspace, more than "
+                  "the %zu bytes of the library's static workspace",
+                  h0, w0, sizeof(S) == 4 ? "float32" : "float64", lay.total, kFgScratchBytes);
+        return HM_ERR_ARG;
+    }
+    unsigned char* work = nullptr;
+    HM_TRY(scratch_address(&work));
+    return fb_ml_run<S>(img0, alpha0, h0, w0, n_levels, lh, lw, reg, n_small, n_big, small_size,
+                        gw, F_out, B_out, lay, work, s);
+}
+
 }  // namespace hm
+
+static int fb_ml_check(const void* image, const void* alpha, int h0, int w0, int n_levels,
+                       const int* level_h, const int* level_w, const void* F, const void* B,
+                       const char* who) {
+    using namespace hm;
+    if (!image || !alpha || !F || !B || !level_h || !level_w || h0 < 1 || w0 < 1 ||
+        n_levels < 1) {
+        set_error("%s: invalid argument", who);
+        return HM_ERR_ARG;
+    }
+    if (level_h[n_levels] != h0 || level_w[n_levels] != w0 || level_h[0] != 1 ||
+        level_w[0] != 1) {
+        set_error("%s: the pyramid must run from 1x1 to the image size", who);
+        return HM_ERR_ARG;
+    }
+    for (int l = 0; l <= n_levels; ++l)
+        if (level_h[l] < 1 || level_w[l] < 1 || level_h[l] > h0 || level_w[l] > w0) {
+            set_error("%s: invalid level size", who);
+            return HM_ERR_ARG;
+        }
+    return HM_OK;
+}
 
 extern "C" int hm_fb_ml(const void* image, const void* alpha, int h0, int w0, int n_levels,
                         const int* level_h, const int* level_w, double regularization,
                         int n_small, int n_big, int small_size, double gradient_weight, int prec,
                         void* F, void* B, void* stream) {
     using namespace hm;
-    if (!image || !alpha || !F || !B || !level_h || !level_w || h0 < 1 || w0 < 1 ||
-        n_levels < 1) {
-        set_error("hm_fb_ml: invalid argument");
-        return HM_ERR_ARG;
-    }
-    if (level_h[n_levels] != h0 || level_w[n_levels] != w0 || level_h[0] != 1 ||
-        level_w[0] != 1) {
-        set_error("hm_fb_ml: the pyramid must run from 1x1 to the image size");
-        return HM_ERR_ARG;
-    }
-    for (int l = 0; l <= n_levels; ++l)
-        if (level_h[l] < 1 || level_w[l] < 1 || level_h[l] > h0 || level_w[l] > w0) {
-            set_error("hm_fb_ml: invalid level size");
-            return HM_ERR_ARG;
-        }
+    HM_TRY(fb_ml_check(image, alpha, h0, w0, n_levels, level_h, level_w, F, B, "hm_fb_ml"));
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (prec == HM_F32)
         return fb_ml_t<float>(static_cast<const float*>(image), static_cast<const float*>(alpha),
@@ -801,4 +909,26 @@ extern "C" int hm_fb_ml(const void* image, const void* alpha, int h0, int w0, in
                            h0, w0, n_levels, level_h, level_w, regularization, n_small, n_big,
                            small_size, gradient_weight, static_cast<double*>(F),
                            static_cast<double*>(B), s);
+}
+
+extern "C" int hm_fb_ml_noalloc(const void* image, const void* alpha, int h0, int w0,
+                                int n_levels, const int* level_h, const int* level_w,
+                                double regularization, int n_small, int n_big, int small_size,
+                                double gradient_weight, int prec, void* F, void* B,
+                                void* stream) {
+    using namespace hm;
+    HM_TRY(fb_ml_check(image, alpha, h0, w0, n_levels, level_h, level_w, F, B,
+                       "hm_fb_ml_noalloc"));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (prec == HM_F32)
+        return fb_ml_noalloc_t<float>(static_cast<const float*>(image),
+                                      static_cast<const float*>(alpha), h0, w0, n_levels,
+                                      level_h, level_w, regularization, n_small, n_big,
+                                      small_size, gradient_weight, static_cast<float*>(F),
+                                      static_cast<float*>(B), s);
+    return fb_ml_noalloc_t<double>(static_cast<const double*>(image),
+                                   static_cast<const double*>(alpha), h0, w0, n_levels, level_h,
+                                   level_w, regularization, n_small, n_big, small_size,
+                                   gradient_weight, static_cast<double*>(F),
+                                   static_cast<double*>(B), s);
 }

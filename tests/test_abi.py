@@ -87,6 +87,15 @@ def test_argument_checks_before_any_device_call():
     assert lib.hm_fb_ml(buf, buf, 4, 5, 2, bad_lh, lw, 1e-5, 10, 2, 32, 1.0, 1, buf, buf,
                         None) == ARG
     assert "from 1x1" in _err(lib)
+    # The allocation-free foreground rejects images beyond its static
+    # workspace before any device work (1080p float32 needs ~165 MB).
+    lib.hm_fb_ml_noalloc.argtypes = lib.hm_fb_ml.argtypes
+    n_levels = 11
+    lh = (i32 * (n_levels + 1))(*[round(1080 ** (i / n_levels)) for i in range(n_levels + 1)])
+    lw = (i32 * (n_levels + 1))(*[round(1920 ** (i / n_levels)) for i in range(n_levels + 1)])
+    assert lib.hm_fb_ml_noalloc(buf, buf, 1080, 1920, n_levels, lh, lw, 1e-5, 10, 2, 32, 1.0, 1,
+                                buf, buf, None) == ARG
+    assert "static workspace" in _err(lib)
     # Vector helpers: negative sizes are rejected, empty ones do nothing.
     lib.hm_diag_apply.argtypes = [P, P, P, ctypes.c_int64, P]
     assert lib.hm_diag_apply(None, None, None, 4, None) == ARG
