@@ -31,7 +31,13 @@ beside it. The CPU baseline times the oracle's restatement of the reference
 (scipy CSR, vcycle PCG at rtol 1e-7, float64) single-threaded on the same
 1920x1080 workload, one solve.
 
-Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+The line also carries the other configurations' figures, measured in the
+same run: "c3" (64 images batched vs one at a time), "c4" (3840x2160 with a
+5-level V(2,2) cycle, and the float32 foreground against float64) and "c5"
+(gradient of a 16x512x512 batch against finite differences and the float64
+adjoint). --c2-only leaves them out.
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--c2-only]
        python bench.py --pmc-probe   (SpMV launches and a 256 MiB copy only,
                                       for hardware-counter runs)
 """
@@ -185,7 +191,8 @@ def cpu_baseline(alpha_path=None):
             "cpu_model": _cpu_model()}
 
 
-def bench_c3(args):
+def bench_c3(args, forms_run=("f64", "mixed", "f64_rtol_1e-7", "mixed_rtol_1e-7",
+                               "mixed_refine")):
     """64 images, heights and widths uniform in 256..1024 (step 16), unknown
     bands 5-40 px: all alpha solves (set-up + PCG + clip) batched, against a
     loop of one-image solves with the same settings. Float64 PCG and the mixed
@@ -242,7 +249,7 @@ def bench_c3(args):
                                                                         maxiter=100000)
            for im, tr in zip(images, trimaps)]
     res, batched_alphas = {}, {}
-    for form in ("f64", "mixed", "f64_rtol_1e-7", "mixed_rtol_1e-7", "mixed_refine"):
+    for form in forms_run:
         tuner = pm.BatchSolver(images, trimaps, precision=forms[form]["precision"])
         cfg = {k: v for k, v in forms[form].items() if k != "precision"}
         choice = tuner.autotune(**cfg)
@@ -282,10 +289,10 @@ def bench_c3(args):
                                "images with h, w uniform in 256..1024 step 16, bands 5-40 px; "
                                "set-up + batched PCG + clip; mixed form with reliable updates",
                    "images": len(images), "total_pixels": npix},
-        "f64": res["f64"], "mixed": res["mixed"], "mixed_refine": res["mixed_refine"],
+        "f64": res["f64"], "mixed": res["mixed"], "mixed_refine": res.get("mixed_refine"),
         "rtol_1e-7": {"f64": res["f64_rtol_1e-7"], "mixed": res["mixed_rtol_1e-7"]},
     }
-    print(json.dumps(line))
+    return line
 
 
 def bench_c4(args):
@@ -393,7 +400,7 @@ def bench_c4(args):
         "foreground_f32": {"ms": round(t_fg * 1e3, 3), "Mpix_per_s": round(n / t_fg / 1e6, 2),
                            "max_abs_dF_vs_f64": dF, "max_abs_dB_vs_f64": dB},
     }
-    print(json.dumps(line))
+    return line
 
 
 def bench_c5(args):
@@ -488,7 +495,7 @@ def bench_c5(args):
                      "max_abs_err_over_max_grad": float(max(fd_err)),
                      "median_abs_err_over_max_grad": float(np.median(fd_err))},
     }
-    print(json.dumps(line))
+    return line
 
 
 def main():
@@ -498,6 +505,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pmc-probe", action="store_true")
+    ap.add_argument("--c2-only", action="store_true",
+                    help="the 1920x1080 line alone, without the C3-C5 entries")
     ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c2",
                     help="c2: 1920x1080 solve (the metric); c3: 64 images of 256..1024 px, "
                          "batched vs one at a time; c4: 3840x2160 5-level V(2,2) solve and "
@@ -510,14 +519,8 @@ def main():
     if args.pmc_probe:
         pmc_probe()
         return
-    if args.workload == "c3":
-        bench_c3(args)
-        return
-    if args.workload == "c4":
-        bench_c4(args)
-        return
-    if args.workload == "c5":
-        bench_c5(args)
+    if args.workload in ("c3", "c4", "c5"):
+        print(json.dumps({"c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.workload](args)))
         return
 
     import numpy as np
@@ -737,7 +740,27 @@ def main():
                              "GBps_over_step": round(step_bytes / t_mixed / 1e9, 1)},
         "cpu_baseline": cpu,
     }
+    if not args.c2_only:
+        # The other configurations' lines, measured in the same run: C3 (64
+        # batched images; the refinement form only under --workload c3), C4
+        # (3840x2160 and its float32 foreground) and C5 (16x512x512
+        # gradient), each with its accuracy figures.
+        del last["solver"], op
+        torch.cuda.empty_cache()
+        line["c3"] = _sub_line(bench_c3(args, ("f64", "mixed", "f64_rtol_1e-7",
+                                               "mixed_rtol_1e-7")))
+        torch.cuda.empty_cache()
+        line["c4"] = _sub_line(bench_c4(args))
+        torch.cuda.empty_cache()
+        line["c5"] = _sub_line(bench_c5(args))
     print(json.dumps(line))
+
+
+def _sub_line(line):
+    """A configuration's line as an entry of the main line (without the keys
+    that repeat the run's own)."""
+    return {k: v for k, v in line.items()
+            if k not in ("n_gpus", "steps", "warmup", "scaling", "vs_baseline", "data")}
 
 
 if __name__ == "__main__":
