@@ -41,11 +41,13 @@ extern "C" {
 #define HM_PC_VCYCLE 2
 #define HM_PC_CSR 3  /* z = M r with a matrix M (cg.py:59-62) */
 #define HM_PC_HOST 4 /* z = M(r) through a host callback (callable M, cg.py:57-58) */
+#define HM_PC_ICHOL 5 /* z = L^-T L^-1 r with an incomplete Cholesky factor (ichol.py:175-181) */
 
 typedef struct hm_cf hm_cf;   /* device image + closed-form window operator */
 typedef struct hm_dia hm_dia; /* stored 25-point grid stencil (coarse levels, ELL form) */
 typedef struct hm_csr hm_csr; /* general CSR matrix (cg on arbitrary matrices) */
 typedef struct hm_vc hm_vc;   /* V-cycle hierarchy */
+typedef struct hm_ichol hm_ichol; /* incomplete Cholesky factor with its device solves */
 
 typedef struct hm_cg_stats {
     int iterations;     /* completed CG iterations */
@@ -95,6 +97,15 @@ int hm_cf_system_indptr(hm_cf* op, int64_t* indptr, int64_t* nnz, void* stream);
 int hm_cf_system_fill(hm_cf* op, const int64_t* indptr, int64_t* indices, double* values,
                       void* stream);
 
+/* cf_laplacian.py:5-177 for any radius r >= 1: the reference's CSR with
+ * (4r+1)^2 slots per row (values n*(4r+1)^2, indices likewise, indptr n+1,
+ * device arrays), entries summed over windows in the reference's order.
+ * known (nullable, device uint8, 1 = known) skips windows whose pixels are
+ * all known (cf_laplacian.py:34-35). */
+int hm_cf_laplacian_csr(const double* image_hwc, int h, int w, double epsilon, int radius,
+                        const uint8_t* known, double* values, int64_t* indices, int64_t* indptr,
+                        void* stream);
+
 /* ---- stored 25-point operators (ELL/DIA, slot s = (dy+2)*5 + (dx+2)) ----
  * coef: device float64 [25][h*w] slot-major. The operator must be symmetric
  * (as every matrix handed to CG is): the library stores the 13 slots with
@@ -111,6 +122,39 @@ int hm_csr_create(int64_t n, int64_t nnz, const int64_t* indptr, const int64_t* 
                   const double* values, hm_csr** out);
 int hm_csr_destroy(hm_csr* A);
 int hm_csr_apply(hm_csr* A, const double* x, double* y, void* stream);
+/* The reduced system of estimate_alpha_cf.py:57-70 from a CSR Laplacian L
+ * (device arrays, e.g. the reference layout of cf_laplacian) and the class
+ * map cls (0 unknown, 1 foreground, 2 background): L_U = L[U, :][:, U] as
+ * scipy's fancy indexing leaves it (the unknown rows in order; in each, the
+ * entries whose column is unknown, renumbered by rank, in stored order,
+ * duplicates and explicit zeros kept) and rhs = -R m with
+ * R = L[U, :][:, K], m = is_fg[K]. hm_csr_reduced_indptr fills indptr_u
+ * (device int64, room for n + 1) and returns n_u and nnz_u;
+ * hm_csr_reduced_fill writes indices_u, values_u (nnz_u) and rhs_u (n_u).
+ * hm_reduced_finalize writes alpha = clip(x, 0, 1) over all n pixels with
+ * x = trimap on known pixels and x_u on unknown ones (:72-76). */
+int hm_csr_reduced_indptr(int64_t n, const int64_t* indptr, const int64_t* indices,
+                          const uint8_t* cls, int64_t* indptr_u, int64_t* n_u, int64_t* nnz_u,
+                          void* stream);
+int hm_csr_reduced_fill(int64_t n, const int64_t* indptr, const int64_t* indices,
+                        const double* values, const uint8_t* cls, const int64_t* indptr_u,
+                        int64_t* indices_u, double* values_u, double* rhs_u, void* stream);
+int hm_reduced_finalize(int64_t n, const uint8_t* cls, const double* x_u, const double* trimap,
+                        double* alpha, void* stream);
+/* make_linear_system (laplacian.py:49-55) on an arbitrary CSR matrix A (device
+ * arrays): the canonical CSR of A + diag(diag) that scipy leaves after
+ * L + diags(c), tocsr() and sum_duplicates(): per row, equal columns summed
+ * in stored order, diag[i] added on the diagonal, zero sums dropped, columns
+ * ascending. hm_csr_add_diagonal_indptr fills out_indptr (n + 1) and
+ * returns nnz; hm_csr_add_diagonal_fill writes the indices and values. Rows
+ * may hold up to 2048 entries (HM_ERR_UNSUPPORTED beyond). */
+int hm_csr_add_diagonal_indptr(int64_t n, const int64_t* indptr, const int64_t* indices,
+                               const double* values, const double* diag, int64_t* out_indptr,
+                               int64_t* nnz, void* stream);
+int hm_csr_add_diagonal_fill(int64_t n, const int64_t* indptr, const int64_t* indices,
+                             const double* values, const double* diag,
+                             const int64_t* out_indptr, int64_t* out_indices, double* out_values,
+                             void* stream);
 
 /* ---- vcycle: pymatting/preconditioner/vcycle.py:6-153 ----
  * Geometric multigrid on the (h, w) grid of the operator: full-weighting
@@ -177,6 +221,7 @@ typedef struct hm_cg_args {
     int check_every;          /* iterations enqueued between host checks */
     hm_iter_cb callback;      /* nullable */
     void* cb_ctx;
+    hm_ichol* ichol;          /* HM_PC_ICHOL */
 } hm_cg_args;
 
 /* Preconditioned CG. Returns HM_ERR_NOCONV after maxiter iterations without
@@ -202,6 +247,38 @@ int hm_cg_solve_mixed(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, doub
 int hm_cg_solve_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x,
                          double rtol, double delta, int maxiter, int check_every,
                          hm_cg_stats* stats, void* stream);
+
+/* ---- ichol: pymatting/preconditioner/ichol.py:6-301 ----
+ * hm_ichol_factor: one attempt of the thresholded incomplete Cholesky
+ * factorisation (_ichol, ichol.py:6-113) of a canonical CSC matrix given in
+ * host memory (its lower triangle is used), with `shift` added to the
+ * diagonal. *result receives the factor's nnz, or -1 when a pivot is not
+ * positive, or -2 when more than max_nnz entries would be stored (the
+ * reference's return codes; HM_OK is returned in all three cases and *out
+ * is null for -1/-2). The factorisation runs on the host in one thread and
+ * makes no device call; the factor and the level schedules of its two
+ * triangular solves are uploaded by the first hm_ichol_apply (or the first
+ * CG iteration that uses it), on that call's stream. hm_ichol_create makes
+ * the handle of a given factor (CholeskyDecomposition(Ltuple),
+ * ichol.py:148-162) from host CSC arrays whose columns start with the
+ * diagonal, rows ascending. */
+int hm_ichol_factor(int64_t n, const int64_t* indptr, const int64_t* indices,
+                    const double* values, double discard_threshold, double shift,
+                    int64_t max_nnz, double relative_discard_threshold, int diag_keep_discarded,
+                    int64_t* result, hm_ichol** out);
+int hm_ichol_create(int64_t n, const double* Lv, const int64_t* Lr, const int64_t* Lp,
+                    hm_ichol** out);
+int64_t hm_ichol_nnz(const hm_ichol* h);
+/* Number of levels of the forward and backward solves. */
+int hm_ichol_levels(const hm_ichol* h, int* forward, int* backward);
+/* Copies the factor (CSC: Lv[nnz], Lr[nnz], Lp[n+1]) to host arrays. */
+int hm_ichol_copy_factor(const hm_ichol* h, double* Lv, int64_t* Lr, int64_t* Lp);
+/* z = L^-T L^-1 r (CholeskyDecomposition.__call__, ichol.py:175-181): forward
+ * then back substitution on device float64 vectors, bit-identical to the
+ * reference's substitutions. Calls on one handle must not overlap (it keeps
+ * the intermediate vector). */
+int hm_ichol_apply(hm_ichol* h, const double* r, double* z, void* stream);
+int hm_ichol_destroy(hm_ichol* h);
 
 /* z = r * inv_diag (jacobi.py:24-31 applied to a vector). */
 int hm_diag_apply(const double* inv_diag, const double* r, double* z, int64_t n, void* stream);

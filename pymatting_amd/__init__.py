@@ -19,7 +19,7 @@ from pymatting_amd.batch import BatchSolver
 from pymatting_amd.foreground import estimate_foreground_ml
 from pymatting_amd.laplacian import (CfLaplacianMatrix, CfOperator, DiaOperator, cf_laplacian,
                                      make_linear_system)
-from pymatting_amd.preconditioner import ichol, jacobi, vcycle
+from pymatting_amd.preconditioner import CholeskyDecomposition, ichol, jacobi, vcycle
 from pymatting_amd.solver import CounterCallback, ProgressCallback, cg
 from pymatting_amd.util import sanity_check_image, trimap_split
 
@@ -28,5 +28,5 @@ __all__ = [
     "vcycle", "ichol", "estimate_alpha_cf", "estimate_alpha", "estimate_foreground_ml",
     "trimap_split", "sanity_check_image", "CfOperator", "DiaOperator", "CfLaplacianMatrix",
     "LambdaSolver", "solve_alpha_lambda", "AlphaSolve", "BatchAlphaSolve", "solve_alpha",
-    "BatchSolver",
+    "BatchSolver", "CholeskyDecomposition",
 ]

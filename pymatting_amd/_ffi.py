@@ -33,6 +33,7 @@ HM_PC_JACOBI = 1
 HM_PC_VCYCLE = 2
 HM_PC_CSR = 3
 HM_PC_HOST = 4
+HM_PC_ICHOL = 5
 
 P = ctypes.c_void_p
 i32 = ctypes.c_int
@@ -53,7 +54,7 @@ class CgArgs(ctypes.Structure):
         ("pc_kind", i32), ("inv_diag", P), ("vc", P), ("pc_csr", P), ("m_cb", APPLY_CB),
         ("m_ctx", P), ("b", P), ("x0", P), ("x", P), ("atol", f64), ("rtol", f64),
         ("maxiter", i32), ("reorthogonalize", i32), ("check_every", i32),
-        ("callback", ITER_CB), ("cb_ctx", P),
+        ("callback", ITER_CB), ("cb_ctx", P), ("ichol", P),
     ]
 
 
@@ -105,6 +106,21 @@ _SIGS = {
     "hm_batch_solve": (i32, [P, P, P, f64, f64, i32, i32, i32, P, P]),
     "hm_batch_solve_reliable": (i32, [P, P, P, f64, f64, i32, P, P]),
     "hm_batch_destroy": (i32, [P]),
+    "hm_ichol_factor": (i32, [i64, P, P, P, f64, f64, i64, f64, i32, ctypes.POINTER(i64),
+                              ctypes.POINTER(P)]),
+    "hm_ichol_create": (i32, [i64, P, P, P, ctypes.POINTER(P)]),
+    "hm_ichol_nnz": (i64, [P]),
+    "hm_ichol_levels": (i32, [P, ctypes.POINTER(i32), ctypes.POINTER(i32)]),
+    "hm_ichol_copy_factor": (i32, [P, P, P, P]),
+    "hm_ichol_apply": (i32, [P, P, P, P]),
+    "hm_ichol_destroy": (i32, [P]),
+    "hm_cf_laplacian_csr": (i32, [P, i32, i32, f64, i32, P, P, P, P, P]),
+    "hm_csr_reduced_indptr": (i32, [i64, P, P, P, P, ctypes.POINTER(i64), ctypes.POINTER(i64),
+                                    P]),
+    "hm_csr_reduced_fill": (i32, [i64, P, P, P, P, P, P, P, P, P]),
+    "hm_reduced_finalize": (i32, [i64, P, P, P, P, P]),
+    "hm_csr_add_diagonal_indptr": (i32, [i64, P, P, P, P, P, ctypes.POINTER(i64), P]),
+    "hm_csr_add_diagonal_fill": (i32, [i64, P, P, P, P, P, P, P, P]),
 }
 
 _lib = None

@@ -1,27 +1,29 @@
 """Closed-form alpha estimation on the GPU.
 
 ``estimate_alpha_cf`` is the drop-in for pymatting/alpha/estimate_alpha_cf.py:8-78
-(hard constraints: L_U x_U = -R m over the unknown pixels). On the device the
-reduced system is the masked full-grid operator [L_UU 0; 0 I] with right-hand
-side -(R m) on unknown pixels and 0 on known ones, whose solution restricted
-to the unknown pixels is the reference's x_U. Its default preconditioner is
-the V-cycle on that operator (the reference's default, ichol, is a sequential
-factorisation).
+(hard constraints: L_U x_U = -R m over the unknown pixels). A preconditioner
+factory gets the reference's reduced matrix L_U, built on the device; the
+default instead solves the masked full-grid operator [L_UU 0; 0 I] with
+right-hand side -(R m) on unknown pixels and 0 on known ones, whose solution
+restricted to the unknown pixels is the reference's x_U, with the V-cycle on
+that operator (the reference's default, ichol, is available as a factory:
+its factorisation is sequential).
 
 ``solve_alpha_lambda`` is the lambda-penalised solve the benchmark workloads
 use: A = L + lambda diag(known), b = lambda is_fg, PCG with the V-cycle, clip
 (tests/test_preconditioners.py:24-45, examples/expert_example.py:14-41), in
 float64 or in the mixed float32/float64 form.
 """
-import time
+import ctypes
 
 import numpy as np
+import scipy.sparse
 import torch
 
 from pymatting_amd import _ffi
 from pymatting_amd._device import as_device, empty, is_torch
-from pymatting_amd.laplacian import CfOperator, DiaOperator
-from pymatting_amd.preconditioner import VCycle
+from pymatting_amd.laplacian import CfOperator, DiaOperator, attached_csr, cf_laplacian
+from pymatting_amd.preconditioner import VCycle, ichol, jacobi, vcycle
 from pymatting_amd.solver import cg
 from pymatting_amd.util import check_trimap, device_classes, sanity_check_image
 
@@ -36,10 +38,17 @@ def _finalize(x, trimap_dev, classes, shape):
 def estimate_alpha_cf(image, trimap, preconditioner=None, laplacian_kwargs={}, cg_kwargs={}):
     """estimate_alpha_cf.py:8-78: closed-form alpha matting (Levin et al. 2007).
 
-    preconditioner: a factory A -> M (e.g. ``jacobi``, or
-    ``lambda A: vcycle(A, A.grid_shape)``), given the matrix-free operator;
-    None selects the V-cycle, and then PCG runs on the stored stencil of the
-    same operator (its entries equal cf_laplacian's bit for bit).
+    preconditioner: a factory A -> M, called as in the reference with the
+    reduced matrix L_U = L[U, :][:, U] of the unknown pixels (a scipy CSR
+    matrix assembled on the device, carrying its device copy); its M may be
+    a matrix, a callable or one of this package's preconditioners, e.g.
+    ``ichol`` (the reference's default), whose factor is applied on the GPU.
+    None (or ``vcycle``) selects this package's default, the V-cycle on the
+    full grid: PCG then runs on the stored stencil of the masked operator
+    [L_UU 0; 0 I] (entries equal to cf_laplacian's bit for bit), whose
+    solution on the unknown pixels is the reference's x_U. ``jacobi`` runs
+    on the same masked operator. A radius other than 1 in laplacian_kwargs
+    uses the reduced system, with ichol when no factory is given.
     Returns alpha in the type of `image` (numpy or torch), shape trimap.shape.
     """
     sanity_check_image(image)
@@ -49,17 +58,75 @@ def estimate_alpha_cf(image, trimap, preconditioner=None, laplacian_kwargs={}, c
     if "is_known" in laplacian_kwargs:
         raise TypeError("cf_laplacian() got multiple values for keyword argument 'is_known'")
     classes, tdev = device_classes(trimap)
-    op = CfOperator(image, mode="masked", classes=classes, trimap=tdev, **laplacian_kwargs)
-    b = op.rhs()
-    if preconditioner is None:
-        A = DiaOperator.from_cf(op)
-        M = VCycle(A, op.grid_shape)
+    shape = tuple(trimap.shape)
+    radius = int(laplacian_kwargs.get("radius", 1))
+    full_grid = radius == 1 and (preconditioner is None or preconditioner is vcycle or
+                                 preconditioner is jacobi)
+    if full_grid:
+        op = CfOperator(image, mode="masked", classes=classes, trimap=tdev, **laplacian_kwargs)
+        b = op.rhs()
+        if preconditioner is jacobi:
+            A, M = op, jacobi(op)
+        else:
+            A = DiaOperator.from_cf(op)
+            M = VCycle(A, op.grid_shape)
+        x = cg(A, b, M=M, **cg_kwargs)
+        alpha = _finalize(x, tdev, classes, shape)
     else:
-        A = op
-        M = preconditioner(op)
-    x = cg(A, b, M=M, **cg_kwargs)
-    alpha = _finalize(x, tdev, classes, tuple(trimap.shape))
+        if preconditioner is None:
+            preconditioner = ichol
+        L_U, b_U = reduced_system(image, classes, laplacian_kwargs)
+        if L_U.shape[0] == 0:
+            x_u = empty(1)
+        else:
+            x_u = cg(L_U, b_U, M=preconditioner(L_U), **cg_kwargs)
+            x_u = as_device(x_u).reshape(-1)
+        alpha = empty(classes.numel())
+        _ffi.check(_ffi.lib().hm_reduced_finalize(classes.numel(), _ffi.ptr(classes),
+                                                  _ffi.ptr(x_u), _ffi.ptr(tdev), _ffi.ptr(alpha),
+                                                  _ffi.stream()))
+        alpha = alpha.reshape(shape)
     return alpha if is_torch(image) else alpha.cpu().numpy()
+
+
+def reduced_system(image, classes, laplacian_kwargs={}):
+    """(L_U, b_U) of estimate_alpha_cf.py:57-70, built on the device.
+
+    L = cf_laplacian(image, is_known=known) in the reference's CSR layout,
+    then L_U = L[U, :][:, U] as scipy's fancy indexing leaves it and
+    b_U = -R m with R = L[U, :][:, K], m = is_fg[K]. L_U is returned as a
+    scipy CSR matrix carrying its device copy, b_U as a device vector.
+    """
+    from pymatting_amd.laplacian import _attach
+    from pymatting_amd.sparse import DeviceCSR
+    n = classes.numel()
+    if int(laplacian_kwargs.get("radius", 1)) == 1:
+        # The reference layout assembled straight into device memory.
+        op = CfOperator(image, mode="laplacian", classes=classes, **laplacian_kwargs)
+        dev = DeviceCSR.from_device(n, *op.reference_csr_device())
+    else:
+        known = (classes != 0).cpu().numpy()
+        dev = attached_csr(cf_laplacian(image, is_known=known, **laplacian_kwargs))
+    lib = _ffi.lib()
+    ptr_u = empty(n + 1, torch.int64)
+    n_u, nnz_u = ctypes.c_int64(), ctypes.c_int64()
+    _ffi.check(lib.hm_csr_reduced_indptr(n, _ffi.ptr(dev._indptr), _ffi.ptr(dev._indices),
+                                         _ffi.ptr(classes), _ffi.ptr(ptr_u), ctypes.byref(n_u),
+                                         ctypes.byref(nnz_u), _ffi.stream()))
+    m, k = n_u.value, nnz_u.value
+    idx_u = empty(max(k, 1), torch.int64)
+    val_u = empty(max(k, 1))
+    b_u = empty(max(m, 1))
+    if m > 0:
+        _ffi.check(lib.hm_csr_reduced_fill(n, _ffi.ptr(dev._indptr), _ffi.ptr(dev._indices),
+                                           _ffi.ptr(dev._values), _ffi.ptr(classes),
+                                           _ffi.ptr(ptr_u), _ffi.ptr(idx_u), _ffi.ptr(val_u),
+                                           _ffi.ptr(b_u), _ffi.stream()))
+    ptr_u = ptr_u[:m + 1]
+    dev_u = DeviceCSR.from_device(m, ptr_u, idx_u[:k], val_u[:k])
+    host = scipy.sparse.csr_matrix((val_u[:k].cpu().numpy(), idx_u[:k].cpu().numpy(),
+                                    ptr_u.cpu().numpy()), shape=(m, m))
+    return _attach(host, dev_u), b_u[:m]
 
 
 estimate_alpha = estimate_alpha_cf

@@ -26,6 +26,8 @@ SOURCES = {
     "scan.hip": [],
     "adjoint.hip": [],
     "batch.hip": [],
+    "ichol.hip": ["-ffp-contract=off"],
+    "sparse.hip": [],
 }
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

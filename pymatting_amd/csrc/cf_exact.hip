@@ -438,6 +438,119 @@ __global__ void k_system_fill(const double* __restrict__ I, const double* __rest
 
 // ------------------------------------------------------------------ launchers
 
+// ---------------------------------------------------------- any radius r
+//
+// cf_laplacian.py:5-129 for windows of (2r+1)^2 pixels, as the reference
+// assembles them for every radius: per-window data as k_window_exact
+// computes it (sums in raster order over the window), and each entry
+// L_ij = sum over the windows holding both pixels, in raster order of the
+// centre, of delta_ij - (1 + c_i^T M c_j) / (2r+1)^2. This is the general
+// (untiled) form; radius 1 runs the tiled kernels above.
+__global__ void k_window_exact_r(const double* __restrict__ I, int h, int w, double eps, int r,
+                                 const uint8_t* __restrict__ known, double* __restrict__ wc) {
+    const long long n = (long long)h * w;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int y = (int)(t / w), x = (int)(t - (long long)y * w);
+    double* flag = wc + 9 * n;
+    if (x < r || x > w - 1 - r || y < r || y > h - 1 - r) {
+        flag[t] = 0.0;
+        return;
+    }
+    if (known) {  // cf_laplacian.py:34-35
+        bool all_known = true;
+        for (int dy = -r; dy <= r; ++dy)
+            for (int dx = -r; dx <= r; ++dx) all_known &= known[t + (long long)dy * w + dx] != 0;
+        if (all_known) {
+            flag[t] = 0.0;
+            return;
+        }
+    }
+    const int size = 2 * r + 1;
+    const double area = (double)(size * size);
+    double mean[3];
+    for (int c = 0; c < 3; ++c) {
+        double sum = 0.0;
+        for (int dy = -r; dy <= r; ++dy)
+            for (int dx = -r; dx <= r; ++dx) sum += I[c * n + t + (long long)dy * w + dx];
+        mean[c] = sum / area;
+    }
+    double a00 = eps, a01 = 0.0, a02 = 0.0, a11 = eps, a12 = 0.0, a22 = eps;
+    for (int dy = -r; dy <= r; ++dy) {
+        for (int dx = -r; dx <= r; ++dx) {
+            const long long q = t + (long long)dy * w + dx;
+            const double c0 = I[q] - mean[0], c1 = I[n + q] - mean[1], c2 = I[2 * n + q] - mean[2];
+            a00 += c0 * c0;
+            a01 += c0 * c1;
+            a02 += c0 * c2;
+            a11 += c1 * c1;
+            a12 += c1 * c2;
+            a22 += c2 * c2;
+        }
+    }
+    a00 /= area; a01 /= area; a02 /= area; a11 /= area; a12 /= area; a22 /= area;
+    const double det = a00 * a12 * a12 + a01 * a01 * a22 + a02 * a02 * a11 - a00 * a11 * a22 -
+                       2 * a01 * a02 * a12;
+    const double inv_det = 1.0 / det;
+    wc[t] = mean[0];
+    wc[n + t] = mean[1];
+    wc[2 * n + t] = mean[2];
+    wc[3 * n + t] = (a12 * a12 - a11 * a22) * inv_det;
+    wc[4 * n + t] = (a01 * a22 - a02 * a12) * inv_det;
+    wc[5 * n + t] = (a02 * a11 - a01 * a12) * inv_det;
+    wc[6 * n + t] = (a02 * a02 - a00 * a22) * inv_det;
+    wc[7 * n + t] = (a00 * a12 - a01 * a02) * inv_det;
+    wc[8 * n + t] = (a01 * a01 - a00 * a11) * inv_det;
+    flag[t] = 1.0;
+}
+
+// One thread per (pixel i, slot): the reference's CSR layout with
+// (4r+1)^2 slots per row, out-of-image slots holding column 0 and value 0.
+__global__ void k_assemble_exact_r(const double* __restrict__ I, const double* __restrict__ wc,
+                                   int h, int w, int r, double* __restrict__ values,
+                                   int64_t* __restrict__ indices, int64_t* __restrict__ indptr) {
+    const long long n = (long long)h * w;
+    const int span = 4 * r + 1, slots = span * span;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * slots) return;
+    const long long i = t / slots;
+    const int sl = (int)(t - i * slots);
+    const int yi = (int)(i / w), xi = (int)(i - (long long)yi * w);
+    const int yj = yi + sl / span - 2 * r, xj = xi + sl % span - 2 * r;
+    double value = 0.0;
+    int64_t col = 0;
+    if (yj >= 0 && yj < h && xj >= 0 && xj < w) {
+        col = (int64_t)yj * w + xj;
+        const double area = (double)((2 * r + 1) * (2 * r + 1));
+        const double Ii0 = I[i], Ii1 = I[n + i], Ii2 = I[2 * n + i];
+        const double Ij0 = I[col], Ij1 = I[n + col], Ij2 = I[2 * n + col];
+        const double delta = (i == col) ? 1.0 : 0.0;
+        const int ky0 = max(max(yi, yj) - r, r), ky1 = min(min(yi, yj) + r, h - 1 - r);
+        const int kx0 = max(max(xi, xj) - r, r), kx1 = min(min(xi, xj) + r, w - 1 - r);
+        for (int ky = ky0; ky <= ky1; ++ky) {
+            for (int kx = kx0; kx <= kx1; ++kx) {
+                const long long k = (long long)ky * w + kx;
+                if (wc[9 * n + k] == 0.0) continue;
+                const double mu0 = wc[k], mu1 = wc[n + k], mu2 = wc[2 * n + k];
+                const double m00 = wc[3 * n + k], m01 = wc[4 * n + k], m02 = wc[5 * n + k];
+                const double m11 = wc[6 * n + k], m12 = wc[7 * n + k], m22 = wc[8 * n + k];
+                const double s0 = Ii0 - mu0, s1 = Ii1 - mu1, s2 = Ii2 - mu2;
+                const double c0 = m00 * s0 + m01 * s1 + m02 * s2;
+                const double c1 = m01 * s0 + m11 * s1 + m12 * s2;
+                const double c2 = m02 * s0 + m12 * s1 + m22 * s2;
+                const double temp = c0 * (Ij0 - mu0) + c1 * (Ij1 - mu1) + c2 * (Ij2 - mu2);
+                value += delta - (1 + temp) / area;
+            }
+        }
+    }
+    values[t] = value;
+    indices[t] = col;
+    if (sl == 0) {
+        indptr[i + 1] = (i + 1) * slots;
+        if (i == 0) indptr[0] = 0;
+    }
+}
+
 int cf_exact_setup(hm_cf* op, hipStream_t s) {
     const long long n = op->n;
     const unsigned g = div_up(n, 256);
@@ -546,3 +659,42 @@ int cf_system_fill(hm_cf* op, const int64_t* indptr, int64_t* indices, double* v
 }
 
 }  // namespace hm
+
+namespace {
+
+struct DevTmp {
+    void* p = nullptr;
+    ~DevTmp() { hm::dev_free(p); }
+};
+
+}  // namespace
+
+extern "C" {
+
+int hm_cf_laplacian_csr(const double* image_hwc, int h, int w, double epsilon, int radius,
+                        const uint8_t* known, double* values, int64_t* indices, int64_t* indptr,
+                        void* stream) {
+    using namespace hm;
+    if (!image_hwc || !values || !indices || !indptr || h < 1 || w < 1 || radius < 1) {
+        set_error("hm_cf_laplacian_csr: invalid argument");
+        return HM_ERR_ARG;
+    }
+    const long long n = (long long)h * w;
+    const long long slots = (long long)(4 * radius + 1) * (4 * radius + 1);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    DevTmp I, wc;
+    HM_TRY(dev_alloc(&I.p, sizeof(double) * 3 * n));
+    HM_TRY(dev_alloc(&wc.p, sizeof(double) * 10 * n));
+    double* Ip = static_cast<double*>(I.p);
+    double* W = static_cast<double*>(wc.p);
+    k_hwc_to_planar_f64<<<div_up(n, 256), 256, 0, s>>>(image_hwc, Ip, n);
+    HM_LAUNCH_CHECK();
+    k_window_exact_r<<<div_up(n, 256), 256, 0, s>>>(Ip, h, w, epsilon, radius, known, W);
+    HM_LAUNCH_CHECK();
+    k_assemble_exact_r<<<div_up(n * slots, 256), 256, 0, s>>>(Ip, W, h, w, radius, values,
+                                                              indices, indptr);
+    HM_LAUNCH_CHECK();
+    return HM_OK;
+}
+
+}  // extern "C"

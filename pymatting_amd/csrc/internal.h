@@ -39,6 +39,8 @@ struct hm_dia {
     float* idiag32 = nullptr;
 };
 
+struct hm_ichol;  // ichol.hip
+
 struct hm_csr {
     long long n = 0, nnz = 0;
     int64_t* indptr = nullptr;
@@ -122,6 +124,11 @@ int dense_inverse(const hm_dia* A, double* inv64, float* inv32, hipStream_t s);
 int dense_inverse_from_dense(const double* A, int n, double* inv64, float* inv32, hipStream_t s);
 int csr_apply(const hm_csr* A, const double* x, double* y, CgState* st, Reducer* red, int kind,
               hipStream_t s);
+
+// ichol.hip: z = L^-T L^-1 r with the factor's triangular solves.
+int ichol_apply(const hm_ichol* h, const double* r, double* z, const CgState* guard,
+                hipStream_t s);
+long long ichol_rows(const hm_ichol* h);
 
 // Stage two of a reduction over `nblocks` partials (engine.h k_finalize).
 int launch_finalize(int nvals, const Reducer& red, unsigned nblocks, CgState* st, int kind,

@@ -221,6 +221,7 @@ struct PcHandle {
     const double* inv_diag = nullptr;
     hm_vc* vc = nullptr;
     const hm_csr* csr = nullptr;
+    const hm_ichol* ichol = nullptr;
     hm_apply_cb cb = nullptr;
     void* cb_ctx = nullptr;
 };
@@ -329,10 +330,12 @@ int pc_apply(const PcHandle& M, const T* r, T* z, CgState* st, const Reducer& re
         set_error("internal: tail side work without a V-cycle");
         return HM_ERR_ARG;
     }
-    if (M.kind == HM_PC_CSR || M.kind == HM_PC_HOST) {
+    if (M.kind == HM_PC_CSR || M.kind == HM_PC_HOST || M.kind == HM_PC_ICHOL) {
         if constexpr (std::is_same<T, double>::value) {
             if (M.kind == HM_PC_CSR)
                 HM_TRY(csr_apply(M.csr, r, z, nullptr, nullptr, 0, s));
+            else if (M.kind == HM_PC_ICHOL)
+                HM_TRY(ichol_apply(M.ichol, r, z, guard, s));
             else
                 HM_TRY(host_call(M.cb, M.cb_ctx, r, z, s));
             if (st) {
@@ -340,7 +343,7 @@ int pc_apply(const PcHandle& M, const T* r, T* z, CgState* st, const Reducer& re
             }
             return HM_OK;
         }
-        set_error("matrix and callback preconditioners are float64 only");
+        set_error("matrix, ichol and callback preconditioners are float64 only");
         return HM_ERR_UNSUPPORTED;
     }
     k_diag_pc<T><<<vec_blocks(n), kBlock, 0, s>>>(r, M.kind == HM_PC_JACOBI ? M.inv_diag : nullptr,
@@ -1419,9 +1422,9 @@ int hm_cg_solve(const hm_cg_args* a, hm_cg_stats* stats, void* stream) {
         return HM_ERR_ARG;
     }
     const int k = a->pc_kind;
-    if (k < HM_PC_NONE || k > HM_PC_HOST || (k == HM_PC_JACOBI && !a->inv_diag) ||
+    if (k < HM_PC_NONE || k > HM_PC_ICHOL || (k == HM_PC_JACOBI && !a->inv_diag) ||
         (k == HM_PC_VCYCLE && !a->vc) || (k == HM_PC_CSR && !a->pc_csr) ||
-        (k == HM_PC_HOST && !a->m_cb)) {
+        (k == HM_PC_HOST && !a->m_cb) || (k == HM_PC_ICHOL && !a->ichol)) {
         set_error("hm_cg_solve: invalid preconditioner");
         return HM_ERR_ARG;
     }
@@ -1441,11 +1444,17 @@ int hm_cg_solve(const hm_cg_args* a, hm_cg_stats* stats, void* stream) {
                   a->vc->levels[0].n, A.n());
         return HM_ERR_ARG;
     }
+    if (k == HM_PC_ICHOL && ichol_rows(a->ichol) != A.n()) {
+        set_error("dimension mismatch: ichol factor of size %lld for an operator of size %lld",
+                  ichol_rows(a->ichol), A.n());
+        return HM_ERR_ARG;
+    }
     PcHandle M;
     M.kind = k;
     M.inv_diag = a->inv_diag;
     M.vc = a->vc;
     M.csr = a->pc_csr;
+    M.ichol = a->ichol;
     M.cb = a->m_cb;
     M.cb_ctx = a->m_ctx;
     PcgOptions opt;

@@ -100,14 +100,20 @@ class CfOperator:
         _ffi.check(_ffi.lib().hm_cf_rhs(self.handle, _ffi.ptr(b), _ffi.stream()))
         return b
 
-    def reference_csr(self):
-        """The reference's 25-slot CSR layout (cf_laplacian.py:169-175)."""
+    def reference_csr_device(self):
+        """(indptr, indices, values) of the reference's 25-slot CSR layout
+        (cf_laplacian.py:169-175) as device tensors."""
         n = self.n
         values = empty(25 * n)
         indices = empty(25 * n, torch.int64)
         indptr = empty(n + 1, torch.int64)
         _ffi.check(_ffi.lib().hm_cf_assemble_csr(self.handle, _ffi.ptr(values), _ffi.ptr(indices),
                                                  _ffi.ptr(indptr), _ffi.stream()))
+        return indptr, indices, values
+
+    def reference_csr(self):
+        """The reference's 25-slot CSR layout (cf_laplacian.py:169-175)."""
+        indptr, indices, values = self.reference_csr_device()
         return (values.cpu().numpy(), indices.cpu().numpy(), indptr.cpu().numpy())
 
     def canonical_csr(self):
@@ -209,7 +215,8 @@ class DiaOperator:
 
 
 class CfLaplacianMatrix(scipy.sparse.csr_matrix):
-    """scipy CSR matrix carrying the device operator it was assembled from.
+    """scipy CSR matrix carrying the device form it was assembled from (a
+    stencil operator, or a device copy of the CSR itself).
 
     The operator stands for the matrix only while the matrix is unchanged:
     a fingerprint of its arrays (their addresses and sizes, and a hash of the
@@ -243,21 +250,50 @@ def _attach(csr, operator):
 def cf_laplacian(image, epsilon=1e-7, radius=1, is_known=None):
     """cf_laplacian.py:132-177: the closed-form matting Laplacian as CSR.
 
-    Entries and layout are the reference's: (4r+1)^2 = 25 slots per row,
-    out-of-image slots with column 0 and value 0. Only radius=1 is implemented
-    on the GPU.
+    Entries and layout are the reference's: (4r+1)^2 slots per row (25 for
+    the default radius 1), out-of-image slots with column 0 and value 0.
+    Radius 1 carries the matrix-free device operator (``.operator``); other
+    radii are assembled by the general-radius kernel and carry their device
+    CSR, which ``cg`` and ``make_linear_system`` use without another upload.
     """
     if not is_torch(image) and np.asarray(image).ndim == 3:
         assert np.asarray(image).shape[2] == 3  # cf_laplacian.py:8
     h, w = int(image.shape[0]), int(image.shape[1])
-    classes = None
+    n = h * w
+    known = None
     if is_known is not None:
-        known = np.asarray(is_known.cpu() if is_torch(is_known) else is_known).reshape(h * w)
-        classes = as_device(known.astype(np.uint8) * 2, torch.uint8)
+        known = np.asarray(is_known.cpu() if is_torch(is_known) else is_known).reshape(n)
+    if int(radius) != 1:
+        return _cf_laplacian_radius(image, float(epsilon), int(radius), known)
+    classes = None if known is None else as_device(known.astype(np.uint8) * 2, torch.uint8)
     op = CfOperator(image, epsilon, radius, "laplacian", classes)
     values, indices, indptr = op.reference_csr()
-    n = h * w
     return _attach(scipy.sparse.csr_matrix((values, indices, indptr), (n, n)), op)
+
+
+def _cf_laplacian_radius(image, epsilon, radius, known):
+    from pymatting_amd.sparse import DeviceCSR
+    if not is_torch(image):
+        image = np.asarray(image)
+        if image.dtype != np.float64:
+            raise TypeError("image must be float64, got %s" % image.dtype)
+    if radius < 1:
+        raise ValueError("radius must be at least 1, got %d" % radius)
+    h, w = int(image.shape[0]), int(image.shape[1])
+    n = h * w
+    slots = (4 * radius + 1) ** 2
+    img = as_device(image, torch.float64).reshape(h, w, 3).contiguous()
+    kd = None if known is None else as_device(known.astype(np.uint8), torch.uint8)
+    values = empty(n * slots)
+    indices = empty(n * slots, torch.int64)
+    indptr = empty(n + 1, torch.int64)
+    _ffi.check(_ffi.lib().hm_cf_laplacian_csr(_ffi.ptr(img), h, w, epsilon, radius, _ffi.ptr(kd),
+                                              _ffi.ptr(values), _ffi.ptr(indices),
+                                              _ffi.ptr(indptr), _ffi.stream()))
+    dev = DeviceCSR.from_device(n, indptr, indices, values)
+    host = scipy.sparse.csr_matrix((values.cpu().numpy(), indices.cpu().numpy(),
+                                    indptr.cpu().numpy()), (n, n))
+    return _attach(host, dev)
 
 
 def make_linear_system(L, trimap, lambda_value=100.0, return_c=False):
@@ -270,12 +306,21 @@ def make_linear_system(L, trimap, lambda_value=100.0, return_c=False):
     is_fg, is_bg, is_known, is_unknown = trimap_split(trimap)
     c = lambda_value * is_known
     b = lambda_value * is_fg
-    base = L if isinstance(L, CfOperator) else getattr(L, "operator", None)
-    if not isinstance(base, CfOperator):
-        raise TypeError("make_linear_system needs the Laplacian returned by "
-                        "pymatting_amd.cf_laplacian (a device operator)")
-    if base.classes is not None:
-        raise NotImplementedError("make_linear_system of a Laplacian built with is_known")
+    base = device_operator(L)
+    if not isinstance(base, CfOperator) or base.classes is not None:
+        # Any other Laplacian (another radius, an edited or user-built
+        # matrix, one built with is_known): the canonical CSR of L + diag(c)
+        # formed on the device.
+        from pymatting_amd.sparse import csr_plus_diagonal
+        if isinstance(L, (CfOperator, DiaOperator)):
+            raise TypeError("make_linear_system needs a matrix or the operator returned by "
+                            "cf_laplacian")
+        dev, A = csr_plus_diagonal(L, c)
+        A = _attach(A, dev)
+        A.has_canonical_format = True
+        if return_c:
+            return A, b, c
+        return A, b
     classes, tdev = device_classes(trimap)
     op = base.derived("lambda", classes, tdev, lambda_value)
     if isinstance(L, CfOperator):
@@ -290,11 +335,21 @@ def make_linear_system(L, trimap, lambda_value=100.0, return_c=False):
 
 
 def device_operator(A):
-    """The device operator behind A (CfOperator/DiaOperator), or None: a
+    """The stencil operator behind A (CfOperator/DiaOperator), or None: a
     matrix from cf_laplacian/make_linear_system yields its operator unless it
     has been edited since."""
     if isinstance(A, (CfOperator, DiaOperator)):
         return A
-    if isinstance(A, CfLaplacianMatrix):
+    if isinstance(A, CfLaplacianMatrix) and isinstance(A.operator, (CfOperator, DiaOperator)):
+        return A.operator if A.matches_operator() else None
+    return None
+
+
+def attached_csr(A):
+    """The device CSR copy a matrix from this package carries (another radius
+    of cf_laplacian, make_linear_system of a general Laplacian, the reduced
+    system of estimate_alpha_cf), or None (also once the matrix is edited)."""
+    from pymatting_amd.sparse import DeviceCSR
+    if isinstance(A, CfLaplacianMatrix) and isinstance(A.operator, DeviceCSR):
         return A.operator if A.matches_operator() else None
     return None

@@ -1,10 +1,12 @@
 """Preconditioners on the GPU.
 
-Drop-ins for pymatting/preconditioner/jacobi.py:1-31 (jacobi) and
-pymatting/preconditioner/vcycle.py:103-153 (vcycle). Both return callables
-(numpy or torch vectors in, the same type out) that ``cg`` recognises and
-runs inside its device loop. pymatting's ``ichol`` (a sequential incomplete
-Cholesky) has no GPU implementation here; ``vcycle`` is the GPU default.
+Drop-ins for pymatting/preconditioner/jacobi.py:1-31 (jacobi),
+pymatting/preconditioner/vcycle.py:103-153 (vcycle) and
+pymatting/preconditioner/ichol.py:148-301 (ichol, CholeskyDecomposition).
+All return callables (numpy or torch vectors in, the same type out) that
+``cg`` recognises and runs inside its device loop. ichol's factorisation is
+sequential and runs in the library on the host; its two triangular solves
+run on the GPU.
 """
 import ctypes
 
@@ -151,7 +153,117 @@ def vcycle(A, shape, num_pre_iter=1, num_post_iter=1, omega=0.8, direct_solve_si
     return M
 
 
-def ichol(A, *args, **kwargs):
-    """pymatting's thresholded incomplete Cholesky is sequential and has no GPU form here."""
-    raise NotImplementedError("ichol has no GPU implementation in pymatting_amd; "
-                              "use vcycle (the default of estimate_alpha_cf) or jacobi")
+class CholeskyDecomposition:
+    """ichol.py:148-181: an incomplete Cholesky factor L (CSC) applied as
+    x = L^-T L^-1 b by forward and back substitution, on the GPU.
+
+    ``Ltuple`` is (Lv, Lr, Lp), the factor's values, row indices and column
+    pointers (each column starts with its diagonal). Calling the object with
+    a numpy vector returns a numpy vector; a torch tensor stays on the device.
+    ``cg`` runs the two solves inside its device loop.
+    """
+
+    def __init__(self, Ltuple, _handle=None):
+        Lv, Lr, Lp = Ltuple
+        self.Ltuple = (np.ascontiguousarray(Lv, dtype=np.float64),
+                       np.ascontiguousarray(Lr, dtype=np.int64),
+                       np.ascontiguousarray(Lp, dtype=np.int64))
+        self.n = len(self.Ltuple[2]) - 1
+        if _handle is None:
+            Lv, Lr, Lp = self.Ltuple
+            _handle = ctypes.c_void_p()
+            _ffi.check(_ffi.lib().hm_ichol_create(self.n, _ptr(Lv), _ptr(Lr), _ptr(Lp),
+                                                  ctypes.byref(_handle)))
+        self.handle = _handle
+
+    def __del__(self):
+        _ffi.release(self, "hm_ichol_destroy")
+
+    @property
+    def L(self):
+        """The factor as a scipy.sparse.csc_matrix."""
+        return scipy.sparse.csc_matrix(self.Ltuple, (self.n, self.n))
+
+    def levels(self):
+        """(forward, backward): the level counts of the two triangular solves."""
+        f, b = ctypes.c_int(), ctypes.c_int()
+        _ffi.check(_ffi.lib().hm_ichol_levels(self.handle, ctypes.byref(f), ctypes.byref(b)))
+        return f.value, b.value
+
+    def __call__(self, b):
+        if _ffi.numel(b) != self.n:
+            raise ValueError("dimension mismatch: factor of size %d, vector of size %d"
+                             % (self.n, _ffi.numel(b)))
+        bd = as_device(b).reshape(-1)
+        out = empty(self.n)
+        _ffi.check(_ffi.lib().hm_ichol_apply(self.handle, _ffi.ptr(bd), _ffi.ptr(out),
+                                             _ffi.stream()))
+        return output_like(out, b)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def ichol(A, discard_threshold=1e-4,
+          shifts=[0.0, 1e-4, 1e-3, 1e-2, 0.1, 0.5, 1.0, 10.0, 100, 1e3, 1e4, 1e5],
+          max_nnz=int(4e9 / 16), relative_discard_threshold=0.0, diag_keep_discarded=True):
+    """ichol.py:184-301: thresholded incomplete Cholesky decomposition.
+
+    A CSR matrix is transposed to CSC (A is symmetric), a non-canonical one
+    has its duplicates summed in place, and the factorisation is retried with
+    each diagonal shift in turn until every pivot is positive (printing the
+    reference's PERFORMANCE WARNING for each failed shift). The factorisation
+    runs in the library on the host, like the reference's; the returned
+    CholeskyDecomposition applies it on the GPU.
+    """
+    if isinstance(A, scipy.sparse.csr_matrix):
+        A = A.T
+    if not isinstance(A, scipy.sparse.csc_matrix):
+        raise ValueError("Matrix A must be a scipy.sparse.csc_matrix")
+    if not A.has_canonical_format:
+        # On a copy: the transpose of a CSR shares its arrays, and summing
+        # duplicates in place would leave the caller's CSR inconsistent.
+        A = A.copy()
+        A.sum_duplicates()
+    m, n = A.shape
+    assert m == n
+    Av = np.ascontiguousarray(A.data, dtype=np.float64)
+    Ar = np.ascontiguousarray(A.indices, dtype=np.int64)
+    Ap = np.ascontiguousarray(A.indptr, dtype=np.int64)
+    L = _ffi.lib()
+    nnz = -1
+    handle = None
+    for shift in shifts:
+        result = ctypes.c_int64()
+        h = ctypes.c_void_p()
+        _ffi.check(L.hm_ichol_factor(n, _ptr(Ap), _ptr(Ar), _ptr(Av), float(discard_threshold),
+                                     float(shift), int(max_nnz),
+                                     float(relative_discard_threshold),
+                                     int(bool(diag_keep_discarded)), ctypes.byref(result),
+                                     ctypes.byref(h)))
+        nnz = result.value
+        if nnz >= 0:
+            handle = h
+            break
+        if nnz == -1:
+            print("PERFORMANCE WARNING:")
+            print("Thresholded incomplete Cholesky decomposition failed due to insufficient "
+                  "positive-definiteness of matrix A with parameters:")
+            print("    discard_threshold = %e" % discard_threshold)
+            print("    shift = %e" % shift)
+            print("Try decreasing discard_threshold or start with a larger shift")
+            print("")
+        if nnz == -2:
+            raise ValueError("Thresholded incomplete Cholesky decomposition failed because more "
+                             "than max_nnz non-zero elements were created. Try increasing "
+                             "max_nnz or discard_threshold.")
+    if nnz < 0:
+        raise ValueError("Thresholded incomplete Cholesky decomposition failed due to "
+                         "insufficient positive-definiteness of matrix A and diagonal shifts "
+                         "did not help.")
+    Lv = np.empty(nnz, dtype=np.float64)
+    Lr = np.empty(nnz, dtype=np.int64)
+    Lp = np.empty(n + 1, dtype=np.int64)
+    _ffi.check(L.hm_ichol_copy_factor(handle, _ptr(Lv), _ptr(Lr), _ptr(Lp)))
+    return CholeskyDecomposition((Lv, Lr, Lp), _handle=handle)

@@ -15,8 +15,9 @@ import torch
 
 from pymatting_amd import _ffi
 from pymatting_amd._device import as_device, empty, is_torch, output_like
-from pymatting_amd.laplacian import CfOperator, DiaOperator, device_operator
-from pymatting_amd.preconditioner import Jacobi, VCycle
+from pymatting_amd.laplacian import CfOperator, DiaOperator, attached_csr, device_operator
+from pymatting_amd.sparse import DeviceCSR
+from pymatting_amd.preconditioner import CholeskyDecomposition, Jacobi, VCycle
 
 
 class CounterCallback:
@@ -38,27 +39,6 @@ class ProgressCallback:
     def __call__(self, A, x, b, norm_b, r, norm_r):
         self.n += 1
         print("iteration %7d - %e (%.20f)" % (self.n, norm_r, norm_r))
-
-
-class DeviceCSR:
-    """A general sparse (or dense) matrix uploaded as float64 CSR."""
-
-    def __init__(self, A):
-        A = scipy.sparse.csr_matrix(A, dtype=np.float64)
-        self.n = A.shape[0]
-        if A.shape[0] != A.shape[1]:
-            raise ValueError("matrix must be square, got %s" % (A.shape,))
-        self._indptr = as_device(A.indptr.astype(np.int64), torch.int64)
-        self._indices = as_device(A.indices.astype(np.int64), torch.int64)
-        self._values = as_device(A.data)
-        handle = ctypes.c_void_p()
-        _ffi.check(_ffi.lib().hm_csr_create(self.n, A.nnz, _ffi.ptr(self._indptr),
-                                            _ffi.ptr(self._indices), _ffi.ptr(self._values),
-                                            ctypes.byref(handle)))
-        self.handle = handle
-
-    def __del__(self):
-        _ffi.release(self, "hm_csr_destroy")
 
 
 def _to_host(ptr, n):
@@ -118,7 +98,7 @@ def _check_sizes(A, op, M, x0, n):
             mismatch("matrix of shape %s" % (tuple(A.shape),))
     if x0 is not None and _ffi.numel(x0) != n:
         mismatch("x0 of size %d" % _ffi.numel(x0))
-    if isinstance(M, (Jacobi, VCycle)):
+    if isinstance(M, (Jacobi, VCycle, CholeskyDecomposition)):
         if M.n != n:
             mismatch("preconditioner of size %d" % M.n)
     elif scipy.sparse.issparse(M) or isinstance(M, np.ndarray):
@@ -150,6 +130,8 @@ def cg(A, b, x0=None, atol=0.0, rtol=1e-7, maxiter=10000, callback=None, M=None,
         args.cf = op.handle
     elif isinstance(op, DiaOperator):
         args.dia = op.handle
+    elif attached_csr(A) is not None:
+        args.csr = attached_csr(A).handle
     elif scipy.sparse.issparse(A) or isinstance(A, np.ndarray):
         csr = DeviceCSR(A)
         keep.append(csr)
@@ -172,6 +154,12 @@ def cg(A, b, x0=None, atol=0.0, rtol=1e-7, maxiter=10000, callback=None, M=None,
             raise ValueError("cg needs a float64 V-cycle")
         args.pc_kind = _ffi.HM_PC_VCYCLE
         args.vc = M.handle
+    elif isinstance(M, CholeskyDecomposition):
+        args.pc_kind = _ffi.HM_PC_ICHOL
+        args.ichol = M.handle
+    elif attached_csr(M) is not None:
+        args.pc_kind = _ffi.HM_PC_CSR
+        args.pc_csr = attached_csr(M).handle
     elif scipy.sparse.issparse(M) or isinstance(M, np.ndarray):
         pc = DeviceCSR(M)
         keep.append(pc)
