@@ -48,6 +48,7 @@ typedef struct hm_dia hm_dia; /* stored 25-point grid stencil (coarse levels, EL
 typedef struct hm_csr hm_csr; /* general CSR matrix (cg on arbitrary matrices) */
 typedef struct hm_vc hm_vc;   /* V-cycle hierarchy */
 typedef struct hm_ichol hm_ichol; /* incomplete Cholesky factor with its device solves */
+typedef struct hm_lkm hm_lkm;     /* large-kernel (box filter) matting Laplacian */
 
 typedef struct hm_cg_stats {
     int iterations;     /* completed CG iterations */
@@ -105,6 +106,23 @@ int hm_cf_system_fill(hm_cf* op, const int64_t* indptr, int64_t* indices, double
 int hm_cf_laplacian_csr(const double* image_hwc, int h, int w, double epsilon, int radius,
                         const uint8_t* known, double* values, int64_t* indices, int64_t* indptr,
                         void* stream);
+
+/* ---- lkm_laplacian: pymatting/laplacian/lkm_laplacian.py:6-76 ----
+ * The closed-form Laplacian of (2r+1)^2 windows applied with box filters
+ * (He et al. 2010), matrix-free. image_hwc: device float64 (h, w, 3), h and
+ * w at least 2r + 1 (boxfilter.py:140-141). cls (nullable, device class
+ * map, nonzero = known) makes the operator L + lambda diag(known), the
+ * system of estimate_alpha_lkm.py:51-58. hm_lkm_apply: y = A x (the
+ * reference's L_matvec, :51-65); hm_lkm_diagonal: diag(A) (:67-74), or its
+ * inverse (the Jacobi preconditioner of :55-61) when inverse != 0;
+ * hm_lkm_rhs: b = lambda * is_fg. They use the handle's work buffers:
+ * calls on one handle must not overlap. */
+int hm_lkm_create(const double* image_hwc, int h, int w, double epsilon, int radius,
+                  const uint8_t* cls, double lambda_value, void* stream, hm_lkm** out);
+int hm_lkm_apply(hm_lkm* L, const double* x, double* y, void* stream);
+int hm_lkm_diagonal(hm_lkm* L, int inverse, double* d, void* stream);
+int hm_lkm_rhs(hm_lkm* L, double* b, void* stream);
+int hm_lkm_destroy(hm_lkm* L);
 
 /* ---- stored 25-point operators (ELL/DIA, slot s = (dy+2)*5 + (dx+2)) ----
  * coef: device float64 [25][h*w] slot-major. The operator must be symmetric
@@ -196,7 +214,7 @@ typedef int (*hm_iter_cb)(void* ctx, const double* x, const double* r, double no
                           double norm_r);
 
 typedef struct hm_cg_args {
-    /* operator: exactly one of cf, dia, csr, a_cb (then n gives the size) */
+    /* operator: exactly one of cf, dia, csr, lkm, a_cb (then n gives the size) */
     hm_cf* cf;
     hm_dia* dia;
     hm_csr* csr;
@@ -222,6 +240,7 @@ typedef struct hm_cg_args {
     hm_iter_cb callback;      /* nullable */
     void* cb_ctx;
     hm_ichol* ichol;          /* HM_PC_ICHOL */
+    hm_lkm* lkm;              /* operator: large-kernel Laplacian */
 } hm_cg_args;
 
 /* Preconditioned CG. Returns HM_ERR_NOCONV after maxiter iterations without
@@ -335,6 +354,29 @@ int hm_fb_ml(const void* image, const void* alpha, int h0, int w0, int n_levels,
              const int* level_h, const int* level_w, double regularization, int n_small,
              int n_big, int small_size, double gradient_weight, int prec, void* F, void* B,
              void* stream);
+
+/* ---- estimate_foreground_cf: pymatting/foreground/estimate_foreground_cf.py:8-130 ----
+ * The 2n x 2n system A = U^T U + [[S, 0], [0, S]] of the closed-form
+ * foreground (U = [diag(a), diag(1 - a)], S = sum_k D_k^T diag(reg + |D_k a|)
+ * D_k over the neighbour offsets (dx[k], dy[k]), k < n_dirs <= 8, with
+ * clamped neighbours, :69-90) as the canonical CSR scipy leaves (columns
+ * ascending, zeros dropped), entries bit-identical to the reference's.
+ * alpha: device float64 (h, w). hm_fgcf_system_indptr fills indptr (2n + 1)
+ * and returns nnz; hm_fgcf_system_fill writes the indices and values.
+ * hm_fgcf_channel forms b = [a I_c; (1 - a) I_c] and x0 = [F0_c; B0_c] for
+ * channel c of (h, w, channels) float64 images (:100-112); hm_fgcf_store
+ * writes clip(x, 0, 1) back into channel c of F and B (:114-121). */
+int hm_fgcf_system_indptr(const double* alpha, int h, int w, double regularization,
+                          int n_dirs, const int* dx, const int* dy, int64_t* indptr,
+                          int64_t* nnz, void* stream);
+int hm_fgcf_system_fill(const double* alpha, int h, int w, double regularization, int n_dirs,
+                        const int* dx, const int* dy, const int64_t* indptr, int64_t* indices,
+                        double* values, void* stream);
+int hm_fgcf_channel(const double* image, const double* alpha, const double* F0,
+                    const double* B0, int64_t n, int channels, int channel, double* b, double* x0,
+                    void* stream);
+int hm_fgcf_store(const double* x, int64_t n, int channels, int channel, double* F, double* B,
+                  void* stream);
 
 /* ---- allocation-free foreground (no counterpart in the reference) ----
  * hm_fb_ml without any device allocation, for callers that cannot allocate

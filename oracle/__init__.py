@@ -23,6 +23,8 @@ Modules and the reference code they restate:
 - ``oracle.foreground``: ``estimate_foreground_ml``
   (pymatting/foreground/estimate_foreground_ml.py:5-253), C restatement in
   ``fb_ml.c``.
+- ``oracle.foreground_cf``: ``estimate_foreground_cf``
+  (pymatting/foreground/estimate_foreground_cf.py:8-130).
 
 How each function is pinned (tests/test_oracle_*.py): known-answer values
 from the reference's own tests and docstrings, brute-force dense assembly on

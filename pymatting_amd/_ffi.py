@@ -54,7 +54,7 @@ class CgArgs(ctypes.Structure):
         ("pc_kind", i32), ("inv_diag", P), ("vc", P), ("pc_csr", P), ("m_cb", APPLY_CB),
         ("m_ctx", P), ("b", P), ("x0", P), ("x", P), ("atol", f64), ("rtol", f64),
         ("maxiter", i32), ("reorthogonalize", i32), ("check_every", i32),
-        ("callback", ITER_CB), ("cb_ctx", P), ("ichol", P),
+        ("callback", ITER_CB), ("cb_ctx", P), ("ichol", P), ("lkm", P),
     ]
 
 
@@ -121,6 +121,15 @@ _SIGS = {
     "hm_reduced_finalize": (i32, [i64, P, P, P, P, P]),
     "hm_csr_add_diagonal_indptr": (i32, [i64, P, P, P, P, P, ctypes.POINTER(i64), P]),
     "hm_csr_add_diagonal_fill": (i32, [i64, P, P, P, P, P, P, P, P]),
+    "hm_fgcf_system_indptr": (i32, [P, i32, i32, f64, i32, P, P, P, ctypes.POINTER(i64), P]),
+    "hm_fgcf_system_fill": (i32, [P, i32, i32, f64, i32, P, P, P, P, P, P]),
+    "hm_fgcf_channel": (i32, [P, P, P, P, i64, i32, i32, P, P, P]),
+    "hm_fgcf_store": (i32, [P, i64, i32, i32, P, P, P]),
+    "hm_lkm_create": (i32, [P, i32, i32, f64, i32, P, f64, P, ctypes.POINTER(P)]),
+    "hm_lkm_apply": (i32, [P, P, P, P]),
+    "hm_lkm_diagonal": (i32, [P, i32, P, P]),
+    "hm_lkm_rhs": (i32, [P, P, P]),
+    "hm_lkm_destroy": (i32, [P]),
 }
 
 _lib = None

@@ -28,6 +28,8 @@ SOURCES = {
     "batch.hip": [],
     "ichol.hip": ["-ffp-contract=off"],
     "sparse.hip": [],
+    "fg_cf.hip": ["-ffp-contract=off"],
+    "lkm.hip": [],
 }
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -40,6 +40,7 @@ struct hm_dia {
 };
 
 struct hm_ichol;  // ichol.hip
+struct hm_lkm;    // lkm.hip
 
 struct hm_csr {
     long long n = 0, nnz = 0;
@@ -129,6 +130,9 @@ int csr_apply(const hm_csr* A, const double* x, double* y, CgState* st, Reducer*
 int ichol_apply(const hm_ichol* h, const double* r, double* z, const CgState* guard,
                 hipStream_t s);
 long long ichol_rows(const hm_ichol* h);
+// lkm.hip: y = A x with the large-kernel operator; its size.
+int lkm_apply(const hm_lkm* L, const double* x, double* y, hipStream_t s);
+long long lkm_rows(const hm_lkm* L);
 
 // Stage two of a reduction over `nblocks` partials (engine.h k_finalize).
 int launch_finalize(int nvals, const Reducer& red, unsigned nblocks, CgState* st, int kind,

@@ -208,10 +208,13 @@ struct OpHandle {
     const hm_cf* cf = nullptr;
     const hm_dia* dia = nullptr;
     const hm_csr* csr = nullptr;
+    const hm_lkm* lkm = nullptr;  // large-kernel operator (float64 only)
     hm_apply_cb cb = nullptr;  // host callback operator (float64 only)
     void* cb_ctx = nullptr;
     long long cb_n = 0;
-    long long n() const { return cf ? cf->n : (dia ? dia->n : (csr ? csr->n : cb_n)); }
+    long long n() const {
+        return cf ? cf->n : (dia ? dia->n : (csr ? csr->n : (lkm ? lkm_rows(lkm) : cb_n)));
+    }
     int h() const { return cf ? cf->h : (dia ? dia->h : 1); }
     int w() const { return cf ? cf->w : (dia ? dia->w : (int)n()); }
 };
@@ -266,7 +269,10 @@ int op_apply(const OpHandle& A, const T* x, T* y, const DotSink& ds, const CgSta
             Reducer red = ds.red;
             return csr_apply(A.csr, x, y, ds.st, ds.st ? &red : nullptr, ds.kind, s);
         }
-        HM_TRY(host_call(A.cb, A.cb_ctx, x, y, s));
+        if (A.lkm)
+            HM_TRY(lkm_apply(A.lkm, x, y, s));
+        else
+            HM_TRY(host_call(A.cb, A.cb_ctx, x, y, s));
         if (ds.st) {
             const long long n = A.n();
             HM_TRY(launch_dot<double>(x, y, nullptr, n, ds.st, ds.red, ds.kind, s));
@@ -1416,7 +1422,7 @@ int hm_cg_solve(const hm_cg_args* a, hm_cg_stats* stats, void* stream) {
         return HM_ERR_ARG;
     }
     const int nops = (a->cf != nullptr) + (a->dia != nullptr) + (a->csr != nullptr) +
-                     (a->a_cb != nullptr);
+                     (a->lkm != nullptr) + (a->a_cb != nullptr);
     if (nops != 1 || !a->b || !a->x || (a->a_cb && a->n < 0)) {
         set_error("hm_cg_solve: needs exactly one operator and b, x");
         return HM_ERR_ARG;
@@ -1436,6 +1442,7 @@ int hm_cg_solve(const hm_cg_args* a, hm_cg_stats* stats, void* stream) {
     A.cf = a->cf;
     A.dia = a->dia;
     A.csr = a->csr;
+    A.lkm = a->lkm;
     A.cb = a->a_cb;
     A.cb_ctx = a->a_ctx;
     A.cb_n = a->n;

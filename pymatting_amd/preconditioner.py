@@ -37,6 +37,14 @@ class Jacobi:
         self.inv_diag = 1.0 / _diagonal_device(A)
         self.n = self.inv_diag.numel()
 
+    @classmethod
+    def from_inverse_diagonal(cls, inv_diag):
+        """x -> x * inv_diag for a given device vector of inverse diagonal entries."""
+        self = cls.__new__(cls)
+        self.inv_diag = inv_diag.reshape(-1)
+        self.n = self.inv_diag.numel()
+        return self
+
     def __call__(self, x):
         if _ffi.numel(x) != self.n:
             raise ValueError("dimension mismatch: Jacobi preconditioner of size %d, vector of "

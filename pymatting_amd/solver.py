@@ -16,6 +16,7 @@ import torch
 from pymatting_amd import _ffi
 from pymatting_amd._device import as_device, empty, is_torch, output_like
 from pymatting_amd.laplacian import CfOperator, DiaOperator, attached_csr, device_operator
+from pymatting_amd.lkm import LkmOperator
 from pymatting_amd.sparse import DeviceCSR
 from pymatting_amd.preconditioner import CholeskyDecomposition, Jacobi, VCycle
 
@@ -90,9 +91,10 @@ def _check_sizes(A, op, M, x0, n):
     """ValueError (dimension mismatch) unless A, M and x0 all match len(b)."""
     def mismatch(what):
         raise ValueError("dimension mismatch: %s, but b has size %d" % (what, n))
-    if op is not None:
-        if op.n != n:
-            mismatch("operator of size %d" % op.n)
+    if op is not None or isinstance(A, LkmOperator):
+        m = (op if op is not None else A).n
+        if m != n:
+            mismatch("operator of size %d" % m)
     elif scipy.sparse.issparse(A) or isinstance(A, np.ndarray):
         if A.ndim != 2 or tuple(A.shape) != (n, n):
             mismatch("matrix of shape %s" % (tuple(A.shape),))
@@ -130,6 +132,8 @@ def cg(A, b, x0=None, atol=0.0, rtol=1e-7, maxiter=10000, callback=None, M=None,
         args.cf = op.handle
     elif isinstance(op, DiaOperator):
         args.dia = op.handle
+    elif isinstance(A, LkmOperator):
+        args.lkm = A.handle
     elif attached_csr(A) is not None:
         args.csr = attached_csr(A).handle
     elif scipy.sparse.issparse(A) or isinstance(A, np.ndarray):
