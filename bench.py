@@ -35,7 +35,9 @@ The line also carries the other configurations' figures, measured in the
 same run: "c3" (64 images batched vs one at a time), "c4" (3840x2160 with a
 5-level V(2,2) cycle, and the float32 foreground against float64) and "c5"
 (gradient of a 16x512x512 batch against finite differences and the float64
-adjoint). --c2-only leaves them out.
+adjoint), and "other_rows_320x240" (ichol, the foreground methods, large-kernel
+matting, radius 2 and the cutout pipeline on the 320x240 image, each beside
+its single-threaded CPU oracle). --c2-only leaves them out.
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--c2-only]
        python bench.py --pmc-probe   (SpMV launches and a 256 MiB copy only,
@@ -498,6 +500,106 @@ def bench_c5(args):
     return line
 
 
+def bench_rows(args):
+    """The rows beyond the alpha solve, on the 320x240 configuration image:
+    estimate_alpha_cf with the reference's default ichol (and this package's
+    V-cycle default), estimate_foreground_ml, estimate_foreground_cf,
+    estimate_alpha_lkm (radius 10), cf_laplacian at radius 2 and the cutout
+    file pipeline. Each is timed on the GPU (numpy in, numpy out, as the
+    reference's callers hand them over) and compared with the oracle's
+    result, which is timed single-threaded on the host in a subprocess."""
+    import tempfile
+    import numpy as np
+    import torch
+    import pymatting_amd as pm
+    from pymatting_amd import synthetic
+
+    image, trimap, a_true = synthetic.config1()
+    h, w = trimap.shape
+    tmp = tempfile.mkdtemp()
+    img_png, tri_png = os.path.join(tmp, "image.png"), os.path.join(tmp, "trimap.png")
+    pm.save_image(img_png, image)
+    pm.save_image(tri_png, trimap)
+    img8, tri8 = pm.load_image(img_png, "RGB"), pm.load_image(tri_png, "GRAY")
+    rows = {
+        "estimate_alpha_cf_ichol": lambda: pm.estimate_alpha_cf(image, trimap,
+                                                                preconditioner=pm.ichol),
+        "estimate_alpha_cf_vcycle": lambda: pm.estimate_alpha_cf(image, trimap),
+        "estimate_foreground_ml": lambda: pm.estimate_foreground_ml(image, a_true),
+        "estimate_foreground_cf": lambda: pm.estimate_foreground_cf(image, a_true),
+        "estimate_alpha_lkm": lambda: pm.estimate_alpha_lkm(image, trimap),
+        "cf_laplacian_radius2": lambda: pm.cf_laplacian(image, radius=2).data,
+        "cutout": lambda: (pm.cutout(img_png, tri_png, os.path.join(tmp, "cut.png")),
+                           pm.load_image(os.path.join(tmp, "cut.png")))[1],
+    }
+    out, res = {}, {}
+    for name, fn in rows.items():
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            r = fn()
+        torch.cuda.synchronize()
+        out[name] = np.asarray(r)
+        res[name] = {"ms": round((time.perf_counter() - t0) / args.steps * 1e3, 3),
+                     "Mpix_per_s": round(h * w / ((time.perf_counter() - t0) / args.steps) / 1e6,
+                                         3)}
+    # The oracle on the host, one thread, same inputs.
+    opath = os.path.join(tmp, "oracle.npz")
+    code = (
+        "import sys, time, json, os; sys.path.insert(0, %r)\n"
+        "import numpy as np\n"
+        "import oracle.cf as ocf, oracle.lkm as olkm\n"
+        "from oracle.foreground import estimate_foreground_ml as ofg\n"
+        "from oracle.foreground_cf import estimate_foreground_cf as ofcf\n"
+        "from oracle.ichol import ichol\n"
+        "from oracle.solver import vcycle\n"
+        "from PIL import Image\n"
+        "from pymatting_amd import synthetic\n"
+        "image, trimap, a_true = synthetic.config1()\n"
+        "tmp = %r\n"
+        "def cut():\n"
+        "    im = np.array(Image.open(os.path.join(tmp, 'image.png')).convert('RGB')) / 255.0\n"
+        "    tr = np.array(Image.open(os.path.join(tmp, 'trimap.png')).convert('L')) / 255.0\n"
+        "    a = ocf.estimate_alpha_cf(im, tr)\n"
+        "    F = ofg(im, a)\n"
+        "    c = np.concatenate([F, a[:, :, None]], axis=2)\n"
+        "    return np.clip(c * 255, 0, 255).astype(np.uint8) / 255.0\n"
+        "rows = {'estimate_alpha_cf_ichol': lambda: ocf.estimate_alpha_cf(image, trimap),\n"
+        "        'estimate_alpha_cf_vcycle': lambda: ocf.estimate_alpha_cf(image, trimap),\n"
+        "        'estimate_foreground_ml': lambda: ofg(image, a_true),\n"
+        "        'estimate_foreground_cf': lambda: ofcf(image, a_true),\n"
+        "        'estimate_alpha_lkm': lambda: olkm.estimate_alpha_lkm(image, trimap),\n"
+        "        'cf_laplacian_radius2': lambda: ocf.cf_laplacian(image, radius=2).data,\n"
+        "        'cutout': cut}\n"
+        "times, outs = {}, {}\n"
+        "for k, f in rows.items():\n"
+        "    t = time.perf_counter(); outs[k] = np.asarray(f()); times[k] = time.perf_counter() - t\n"
+        "np.savez(%r, **outs)\n"
+        "print(json.dumps(times))\n" % (ROOT, tmp, opath))
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1",
+               HIP_VISIBLE_DEVICES="", CUDA_VISIBLE_DEVICES="")
+    if not args.no_cpu:
+        p = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env,
+                           timeout=900)
+        if p.returncode == 0:
+            times = json.loads(p.stdout.strip().splitlines()[-1])
+            ref = dict(np.load(opath))
+            for name in rows:
+                d = np.abs(out[name].astype(np.float64) - ref[name].astype(np.float64))
+                res[name]["cpu_oracle_ms"] = round(times[name] * 1e3, 1)
+                res[name]["cpu_oracle_threads"] = 1
+                res[name]["max_abs_diff_vs_oracle"] = float(d.max())
+        else:
+            res["cpu_oracle_error"] = p.stderr[-400:]
+    res["_note"] = ("320x240 configuration image; estimate_alpha_cf_vcycle is this package's "
+                    "default preconditioner compared with the oracle's ichol-preconditioned "
+                    "alpha (both at rtol 1e-7); the cutout's alpha channel is the ichol-free "
+                    "default against the oracle's ichol, 8-bit quantised")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -507,7 +609,7 @@ def main():
     ap.add_argument("--pmc-probe", action="store_true")
     ap.add_argument("--c2-only", action="store_true",
                     help="the 1920x1080 line alone, without the C3-C5 entries")
-    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "rows"], default="c2",
                     help="c2: 1920x1080 solve (the metric); c3: 64 images of 256..1024 px, "
                          "batched vs one at a time; c4: 3840x2160 5-level V(2,2) solve and "
                          "float32 foreground; c5: gradient of a 16x512x512 batch")
@@ -519,8 +621,9 @@ def main():
     if args.pmc_probe:
         pmc_probe()
         return
-    if args.workload in ("c3", "c4", "c5"):
-        print(json.dumps({"c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.workload](args)))
+    if args.workload in ("c3", "c4", "c5", "rows"):
+        print(json.dumps({"c3": bench_c3, "c4": bench_c4, "c5": bench_c5,
+                          "rows": bench_rows}[args.workload](args)))
         return
 
     import numpy as np
@@ -753,6 +856,8 @@ def main():
         line["c4"] = _sub_line(bench_c4(args))
         torch.cuda.empty_cache()
         line["c5"] = _sub_line(bench_c5(args))
+        torch.cuda.empty_cache()
+        line["other_rows_320x240"] = bench_rows(args)
     print(json.dumps(line))
 
 

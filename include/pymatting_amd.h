@@ -41,13 +41,14 @@ extern "C" {
 #define HM_PC_VCYCLE 2
 #define HM_PC_CSR 3  /* z = M r with a matrix M (cg.py:59-62) */
 #define HM_PC_HOST 4 /* z = M(r) through a host callback (callable M, cg.py:57-58) */
-#define HM_PC_ICHOL 5 /* z = L^-T L^-1 r with an incomplete Cholesky factor (ichol.py:175-181) */
+#define HM_PC_ICHOL 5 /* z = L^-T L^-1 r with an incomplete Cholesky factor (ichol.py:175-181);
+                       * the substitutions run on the host, the state is read every iteration */
 
 typedef struct hm_cf hm_cf;   /* device image + closed-form window operator */
 typedef struct hm_dia hm_dia; /* stored 25-point grid stencil (coarse levels, ELL form) */
 typedef struct hm_csr hm_csr; /* general CSR matrix (cg on arbitrary matrices) */
 typedef struct hm_vc hm_vc;   /* V-cycle hierarchy */
-typedef struct hm_ichol hm_ichol; /* incomplete Cholesky factor with its device solves */
+typedef struct hm_ichol hm_ichol; /* incomplete Cholesky factor */
 typedef struct hm_lkm hm_lkm;     /* large-kernel (box filter) matting Laplacian */
 
 typedef struct hm_cg_stats {
@@ -275,12 +276,9 @@ int hm_cg_solve_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, d
  * positive, or -2 when more than max_nnz entries would be stored (the
  * reference's return codes; HM_OK is returned in all three cases and *out
  * is null for -1/-2). The factorisation runs on the host in one thread and
- * makes no device call; the factor and the level schedules of its two
- * triangular solves are uploaded by the first hm_ichol_apply (or the first
- * CG iteration that uses it), on that call's stream. hm_ichol_create makes
- * the handle of a given factor (CholeskyDecomposition(Ltuple),
- * ichol.py:148-162) from host CSC arrays whose columns start with the
- * diagonal, rows ascending. */
+ * makes no device call. hm_ichol_create makes the handle of a given factor
+ * (CholeskyDecomposition(Ltuple), ichol.py:148-162) from host CSC arrays
+ * whose columns start with the diagonal, rows ascending. */
 int hm_ichol_factor(int64_t n, const int64_t* indptr, const int64_t* indices,
                     const double* values, double discard_threshold, double shift,
                     int64_t max_nnz, double relative_discard_threshold, int diag_keep_discarded,
@@ -288,14 +286,14 @@ int hm_ichol_factor(int64_t n, const int64_t* indptr, const int64_t* indices,
 int hm_ichol_create(int64_t n, const double* Lv, const int64_t* Lr, const int64_t* Lp,
                     hm_ichol** out);
 int64_t hm_ichol_nnz(const hm_ichol* h);
-/* Number of levels of the forward and backward solves. */
-int hm_ichol_levels(const hm_ichol* h, int* forward, int* backward);
 /* Copies the factor (CSC: Lv[nnz], Lr[nnz], Lp[n+1]) to host arrays. */
 int hm_ichol_copy_factor(const hm_ichol* h, double* Lv, int64_t* Lr, int64_t* Lp);
-/* z = L^-T L^-1 r (CholeskyDecomposition.__call__, ichol.py:175-181): forward
- * then back substitution on device float64 vectors, bit-identical to the
- * reference's substitutions. Calls on one handle must not overlap (it keeps
- * the intermediate vector). */
+/* z = L^-T L^-1 r (CholeskyDecomposition.__call__, ichol.py:175-181) for
+ * device float64 vectors: r is copied to pinned host memory on `stream`, the
+ * forward and back substitutions run on the host (sequential chains of
+ * thousands of rows; bit-identical to the reference's), and z is copied back
+ * on `stream`. Calls on one handle must not overlap (it keeps the staging
+ * buffer). */
 int hm_ichol_apply(hm_ichol* h, const double* r, double* z, void* stream);
 int hm_ichol_destroy(hm_ichol* h);
 

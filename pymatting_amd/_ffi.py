@@ -110,7 +110,6 @@ _SIGS = {
                               ctypes.POINTER(P)]),
     "hm_ichol_create": (i32, [i64, P, P, P, ctypes.POINTER(P)]),
     "hm_ichol_nnz": (i64, [P]),
-    "hm_ichol_levels": (i32, [P, ctypes.POINTER(i32), ctypes.POINTER(i32)]),
     "hm_ichol_copy_factor": (i32, [P, P, P, P]),
     "hm_ichol_apply": (i32, [P, P, P, P]),
     "hm_ichol_destroy": (i32, [P]),

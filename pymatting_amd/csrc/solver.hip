@@ -510,7 +510,11 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
     st.host->atol = opt.atol;
     st.host->rtol = opt.rtol;
     HM_TRY(st.push(s));
-    const bool interactive = A.cb || M.kind == HM_PC_HOST || opt.callback;
+    // ichol's application runs on the host (it synchronises the stream
+    // anyway): the state is read after every iteration so that no
+    // application follows convergence.
+    const bool interactive =
+        A.cb || M.kind == HM_PC_HOST || M.kind == HM_PC_ICHOL || opt.callback;
 
     // norm_b = ||b|| (cg.py:66)
     HM_TRY(launch_dot<T>(b, b, nullptr, n, st.dev, red, kDotAux, s));

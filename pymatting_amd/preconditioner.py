@@ -4,9 +4,9 @@ Drop-ins for pymatting/preconditioner/jacobi.py:1-31 (jacobi),
 pymatting/preconditioner/vcycle.py:103-153 (vcycle) and
 pymatting/preconditioner/ichol.py:148-301 (ichol, CholeskyDecomposition).
 All return callables (numpy or torch vectors in, the same type out) that
-``cg`` recognises and runs inside its device loop. ichol's factorisation is
-sequential and runs in the library on the host; its two triangular solves
-run on the GPU.
+``cg`` recognises and runs inside its device loop. ichol is sequential by
+construction: its factorisation and its triangular solves run in the
+library on the host.
 """
 import ctypes
 
@@ -163,12 +163,14 @@ def vcycle(A, shape, num_pre_iter=1, num_post_iter=1, omega=0.8, direct_solve_si
 
 class CholeskyDecomposition:
     """ichol.py:148-181: an incomplete Cholesky factor L (CSC) applied as
-    x = L^-T L^-1 b by forward and back substitution, on the GPU.
+    x = L^-T L^-1 b by forward and back substitution.
 
     ``Ltuple`` is (Lv, Lr, Lp), the factor's values, row indices and column
     pointers (each column starts with its diagonal). Calling the object with
-    a numpy vector returns a numpy vector; a torch tensor stays on the device.
-    ``cg`` runs the two solves inside its device loop.
+    a numpy vector returns a numpy vector; a torch tensor stays on the
+    device. ``cg`` applies it inside its device loop: the substitutions, long
+    sequential chains, run in the library on the host, between copies of
+    the vector.
     """
 
     def __init__(self, Ltuple, _handle=None):
@@ -191,12 +193,6 @@ class CholeskyDecomposition:
     def L(self):
         """The factor as a scipy.sparse.csc_matrix."""
         return scipy.sparse.csc_matrix(self.Ltuple, (self.n, self.n))
-
-    def levels(self):
-        """(forward, backward): the level counts of the two triangular solves."""
-        f, b = ctypes.c_int(), ctypes.c_int()
-        _ffi.check(_ffi.lib().hm_ichol_levels(self.handle, ctypes.byref(f), ctypes.byref(b)))
-        return f.value, b.value
 
     def __call__(self, b):
         if _ffi.numel(b) != self.n:
@@ -222,8 +218,9 @@ def ichol(A, discard_threshold=1e-4,
     has its duplicates summed in place, and the factorisation is retried with
     each diagonal shift in turn until every pivot is positive (printing the
     reference's PERFORMANCE WARNING for each failed shift). The factorisation
-    runs in the library on the host, like the reference's; the returned
-    CholeskyDecomposition applies it on the GPU.
+    runs in the library on the host, like the reference's, and so do the
+    substitutions that apply it (CholeskyDecomposition); cg keeps every
+    other step of the iteration on the GPU.
     """
     if isinstance(A, scipy.sparse.csr_matrix):
         A = A.T

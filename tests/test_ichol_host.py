@@ -28,7 +28,6 @@ def _lib():
     lib.hm_ichol_copy_factor.argtypes = [P, P, P, P]
     lib.hm_ichol_nnz.restype = i64
     lib.hm_ichol_nnz.argtypes = [P]
-    lib.hm_ichol_levels.argtypes = [P, ctypes.POINTER(i32), ctypes.POINTER(i32)]
     lib.hm_ichol_destroy.argtypes = [P]
     return lib
 
@@ -50,14 +49,12 @@ def factor(A, discard=1e-4, shift=0.0, max_nnz=10 ** 8, rel=0.0, keep=True):
                                int(keep), ctypes.byref(res), ctypes.byref(h)) == 0
     if res.value < 0:
         assert not h.value
-        return res.value, None, None
+        return res.value, None
     nnz = lib.hm_ichol_nnz(h)
     Lv, Lr, Lp = np.empty(nnz), np.empty(nnz, np.int64), np.empty(A.shape[0] + 1, np.int64)
     assert lib.hm_ichol_copy_factor(h, _p(Lv), _p(Lr), _p(Lp)) == 0
-    f, b = ctypes.c_int(), ctypes.c_int()
-    assert lib.hm_ichol_levels(h, ctypes.byref(f), ctypes.byref(b)) == 0
     lib.hm_ichol_destroy(h)
-    return res.value, (Lv, Lr, Lp), (f.value, b.value)
+    return res.value, (Lv, Lr, Lp)
 
 
 def oracle_factor(A, discard=1e-4, shift=0.0, max_nnz=10 ** 8, rel=0.0, keep=True):
@@ -90,12 +87,11 @@ def test_factor_matches_oracle_on_random_spd():
         A[np.random.rand(n, n) < 0.8] = 0
         A += A.T + n * np.eye(n)
         for discard in (0.0, 1e-4, 0.05):
-            nnz, L, levels = factor(A, discard=discard, max_nnz=100000)
+            nnz, L = factor(A, discard=discard, max_nnz=100000)
             nnz_o, L_o = oracle_factor(A, discard=discard, max_nnz=100000)
             assert nnz == nnz_o and _same(L, L_o)
-            assert levels[0] >= 1 and levels[1] >= 1
         # No dropping: L L^T = A exactly up to rounding.
-        _, (Lv, Lr, Lp), _ = factor(A, discard=0.0, max_nnz=100000)
+        _, (Lv, Lr, Lp) = factor(A, discard=0.0, max_nnz=100000)
         Lm = scipy.sparse.csc_matrix((Lv, Lr, Lp), (n, n)).toarray()
         assert np.abs(Lm @ Lm.T - A).max() < 1e-10 * np.abs(A).max()
 
@@ -105,18 +101,16 @@ def test_factor_matches_oracle_on_lemur_system():
     d = load_golden("lemur_0p125")
     A, _ = ocf.make_linear_system(ocf.cf_laplacian(d["image"]), d["trimap"])
     for kw in (dict(), dict(rel=1e-3), dict(keep=False), dict(discard=1e-3, shift=2e-3)):
-        nnz, L, levels = factor(A, max_nnz=500000, **kw)
+        nnz, L = factor(A, max_nnz=500000, **kw)
         nnz_o, L_o = oracle_factor(A, max_nnz=500000, **kw)
         assert nnz == nnz_o and nnz > A.shape[0] and _same(L, L_o), kw
-        # The level schedules are shorter than the sequential order.
-        assert 1 <= levels[0] < A.shape[0] and 1 <= levels[1] < A.shape[0]
 
 
 def test_return_codes():
     A = np.array([[1.0, 2.0], [2.0, 1.0]])  # indefinite: the second pivot is -3
     assert factor(A, discard=0.0)[0] == -1
     assert oracle_factor(A, discard=0.0)[0] == -1
-    nnz, L, _ = factor(A, discard=0.0, shift=4.0)
+    nnz, L = factor(A, discard=0.0, shift=4.0)
     assert nnz == 3 and _same(L, oracle_factor(A, discard=0.0, shift=4.0)[1])
     B = np.random.default_rng(1).random((30, 30))
     B = B + B.T + 30 * np.eye(30)
