@@ -20,14 +20,16 @@ value together with a strict pair (both forms at rtol 1e-9, each within
 1e-4 of the converged alpha) and the restarted float32-in-float64
 refinement.
 
-The roofline object covers the stored-stencil SpMV on the finest grid, the
-pass that every PCG iteration and every fine-level smoothing sweep makes
-(y = A x over the 13 symmetric coefficient planes, float32): HIP events on
-the library's stream bracket 50 back-to-back launches on the solve's own
-operator and vectors, and achieved = algorithmic bytes per launch / mean
-launch duration. A per-class breakdown of one extra mixed step (events
-around every launch, which adds a few microseconds per launch) is reported
-beside it. The CPU baseline times the oracle's restatement of the reference
+The roofline object covers the longest kernel of the value form's
+iteration, the finest grid's pre-smooth + residual + full-weighting
+restriction (13 float32 coefficient planes, r and 1/D in, the coarse
+right-hand side out): HIP events on the library's stream bracket 50
+back-to-back launches on the solve's own V-cycle, and achieved = algorithmic
+bytes per launch / mean launch duration. The fine prolongation +
+post-smooth and the stored-stencil SpMV y = A x are measured the same way
+beside it. Per-class breakdowns of one extra step of each form (events
+around every launch, which adds a few microseconds to each) give the
+per-grid-level figures and the step's algorithmic bytes. The CPU baseline times the oracle's restatement of the reference
 (scipy CSR, vcycle PCG at rtol 1e-7, float64) single-threaded on the same
 1920x1080 workload, one solve.
 
@@ -40,8 +42,8 @@ matting, radius 2 and the cutout pipeline on the 320x240 image, each beside
 its single-threaded CPU oracle). --c2-only leaves them out.
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--c2-only]
-       python bench.py --pmc-probe   (SpMV launches and a 256 MiB copy only,
-                                      for hardware-counter runs)
+       python bench.py --pmc-probe   (SpMV launches, V-cycle applications and a
+                                      256 MiB copy only, for hardware-counter runs)
 """
 import argparse
 import ctypes
@@ -55,12 +57,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the float32 stored-stencil SpMV at 1920x1080, from
-# two rocprofv3 --pmc passes over `python bench.py --pmc-probe` (FETCH_SIZE,
-# then WRITE_SIZE, KiB per dispatch), each scaled by what the same counter
-# reads for a 256 MiB copy (FETCH_SIZE x 1.9998, WRITE_SIZE x 1.0):
-# 57668.3 KiB x 1.9998 + 8100.0 KiB = 126.4 MB (algorithmic: 124.4 MB).
-TRAFFIC_SPMV_F32 = 126384910
+# HBM bytes per launch of three float32 kernels at 1920x1080, from two
+# rocprofv3 --pmc passes over `python bench.py --pmc-probe` (FETCH_SIZE, then
+# WRITE_SIZE, KiB per dispatch), each scaled by what the same counter reads
+# for a 256 MiB copy (FETCH_SIZE x 1.999796, WRITE_SIZE x 1.0):
+#   fine-grid restriction: 62025.7 KiB x 1.999796 + 2025.0 KiB = 129.1 MB
+#     (algorithmic 126.5 MB);
+#   fine-grid prolongation + smoothing: 63358.6 KiB x 1.999796 + 8100.0 KiB
+#     = 138.0 MB (algorithmic 134.8 MB);
+#   stored-stencil SpMV: 57579.3 KiB x 1.999796 + 8100.0 KiB = 126.2 MB
+#     (algorithmic 124.4 MB).
+TRAFFIC_RESTRICT_F32 = 129089331
+TRAFFIC_SMOOTH_F32 = 138039603
+TRAFFIC_SPMV_F32 = 126204826
 
 # Algorithmic bytes per grid point and launch, by kernel class, for vector
 # element size s (8 float64, 4 float32). Matrix-free passes read the image
@@ -87,13 +96,17 @@ def bytes_per_point(kind, s):
         "coarse_residual": 13 * s + 3 * s,
         "cg_update_x64": 8 + s + 2 * s + s + 8,            # x (f64), p, r, q -> x, r
         "cg_update_r": 3 * s,                               # r, q -> r (x beside the tail)
+        "l1_apply": 13 * s + 2 * s,
+        "l1_restrict": 13 * s + 2 * s + s / 4,
+        "l1_smooth": 13 * s + 2 * s + s / 4 + s,
+        "l1_residual": 13 * s + 3 * s,
     }[kind]
 
 
 KINDS = ["cf_apply", "cf_restrict", "cf_smooth", "cf_residual", "dia_apply", "dia_restrict",
          "dia_smooth", "dia_residual", "cg_update", "cg_pupdate", "tail", "fg_sweep",
          "coarse_apply", "coarse_restrict", "coarse_smooth", "coarse_residual", "cg_update_x64",
-         "cg_update_r"]
+         "cg_update_r", "l1_apply", "l1_restrict", "l1_smooth", "l1_residual"]
 
 
 # Algorithmic bytes per grid point of the stored-stencil SpMV y = A x:
@@ -137,13 +150,18 @@ def pmc_probe():
     for _ in range(20):
         _ffi.check(L.hm_dia_apply(solver.op.handle, _ffi.HM_F32, _ffi.ptr(x), _ffi.ptr(y),
                                   _ffi.stream()))
+    # V-cycle applications: each runs the fine-grid restriction (pre-smooth +
+    # residual + full weighting) and prolongation + post-smooth once.
+    for _ in range(20):
+        solver.M.apply(x, y)
     a = torch.empty(64 << 20, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
     for _ in range(5):
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"probe": "spmv_f32_1080p", "n": n, "spmv_launches": 20,
-                      "copy_bytes": a.numel() * 4, "copy_launches": 5}))
+                      "vcycle_applications": 20, "copy_bytes": a.numel() * 4,
+                      "copy_launches": 5}))
 
 
 def _cpu_model():
@@ -345,6 +363,13 @@ def bench_c4(args):
     alpha_f, st_f, t_f = timed("mixed", "matrix_free", rtol=1e-7, inner_rtol=INNER)
     conv = pm.LambdaSolver(image, trimap, precision="f64", level0="stencil")
     alpha_c = conv.solve(rtol=1e-11, maxiter=100000)
+    # The float32 stored-stencil SpMV at 4K: 431 MB of coefficients, more
+    # than the 256 MB MALL, so every launch streams from HBM.
+    from pymatting_amd import _ffi
+    xs = torch.rand(n, device="cuda", dtype=torch.float32)
+    ys = torch.empty_like(xs)
+    t_spmv = time_spmv(_ffi.lib(), conv.op.handle, _ffi.HM_F32, xs, ys)
+    spmv_gbps = spmv_bytes_per_point(4) * n / t_spmv / 1e9
 
     def err(a, ref=None):
         d = (a - (alpha_c if ref is None else ref)).abs()
@@ -401,6 +426,10 @@ def bench_c4(args):
                                         "max_abs_dalpha_vs_converged": err(alpha_f)[0]},
         "foreground_f32": {"ms": round(t_fg * 1e3, 3), "Mpix_per_s": round(n / t_fg / 1e6, 2),
                            "max_abs_dF_vs_f64": dF, "max_abs_dB_vs_f64": dB},
+        "spmv_f32_4k": {"bytes_per_launch": int(spmv_bytes_per_point(4) * n),
+                        "avg_launch_us": round(t_spmv * 1e6, 2),
+                        "achieved_GBps": round(spmv_gbps, 1),
+                        "frac": round(spmv_gbps / HBM_PEAK_GBS, 4)},
     }
     return line
 
@@ -687,16 +716,30 @@ def main():
     xd = xs.double()
     yd = torch.empty_like(xd)
     t_spmv64 = time_spmv(L, op.handle, _ffi.HM_F64, xd, yd)
+    # The roofline kernel: the finest grid's pre-smooth + residual +
+    # restriction of the value form's own V-cycle (the longest kernel of its
+    # iteration), with its prolongation + post-smooth beside it; 50
+    # back-to-back launches each, HIP events on the library's stream.
+    r_us, s_us = ctypes.c_double(), ctypes.c_double()
+    _ffi.check(L.hm_vcycle_time_fine(last["solver"].M.handle, _ffi.ptr(xs), 50, ctypes.byref(r_us),
+                                     ctypes.byref(s_us), _ffi.stream()))
+    t_restrict, t_smooth = r_us.value * 1e-6, s_us.value * 1e-6
     # Separate pass with per-launch events (they would perturb the timing).
-    timed(MIXED, 1, prof=True)
-    prof = {}
-    for k, kind in enumerate(KINDS):
-        for prec, s in ((0, 8), (1, 4)):
-            ms, cnt, pts = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64()
-            L.hm_prof_query(k, prec, ctypes.byref(ms), ctypes.byref(cnt), ctypes.byref(pts))
-            if cnt.value:
-                prof[(kind, prec)] = (ms.value, cnt.value, pts.value, s)
-    L.hm_prof_set(0)
+    def profile(cfg):
+        """Per-class event timing of one extra step of cfg."""
+        timed(cfg, 1, prof=True)
+        out = {}
+        for k, kind in enumerate(KINDS):
+            for prec, s in ((0, 8), (1, 4)):
+                ms, cnt, pts = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64()
+                L.hm_prof_query(k, prec, ctypes.byref(ms), ctypes.byref(cnt), ctypes.byref(pts))
+                if cnt.value:
+                    out[(kind, prec)] = (ms.value, cnt.value, pts.value, s)
+        L.hm_prof_set(0)
+        return out
+
+    prof = profile(MIXED)
+    prof64 = profile(FP64)
     alpha_d, st_d, t_f64 = timed(FP64, args.steps)
     alpha_ms, st_ms, t_ms = timed(MIXED_STRICT, args.steps)
     alpha_ds, st_ds, t_ds = timed(FP64_STRICT, args.steps)
@@ -741,17 +784,39 @@ def main():
     bytes_launch = spmv_bytes_per_point(4) * n
     achieved = bytes_launch / t_spmv32 / 1e9
     achieved64 = spmv_bytes_per_point(8) * n / t_spmv64 / 1e9
-    breakdown = {}
-    step_bytes = 0.0
-    for (k, p), v in sorted(prof.items(), key=lambda kv: -kv[1][0]):
-        ms, cnt, pts, s = v
-        entry = {"ms_total": round(ms, 3), "launches": cnt,
-                 "avg_us_with_events": round(ms / cnt * 1e3, 2)}
-        bpp = bytes_per_point(k, s)
-        if bpp is not None:
-            entry["GBps_with_events"] = round(bpp * pts / (ms * 1e-3) / 1e9, 1)
-            step_bytes += bpp * pts
-        breakdown["%s_%s" % (k, "f32" if p else "f64")] = entry
+    restrict_bytes = bytes_per_point("dia_restrict", 4) * n
+    smooth_bytes = bytes_per_point("dia_smooth", 4) * n
+    achieved_r = restrict_bytes / t_restrict / 1e9
+    achieved_s = smooth_bytes / t_smooth / 1e9
+    def summarise(pr):
+        """Per-class launches, time and algorithmic GB/s (events around each
+        launch add a few microseconds to each), and the step's bytes."""
+        table, total = {}, 0.0
+        for (k, p), v in sorted(pr.items(), key=lambda kv: -kv[1][0]):
+            ms, cnt, pts, s = v
+            entry = {"ms_total": round(ms, 3), "launches": cnt,
+                     "avg_us_with_events": round(ms / cnt * 1e3, 2)}
+            bpp = bytes_per_point(k, s)
+            if bpp is not None:
+                gbps = bpp * pts / (ms * 1e-3) / 1e9
+                entry["bytes_per_launch"] = int(bpp * pts / cnt)
+                entry["GBps_with_events"] = round(gbps, 1)
+                entry["frac_of_hbm_peak"] = round(gbps / HBM_PEAK_GBS, 4)
+                total += bpp * pts
+            table["%s_%s" % (k, "f32" if p else "f64")] = entry
+        return table, total
+
+    breakdown, step_bytes = summarise(prof)
+    breakdown64, step_bytes64 = summarise(prof64)
+    # The solve's own kernels (fine grid: the fused SpMV with the direction
+    # update, the pre-smooth + residual + restriction, the prolongation +
+    # post-smooth; the first Galerkin grid; the coarser grids) as
+    # algorithmic bytes per launch over the mean event-timed launch.
+    solve_kernels = {k: {q: breakdown[k][q] for q in ("bytes_per_launch", "avg_us_with_events",
+                                                       "GBps_with_events", "frac_of_hbm_peak")}
+                     for k in ("dia_apply_f32", "dia_restrict_f32", "dia_smooth_f32",
+                               "l1_restrict_f32", "l1_smooth_f32", "coarse_restrict_f32",
+                               "coarse_smooth_f32") if k in breakdown}
 
     oracle_vs = None
     if args.no_cpu:
@@ -826,21 +891,47 @@ def main():
                                        "max_abs_dalpha_vs_converged": e_x[0],
                                        "rms_dalpha_vs_converged": e_x[1]},
         "roofline": {"bound": "hbm",
-                     "kernel": "tile_kernel<float, DiaOp<float, false>, InVec<float>, "
-                               "OutApply<float, 0>> (stored-stencil SpMV, 1920x1080)",
-                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_SPMV_F32,
-                     "bytes_per_launch": int(bytes_launch),
-                     "avg_launch_us": round(t_spmv32 * 1e6, 2),
-                     "f64": {"achieved": round(achieved64, 1),
-                             "frac": round(achieved64 / HBM_PEAK_GBS, 4),
-                             "avg_launch_us": round(t_spmv64 * 1e6, 2)}},
+                     "kernel": "tile_kernel<float, DiaOp<float, false>, InJac0<float>, "
+                               "OutRestrict<float, 0>, 0> (finest-grid pre-smooth + residual + "
+                               "restriction of the value form's V-cycle, 1920x1080)",
+                     "achieved": round(achieved_r, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved_r / HBM_PEAK_GBS, 4),
+                     "traffic": TRAFFIC_RESTRICT_F32,
+                     "bytes_per_launch": int(restrict_bytes),
+                     "avg_launch_us": round(t_restrict * 1e6, 2),
+                     "fine_smooth": {"kernel": "tile_kernel<float, DiaOp<float, false>, "
+                                               "InProlong<float, InJac0<float> >, "
+                                               "OutSmooth<float, 0>, 0>",
+                                     "achieved": round(achieved_s, 1),
+                                     "frac": round(achieved_s / HBM_PEAK_GBS, 4),
+                                     "traffic": TRAFFIC_SMOOTH_F32,
+                                     "bytes_per_launch": int(smooth_bytes),
+                                     "avg_launch_us": round(t_smooth * 1e6, 2)},
+                     "proxy_spmv": {"kernel": "tile_kernel<float, DiaOp<float, false>, "
+                                              "InVec<float>, OutApply<float, 0>, 0> (y = A x on "
+                                              "the solve's stored operator)",
+                                    "achieved": round(achieved, 1),
+                                    "frac": round(achieved / HBM_PEAK_GBS, 4),
+                                    "traffic": TRAFFIC_SPMV_F32,
+                                    "bytes_per_launch": int(bytes_launch),
+                                    "avg_launch_us": round(t_spmv32 * 1e6, 2),
+                                    "f64": {"achieved": round(achieved64, 1),
+                                            "frac": round(achieved64 / HBM_PEAK_GBS, 4),
+                                            "avg_launch_us": round(t_spmv64 * 1e6, 2)}}},
+        "solve_kernels_mixed": solve_kernels,
         "kernel_breakdown_timed_mixed": breakdown,
-        # Algorithmic bytes of one value-form step (the classes above; the
-        # LDS tail's launches, with the x update they carry, are not
-        # counted) over the step's time.
+        "kernel_breakdown_timed_fp64": breakdown64,
+        # Algorithmic bytes of one step (the classes above; the LDS tail's
+        # launches, with the x update they carry, are not counted) over the
+        # step's time, for both forms.
         "step_bytes_model": {"GB": round(step_bytes / 1e9, 3),
-                             "GBps_over_step": round(step_bytes / t_mixed / 1e9, 1)},
+                             "GBps_over_step": round(step_bytes / t_mixed / 1e9, 1),
+                             "frac_of_hbm_peak": round(step_bytes / t_mixed / 1e9
+                                                       / HBM_PEAK_GBS, 4)},
+        "step_bytes_model_fp64": {"GB": round(step_bytes64 / 1e9, 3),
+                                  "GBps_over_step": round(step_bytes64 / t_f64 / 1e9, 1),
+                                  "frac_of_hbm_peak": round(step_bytes64 / t_f64 / 1e9
+                                                            / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
     }
     if not args.c2_only:

@@ -409,11 +409,24 @@ int hm_fb_ml_noalloc(const void* image, const void* alpha, int h0, int w0, int n
 #define HM_PROF_CG_PUPDATE 9
 #define HM_PROF_DENSE 10
 #define HM_PROF_FG_SWEEP 11
-#define HM_PROF_COARSE_APPLY 12    /* stored stencils of Galerkin levels (>= 1) */
+#define HM_PROF_COARSE_APPLY 12    /* stored stencils of Galerkin levels >= 2 */
 #define HM_PROF_COARSE_RESTRICT 13
 #define HM_PROF_COARSE_SMOOTH 14
 #define HM_PROF_COARSE_RESIDUAL 15
+#define HM_PROF_CG_UPDATE_X64 16   /* float32 CG update with the solution in float64 */
+#define HM_PROF_CG_UPDATE_R 17     /* CG update of r alone (x beside the V-cycle tail) */
+#define HM_PROF_LEVEL1_APPLY 18    /* the first Galerkin level */
+#define HM_PROF_LEVEL1_RESTRICT 19
+#define HM_PROF_LEVEL1_SMOOTH 20
+#define HM_PROF_LEVEL1_RESIDUAL 21
 int hm_prof_set(int enabled);
+/* Mean duration (microseconds) of reps back-to-back launches of the finest
+ * grid's pre-smooth + residual + restriction and of its prolongation +
+ * post-smooth, on the hierarchy's own operator and buffers with r (device
+ * vector in the hierarchy's precision) as the right-hand side: the two
+ * fine-grid kernels of each V(1,1) cycle, timed with HIP events. */
+int hm_vcycle_time_fine(hm_vc* vc, const void* r, int reps, double* restrict_us,
+                        double* smooth_us, void* stream);
 int hm_prof_query(int kind, int prec, double* total_ms, int64_t* launches, int64_t* points);
 
 #ifdef __cplusplus

@@ -98,6 +98,7 @@ _SIGS = {
     "hm_diag_apply": (i32, [P, P, P, i64, P]),
     "hm_alpha_finalize": (i32, [P, P, P, P, i64, P]),
     "hm_prof_set": (i32, [i32]),
+    "hm_vcycle_time_fine": (i32, [P, P, i32, ctypes.POINTER(f64), ctypes.POINTER(f64), P]),
     "hm_prof_query": (i32, [i32, i32, ctypes.POINTER(f64), ctypes.POINTER(i64), ctypes.POINTER(i64)]),
     "hm_fb_ml": (i32, [P, P, i32, i32, i32, P, P, f64, i32, i32, i32, f64, i32, P, P, P]),
     "hm_fb_ml_noalloc": (i32, [P, P, i32, i32, i32, P, P, f64, i32, i32, i32, f64, i32, P, P,

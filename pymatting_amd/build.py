@@ -53,7 +53,8 @@ def torch_lib_dir():
 
 def compile_one(src):
     out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *COMMON, *SOURCES[src], "-c", os.path.join(CSRC, src), "-o", out]
+    extra = os.environ.get("HM_EXTRA_FLAGS", "").split()  # e.g. tuning -D overrides
+    cmd = [HIPCC, *COMMON, *SOURCES[src], *extra, "-c", os.path.join(CSRC, src), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (src, " ".join(cmd), r.stderr))

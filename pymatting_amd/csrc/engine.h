@@ -491,8 +491,11 @@ __device__ __forceinline__ void dot_finalize(CgState* st, int kind, double s, do
 // keeps the halo overhead low. S = 1 (coarse grids, latency-bound): one point
 // per thread, so a thread's coefficient loads (25) stay within the loads a
 // wave can keep in flight, and a small grid spreads over more workgroups.
+#ifndef HM_RESTRICT_CY
+#define HM_RESTRICT_CY 4  // coarse rows of a large grid's restriction tile
+#endif
 template <int S>
-struct TileDims { static constexpr int X = 64, Y = 8, CX = 32, CY = 4; };
+struct TileDims { static constexpr int X = 64, Y = 8, CX = 32, CY = HM_RESTRICT_CY; };
 template <>
 struct TileDims<1> { static constexpr int X = 32, Y = 8, CX = 16, CY = 3; };
 
@@ -725,7 +728,9 @@ __device__ __forceinline__ void tile_body(const Op& op, const In& in, const Out&
     }
 }
 
-template <typename T, class Op, class In, class Out>
+// kLevel only names the grid class of a launch in traces (0 the finest grid,
+// 1 the first Galerkin grid, 2 the coarser ones); the code is the same.
+template <typename T, class Op, class In, class Out, int kLevel = 0>
 __global__ __launch_bounds__(kBlock) void tile_kernel(Op op, In in, Out out, int h, int w,
                                                       const CgState* guard, TileMap map) {
     const bool done = solve_done(guard);

@@ -1106,6 +1106,49 @@ int vc_create(hm_cf* cf, hm_dia* dia, int h, int w, int pre, int post, double om
 
 using namespace hm;
 
+namespace hm {
+
+// Mean duration of `reps` back-to-back launches of the finest grid's
+// pre-smooth + residual + restriction (the V-cycle's first kernel, on the
+// hierarchy's own operator and buffers, with r as the right-hand side) and
+// of its prolongation + post-smooth, timed with HIP events on the stream.
+template <typename T>
+int time_fine_t(hm_vc* vc, const T* r, int reps, double* restrict_us, double* smooth_us,
+                       hipStream_t s) {
+    const VcLevel& lev = vc->levels[0];
+    const VcLevel& cl = vc->levels[1];
+    const LevelOp<T> op = vc_level_op<T>(vc, 0);
+    const T omega = (T)vc->omega;
+    const DotSink nodot{nullptr, Reducer{}, 0};
+    DevBuf z;
+    HM_TRY(z.alloc(sizeof(T) * lev.n));
+    hipEvent_t e[3];
+    for (auto& ev : e) HM_HIP(hipEventCreate(&ev));
+    struct Ev { hipEvent_t* e; ~Ev() { for (int k = 0; k < 3; ++k) (void)hipEventDestroy(e[k]); } } g{e};
+    const OutRestrict<T> rest{r, vptr<T>(cl.b), cl.w, cl.h, nullptr, nodot};
+    const InJac0<T> jac{r, op.iD, omega};
+    const InProlong<T, InJac0<T>> pro{jac, vptr<T>(cl.b), cl.w, cl.h};
+    const OutSmooth<T> sm{r, op.iD, omega, z.as<T>(), nodot};
+    for (int k = 0; k < 3; ++k) {  // warm-up
+        HM_TRY(run_level<T>(op, jac, rest, nullptr, s));
+        HM_TRY(run_level<T>(op, pro, sm, nullptr, s));
+    }
+    HM_HIP(hipEventRecord(e[0], s));
+    for (int k = 0; k < reps; ++k) HM_TRY(run_level<T>(op, jac, rest, nullptr, s));
+    HM_HIP(hipEventRecord(e[1], s));
+    for (int k = 0; k < reps; ++k) HM_TRY(run_level<T>(op, pro, sm, nullptr, s));
+    HM_HIP(hipEventRecord(e[2], s));
+    HM_HIP(hipEventSynchronize(e[2]));
+    float a = 0.f, b = 0.f;
+    HM_HIP(hipEventElapsedTime(&a, e[0], e[1]));
+    HM_HIP(hipEventElapsedTime(&b, e[1], e[2]));
+    *restrict_us = 1e3 * a / reps;
+    *smooth_us = 1e3 * b / reps;
+    return HM_OK;
+}
+
+}  // namespace hm
+
 extern "C" {
 
 const char* hm_last_error(void) { return g_last_error.c_str(); }
@@ -1404,6 +1447,24 @@ int hm_vcycle_apply(hm_vc* vc, const void* r, void* z, void* stream) {
         return HM_ERR_ARG;
     }
     return vcycle_apply(vc, r, z, nullptr, nullptr, 0, nullptr, static_cast<hipStream_t>(stream));
+}
+
+int hm_vcycle_time_fine(hm_vc* vc, const void* r, int reps, double* restrict_us,
+                        double* smooth_us, void* stream) {
+    if (!vc || !r || reps < 1 || !restrict_us || !smooth_us) {
+        set_error("hm_vcycle_time_fine: invalid argument");
+        return HM_ERR_ARG;
+    }
+    if (vc->levels.size() < 2 || vc->pre != 1 || vc->post != 1) {
+        set_error("hm_vcycle_time_fine: needs a V(1,1) cycle of at least two grids");
+        return HM_ERR_ARG;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (vc->prec == HM_F64)
+        return hm::time_fine_t<double>(vc, static_cast<const double*>(r), reps, restrict_us,
+                                       smooth_us, s);
+    return hm::time_fine_t<float>(vc, static_cast<const float*>(r), reps, restrict_us,
+                                  smooth_us, s);
 }
 
 int hm_copy(void* dst, const void* src, int64_t bytes, int kind, void* stream) {

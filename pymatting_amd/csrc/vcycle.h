@@ -15,17 +15,33 @@ namespace hm {
 
 // ------------------------------------------------------------- launch helpers
 
+// level: the grid's place in the hierarchy (0 finest); profiling classes
+// and kernel names separate the finest grid, the first Galerkin grid and the
+// coarser ones.
 template <typename T, class Op, class In, class Out>
 int launch_tile(const Op& op, const In& in, const Out& out, int h, int w, int h2, int w2,
-                const CgState* guard, hipStream_t s, bool coarse = false) {
+                const CgState* guard, hipStream_t s, int level = 0) {
     const TileMap map = tile_map(out, h, w, h2, w2);
     if (out.dot.st != nullptr && (long long)map.ntiles > out.dot.red.max_blocks) {
         set_error("reduction buffer too small for %u blocks", map.ntiles);
         return HM_ERR_ARG;
     }
-    ProfScope prof((coarse ? (int)kProfCoarseApply : Op::kProfBase) + Out::kProf, (long long)h * w,
-                   sizeof(T) == 4, s);
-    tile_kernel<T, Op, In, Out><<<map.ntiles, kBlock, 0, s>>>(op, in, out, h, w, guard, map);
+    const int base = level == 0 ? (int)Op::kProfBase
+                                : (level == 1 ? (int)kProfLevel1Apply : (int)kProfCoarseApply);
+    ProfScope prof(base + Out::kProf, (long long)h * w, sizeof(T) == 4, s);
+    if constexpr (std::is_same<Op, CfOp<T>>::value) {
+        tile_kernel<T, Op, In, Out, 0><<<map.ntiles, kBlock, 0, s>>>(op, in, out, h, w, guard, map);
+    } else {
+        if (level == 0)
+            tile_kernel<T, Op, In, Out, 0><<<map.ntiles, kBlock, 0, s>>>(op, in, out, h, w, guard,
+                                                                         map);
+        else if (level == 1)
+            tile_kernel<T, Op, In, Out, 1><<<map.ntiles, kBlock, 0, s>>>(op, in, out, h, w, guard,
+                                                                         map);
+        else
+            tile_kernel<T, Op, In, Out, 2><<<map.ntiles, kBlock, 0, s>>>(op, in, out, h, w, guard,
+                                                                         map);
+    }
     HM_LAUNCH_CHECK();
     if (out.dot.st != nullptr && !out.dot.defer)
         return launch_finalize(1, out.dot.red, map.ntiles, out.dot.st, out.dot.kind, guard, s);
@@ -35,7 +51,7 @@ int launch_tile(const Op& op, const In& in, const Out& out, int h, int w, int h2
 template <typename T>
 struct LevelOp {
     bool is_cf = false;
-    bool coarse = false;  // a Galerkin level (profiling class only)
+    int level = 0;  // place in the hierarchy (profiling classes and kernel names only)
     CfOp<T> cf{};
     DiaOp<T> dia{};
     const T* iD = nullptr;  // 1 / diag
@@ -83,11 +99,12 @@ int run_level(const LevelOp<T>& L, const In& in, const Out& out, const CgState* 
     // thread with its coefficients loaded before the tile barrier. Larger
     // grids keep the bandwidth-oriented shape of the finest level.
     constexpr long long kSmallGrid = 1 << 18;
-    if (L.coarse && (long long)L.h * L.w < kSmallGrid) {
+    if (L.level > 0 && (long long)L.h * L.w < kSmallGrid) {
         const DiaOp<T, true> op{L.dia.h, L.dia.w, L.dia.coef};
-        return launch_tile<T>(op, in, out.small(), L.h, L.w, L.h / 2, L.w / 2, guard, s, true);
+        return launch_tile<T>(op, in, out.small(), L.h, L.w, L.h / 2, L.w / 2, guard, s,
+                              L.level);
     }
-    return launch_tile<T>(L.dia, in, out, L.h, L.w, L.h / 2, L.w / 2, guard, s, L.coarse);
+    return launch_tile<T>(L.dia, in, out, L.h, L.w, L.h / 2, L.w / 2, guard, s, L.level);
 }
 
 // ------------------------------------------------------------- vector kernels
@@ -253,7 +270,7 @@ template <typename T>
 LevelOp<T> vc_level_op(const hm_vc* vc, size_t l) {
     if (l == 0 && vc->cf) return cf_level<T>(vc->cf);
     LevelOp<T> L = dia_level<T>(vc->levels[l].A);
-    L.coarse = l > 0;
+    L.level = (int)l;
     return L;
 }
 
