@@ -100,19 +100,62 @@ def bytes_per_point(kind, s):
         "l1_restrict": 13 * s + 2 * s + s / 4,
         "l1_smooth": 13 * s + 2 * s + s / 4 + s,
         "l1_residual": 13 * s + 3 * s,
+        # Matrix-free passes that recompute the windows from the staged image
+        # read no window planes.
+        "cfrec_apply": 2 * s + s + 3 * s + 1 + 2 * s,
+        "cfrec_restrict": s + s + 3 * s + 1 + s / 4,
+        "cfrec_smooth": s + s + 3 * s + 1 + s / 4 + s,
+        "cfrec_residual": s + s + 3 * s + 1 + s,
     }[kind]
 
 
 KINDS = ["cf_apply", "cf_restrict", "cf_smooth", "cf_residual", "dia_apply", "dia_restrict",
          "dia_smooth", "dia_residual", "cg_update", "cg_pupdate", "tail", "fg_sweep",
          "coarse_apply", "coarse_restrict", "coarse_smooth", "coarse_residual", "cg_update_x64",
-         "cg_update_r", "l1_apply", "l1_restrict", "l1_smooth", "l1_residual"]
+         "cg_update_r", "l1_apply", "l1_restrict", "l1_smooth", "l1_residual", "cfrec_apply",
+         "cfrec_restrict", "cfrec_smooth", "cfrec_residual"]
 
 
 # Algorithmic bytes per grid point of the stored-stencil SpMV y = A x:
 # 13 coefficient planes, x and y.
 def spmv_bytes_per_point(s):
     return 13 * s + 2 * s
+
+
+def profile_classes(L, fn):
+    """Runs fn() once with per-launch events and returns {(kind, prec): (ms,
+    launches, points, element size)} for every kernel class it launched."""
+    import torch
+    L.hm_prof_set(1)
+    torch.cuda.synchronize()
+    fn()
+    torch.cuda.synchronize()
+    out = {}
+    for k, kind in enumerate(KINDS):
+        for prec, s in ((0, 8), (1, 4)):
+            ms, cnt, pts = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64()
+            L.hm_prof_query(k, prec, ctypes.byref(ms), ctypes.byref(cnt), ctypes.byref(pts))
+            if cnt.value:
+                out[(kind, prec)] = (ms.value, cnt.value, pts.value, s)
+    L.hm_prof_set(0)
+    return out
+
+
+def fine_passes(prof, prefix):
+    """Per fine-grid pass class (apply, restrict, smooth) of a float32
+    level-0 form: algorithmic bytes per point, event-timed mean launch and the
+    GB/s they give (events around each launch add a few microseconds)."""
+    out = {}
+    for cls in ("apply", "restrict", "smooth"):
+        v = prof.get(("%s_%s" % (prefix, cls), 1))
+        if v is None:
+            continue
+        ms, cnt, pts, s = v
+        bpp = bytes_per_point("%s_%s" % (prefix, cls), s)
+        out[cls] = {"bytes_per_point": bpp, "launches": cnt,
+                    "avg_us_with_events": round(ms / cnt * 1e3, 2),
+                    "GBps_with_events": round(bpp * pts / (ms * 1e-3) / 1e9, 1)}
+    return out
 
 
 def time_spmv(L, handle, prec, x, y, reps=50):
@@ -135,8 +178,9 @@ def time_spmv(L, handle, prec, x, y, reps=50):
 
 
 def pmc_probe():
-    """20 float32 SpMV launches on the 1080p stored operator and 5 copies of
-    256 MiB (the calibration kernel for FETCH_SIZE/WRITE_SIZE), nothing else."""
+    """20 float32 SpMV launches and 20 V-cycle applications on the 1080p stored
+    operator, 20 SpMV launches on the 3840x2160 one, and 5 copies of 256 MiB
+    (the calibration kernel for FETCH_SIZE/WRITE_SIZE), nothing else."""
     import torch
     import pymatting_amd as pm
     from pymatting_amd import _ffi, synthetic
@@ -154,14 +198,27 @@ def pmc_probe():
     # residual + full weighting) and prolongation + post-smooth once.
     for _ in range(20):
         solver.M.apply(x, y)
+    del solver, x, y
+    torch.cuda.empty_cache()
+    # The 3840x2160 SpMV, whose 497.7 MB working set is larger than the
+    # 256 MB Infinity Cache.
+    image_h, trimap_h, _ = synthetic.config4()
+    solver = pm.LambdaSolver(torch.from_numpy(image_h).cuda(), torch.from_numpy(trimap_h).cuda(),
+                             precision="mixed", level0="stencil")
+    n4 = solver.op.n
+    x = torch.rand(n4, device="cuda", dtype=torch.float32)
+    y = torch.empty_like(x)
+    for _ in range(20):
+        _ffi.check(L.hm_dia_apply(solver.op.handle, _ffi.HM_F32, _ffi.ptr(x), _ffi.ptr(y),
+                                  _ffi.stream()))
     a = torch.empty(64 << 20, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
     for _ in range(5):
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"probe": "spmv_f32_1080p", "n": n, "spmv_launches": 20,
-                      "vcycle_applications": 20, "copy_bytes": a.numel() * 4,
-                      "copy_launches": 5}))
+                      "vcycle_applications": 20, "n_4k": n4, "spmv_4k_launches": 20,
+                      "copy_bytes": a.numel() * 4, "copy_launches": 5}))
 
 
 def _cpu_model():
@@ -361,11 +418,22 @@ def bench_c4(args):
     alpha_d, st_d, t_d = timed("f64", "stencil", rtol=1e-7)
     alpha_s, st_s, t_s = timed("mixed", "stencil", rtol=1e-7, inner_rtol=INNER)
     alpha_f, st_f, t_f = timed("mixed", "matrix_free", rtol=1e-7, inner_rtol=INNER)
+    # The value form on the two matrix-free level-0 operators.
+    alpha_fw, st_fw, t_fw = timed("mixed", "matrix_free", rtol=4e-10, mixed="reliable")
+    alpha_fr, st_fr, t_fr = timed("mixed", "matrix_free_recompute", rtol=4e-10, mixed="reliable")
+    from pymatting_amd import _ffi
+    L = _ffi.lib()
+    passes = {}
+    for name, level0, prefix in (("stored_stencil", "stencil", "dia"),
+                                 ("matrix_free_stored_windows", "matrix_free", "cf"),
+                                 ("matrix_free_recomputed_windows", "matrix_free_recompute",
+                                  "cfrec")):
+        passes[name] = fine_passes(profile_classes(L, lambda: run(
+            "mixed", level0, rtol=4e-10, mixed="reliable")), prefix)
     conv = pm.LambdaSolver(image, trimap, precision="f64", level0="stencil")
     alpha_c = conv.solve(rtol=1e-11, maxiter=100000)
     # The float32 stored-stencil SpMV at 4K: 431 MB of coefficients, more
     # than the 256 MB MALL, so every launch streams from HBM.
-    from pymatting_amd import _ffi
     xs = torch.rand(n, device="cuda", dtype=torch.float32)
     ys = torch.empty_like(xs)
     t_spmv = time_spmv(_ffi.lib(), conv.op.handle, _ffi.HM_F32, xs, ys)
@@ -424,6 +492,17 @@ def bench_c4(args):
                                         "ms_per_step": round(t_f * 1e3, 3), "rtol": 1e-7,
                                         "iterations": st_f["iterations"],
                                         "max_abs_dalpha_vs_converged": err(alpha_f)[0]},
+        "level0_forms_reliable_rtol_4e-10": {
+            "stored_stencil": {"value": round(n / t_rs / 1e6, 3),
+                               "iterations": st_rs["iterations"],
+                               "max_abs_dalpha_vs_converged": err(alpha_rs)[0]},
+            "matrix_free_stored_windows": {"value": round(n / t_fw / 1e6, 3),
+                                           "iterations": st_fw["iterations"],
+                                           "max_abs_dalpha_vs_converged": err(alpha_fw)[0]},
+            "matrix_free_recomputed_windows": {"value": round(n / t_fr / 1e6, 3),
+                                               "iterations": st_fr["iterations"],
+                                               "max_abs_dalpha_vs_converged": err(alpha_fr)[0]},
+            "fine_passes_f32": passes},
         "foreground_f32": {"ms": round(t_fg * 1e3, 3), "Mpix_per_s": round(n / t_fg / 1e6, 2),
                            "max_abs_dF_vs_f64": dF, "max_abs_dB_vs_f64": dB},
         "spmv_f32_4k": {"bytes_per_launch": int(spmv_bytes_per_point(4) * n),
@@ -680,6 +759,7 @@ def main():
     FP64_STRICT = dict(FP64, rtol=1e-9)
     MIXED_REFINE = dict(precision="mixed", level0="stencil", rtol=1e-7, inner_rtol=1e-3)
     MIXED_MF = dict(MIXED, level0="matrix_free")
+    MIXED_MFR = dict(MIXED, level0="matrix_free_recompute")
 
     last = {}
 
@@ -691,9 +771,7 @@ def main():
         last["solver"] = solver
         return alpha, solver.stats
 
-    def timed(cfg, steps, prof=False):
-        if prof:
-            L.hm_prof_set(1)
+    def timed(cfg, steps):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
@@ -703,7 +781,7 @@ def main():
         return alpha, st, t
 
     for _ in range(args.warmup):
-        for cfg in (MIXED, FP64, MIXED_STRICT, FP64_STRICT, MIXED_REFINE, MIXED_MF):
+        for cfg in (MIXED, FP64, MIXED_STRICT, FP64_STRICT, MIXED_REFINE, MIXED_MF, MIXED_MFR):
             step(cfg)
     torch.cuda.synchronize()
 
@@ -727,23 +805,17 @@ def main():
     # Separate pass with per-launch events (they would perturb the timing).
     def profile(cfg):
         """Per-class event timing of one extra step of cfg."""
-        timed(cfg, 1, prof=True)
-        out = {}
-        for k, kind in enumerate(KINDS):
-            for prec, s in ((0, 8), (1, 4)):
-                ms, cnt, pts = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64()
-                L.hm_prof_query(k, prec, ctypes.byref(ms), ctypes.byref(cnt), ctypes.byref(pts))
-                if cnt.value:
-                    out[(kind, prec)] = (ms.value, cnt.value, pts.value, s)
-        L.hm_prof_set(0)
-        return out
+        return profile_classes(L, lambda: step(cfg))
 
     prof = profile(MIXED)
     prof64 = profile(FP64)
+    prof_mf = profile(MIXED_MF)
+    prof_mfr = profile(MIXED_MFR)
     alpha_d, st_d, t_f64 = timed(FP64, args.steps)
     alpha_ms, st_ms, t_ms = timed(MIXED_STRICT, args.steps)
     alpha_ds, st_ds, t_ds = timed(FP64_STRICT, args.steps)
     alpha_f, st_f, t_mf = timed(MIXED_MF, args.steps)
+    alpha_fr, st_fr, t_mfr = timed(MIXED_MFR, args.steps)
     alpha_x, st_x, t_rf = timed(MIXED_REFINE, args.steps)
 
     # The value form from host buffers (numpy in, numpy out), as the reference's
@@ -780,6 +852,7 @@ def main():
     e_m, e_d, e_f, e_x = err(alpha_m), err(alpha_d), err(alpha_f), err(alpha_x)
     e_ms, e_ds = err(alpha_ms), err(alpha_ds)
     v_m, v_f, v_x = err(alpha_m, alpha_d), err(alpha_f, alpha_d), err(alpha_x, alpha_d)
+    e_fr, v_fr = err(alpha_fr), err(alpha_fr, alpha_d)
 
     bytes_launch = spmv_bytes_per_point(4) * n
     achieved = bytes_launch / t_spmv32 / 1e9
@@ -808,6 +881,14 @@ def main():
 
     breakdown, step_bytes = summarise(prof)
     breakdown64, step_bytes64 = summarise(prof64)
+    # The three forms of the finest-grid operator in the value form: stored
+    # 13-slot stencil, matrix-free with stored window planes, matrix-free with
+    # the windows recomputed per pass.
+    level0_forms = {
+        "stored_stencil": fine_passes(prof, "dia"),
+        "matrix_free_stored_windows": fine_passes(prof_mf, "cf"),
+        "matrix_free_recomputed_windows": fine_passes(prof_mfr, "cfrec"),
+    }
     # The solve's own kernels (fine grid: the fused SpMV with the direction
     # update, the pre-smooth + residual + restriction, the prolongation +
     # post-smooth; the first Galerkin grid; the coarser grids) as
@@ -877,6 +958,11 @@ def main():
                               "iterations": st_f["iterations"],
                               "max_abs_dalpha_vs_fp64": v_f[0],
                               "max_abs_dalpha_vs_converged": e_f[0]},
+        "mixed_matrix_free_recompute": {"value": round(n / t_mfr / 1e6, 3), "unit": "Mpix/s",
+                                        "iterations": st_fr["iterations"],
+                                        "max_abs_dalpha_vs_fp64": v_fr[0],
+                                        "max_abs_dalpha_vs_converged": e_fr[0]},
+        "fine_passes_by_level0_form_f32": level0_forms,
         "mixed_host_buffers": {"value": round(n / t_host / 1e6, 3), "unit": "Mpix/s",
                                "ms_per_step": round(t_host * 1e3, 3),
                                "max_abs_dalpha_vs_device": float(np.abs(

@@ -83,6 +83,15 @@ int hm_cf_assemble_csr(hm_cf* op, double* values, int64_t* indices, int64_t* ind
                        void* stream);
 /* y = A x with the matrix-free stencil; prec selects float64/float32 vectors. */
 int hm_cf_apply(hm_cf* op, int prec, const void* x, void* y, void* stream);
+/* Where the float32 matrix-free passes of this operator (hm_cf_apply with
+ * HM_F32, float32 V-cycles and the mixed solves over it) take each 3x3
+ * window's mean and inverse covariance from: 0 (default) the float64 values
+ * computed once at creation (the quantities of cf_laplacian.py:38-92) rounded
+ * to float32 and read per pass; 1 recomputed per pass from the image tile the
+ * pass stages, in float64 arithmetic, then rounded to float32. float64 passes
+ * always use the stored values. No counterpart in the reference, which only
+ * assembles the matrix. */
+int hm_cf_set_windows(hm_cf* op, int recompute);
 /* d = diag(A) (float64). */
 int hm_cf_diagonal(hm_cf* op, double* d, void* stream);
 /* Right-hand side: lambda*is_fg (HM_OP_LAMBDA, make_linear_system,
@@ -419,6 +428,10 @@ int hm_fb_ml_noalloc(const void* image, const void* alpha, int h0, int w0, int n
 #define HM_PROF_LEVEL1_RESTRICT 19
 #define HM_PROF_LEVEL1_SMOOTH 20
 #define HM_PROF_LEVEL1_RESIDUAL 21
+#define HM_PROF_CFREC_APPLY 22     /* matrix-free level 0, windows recomputed per pass */
+#define HM_PROF_CFREC_RESTRICT 23
+#define HM_PROF_CFREC_SMOOTH 24
+#define HM_PROF_CFREC_RESIDUAL 25
 int hm_prof_set(int enabled);
 /* Mean duration (microseconds) of reps back-to-back launches of the finest
  * grid's pre-smooth + residual + restriction and of its prolongation +

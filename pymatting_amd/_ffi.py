@@ -73,6 +73,7 @@ _SIGS = {
     "hm_cf_system_indptr": (i32, [P, P, ctypes.POINTER(i64), P]),
     "hm_cf_system_fill": (i32, [P, P, P, P, P]),
     "hm_cf_apply": (i32, [P, i32, P, P, P]),
+    "hm_cf_set_windows": (i32, [P, i32]),
     "hm_cf_diagonal": (i32, [P, P, P]),
     "hm_cf_rhs": (i32, [P, P, P]),
     "hm_cf_adjoint_image": (i32, [P, P, P, P, i32, P]),

@@ -150,9 +150,17 @@ class LambdaSolver:
         self.classes, self.tdev = device_classes(trimap)
         self.cf = CfOperator(image, epsilon, 1, "lambda", self.classes, self.tdev, lambda_value)
         self.b = self.cf.rhs()
-        # level0 "matrix_free": window stencil recomputed from the image on
-        # every pass; "stencil": the 25-point operator assembled once (exactly)
-        # and streamed (the stored ELL/DIA form).
+        # level0 "matrix_free": the stencil formed from the image and the
+        # stored window means and inverse covariances on every pass;
+        # "matrix_free_recompute": float32 passes also recompute the windows
+        # from the image tile (float64 arithmetic); "stencil": the 25-point
+        # operator assembled once (exactly) and streamed (the stored ELL/DIA
+        # form).
+        if level0 not in ("matrix_free", "matrix_free_recompute", "stencil"):
+            raise ValueError("level0 must be 'matrix_free', 'matrix_free_recompute' or "
+                             "'stencil'")
+        if level0 == "matrix_free_recompute":
+            self.cf.set_windows(True)
         self.op = DiaOperator.from_cf(self.cf) if level0 == "stencil" else self.cf
         self.M = VCycle(self.op, (self.h, self.w), num_pre_iter, num_post_iter, omega,
                         direct_solve_size, "f64" if precision == "f64" else "f32",

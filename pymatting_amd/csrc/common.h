@@ -139,6 +139,7 @@ enum ProfKind {
     kProfCgUpdateX64,  // float32 CG update with the solution in float64
     kProfCgUpdateR,    // CG update of r alone (x updated beside the V-cycle tail)
     kProfLevel1Apply, kProfLevel1Restrict, kProfLevel1Smooth, kProfLevel1Residual,
+    kProfCfRecApply, kProfCfRecRestrict, kProfCfRecSmooth, kProfCfRecResidual,
     kProfCount
 };
 

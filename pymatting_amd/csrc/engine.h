@@ -62,17 +62,86 @@ struct CfShared {
 // Both precisions store the window mean and inverse covariance (9 values),
 // float32 ones rounded from the float64 values: recomputing the mean in
 // float32 perturbs nearly singular windows far more than the rounding.
-template <typename T, int OX, int OY>
+// Rec: the windows are recomputed from the staged image instead (below), and
+// only their geometry is kept.
+template <typename T, int OX, int OY, bool Rec = false>
 struct CfPrefetch {
     static constexpr int NW = ((OX + 2) * (OY + 2) + kBlock - 1) / kBlock;
-    static constexpr int NQ = 9;
-    T cf[NW][NQ];
+    static constexpr int NQ = Rec ? 0 : 9;
+    T cf[NW][Rec ? 1 : 9];
     bool geo[NW];
 };
 
+// Mean and inverse regularised covariance of a 3x3 window from its nine
+// (float32) pixels, in float64 and in the operation order of
+// cf_laplacian.py:38-92 (the order of the stored window planes): window sum
+// and mean, centred colours, covariance with eps, the negated determinant and
+// its cofactors, no fused multiply-adds. Windows with two eigenvalues near
+// eps / 9 make the inverse sensitive to the last bit of the float64
+// arithmetic (a product with 1/9 in place of the division moves entries by
+// 1e-5 relative there), so any other order would give a different operator
+// from the stored and assembled ones; with this one, an image whose values
+// are float32 numbers gets exactly the stored windows.
 template <typename T>
+__device__ __forceinline__ void cf_window_f64(const Vec4<T> (&p)[9], double eps, T (&mu)[3],
+                                              T (&m)[6]) {
+#pragma clang fp contract(off)
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        s0 += (double)p[j].a;
+        s1 += (double)p[j].b;
+        s2 += (double)p[j].c;
+    }
+    const double u0 = s0 / 9, u1 = s1 / 9, u2 = s2 / 9;
+    double a00 = eps, a01 = 0.0, a02 = 0.0, a11 = eps, a12 = 0.0, a22 = eps;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        const double c0 = (double)p[j].a - u0, c1 = (double)p[j].b - u1;
+        const double c2 = (double)p[j].c - u2;
+        a00 += c0 * c0;
+        a01 += c0 * c1;
+        a02 += c0 * c2;
+        a11 += c1 * c1;
+        a12 += c1 * c2;
+        a22 += c2 * c2;
+    }
+    a00 /= 9;
+    a01 /= 9;
+    a02 /= 9;
+    a11 /= 9;
+    a12 /= 9;
+    a22 /= 9;
+    const double det = a00 * a12 * a12 + a01 * a01 * a22 + a02 * a02 * a11 - a00 * a11 * a22 -
+                       2 * a01 * a02 * a12;
+    const double inv_det = 1.0 / det;
+    mu[0] = (T)u0;
+    mu[1] = (T)u1;
+    mu[2] = (T)u2;
+    m[0] = (T)((a12 * a12 - a11 * a22) * inv_det);
+    m[1] = (T)((a01 * a22 - a02 * a12) * inv_det);
+    m[2] = (T)((a02 * a11 - a01 * a12) * inv_det);
+    m[3] = (T)((a02 * a02 - a00 * a22) * inv_det);
+    m[4] = (T)((a00 * a12 - a01 * a02) * inv_det);
+    m[5] = (T)((a01 * a01 - a00 * a11) * inv_det);
+}
+
+// a * b + c with one rounding, written out so that every instantiation of
+// the matrix-free operator rounds alike (the compiler's own contraction
+// choices depend on where the operands come from, and the operator's nearly
+// singular windows turn a different choice into a visible difference).
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmadd(double a, double b, double c) {
+    return __builtin_fma(a, b, c);
+}
+
+// Rec = false: the window data are read from W. Rec = true (float32 passes):
+// every pass recomputes each window's mean and inverse covariance from the
+// image it stages anyway (cf_window_f64), so a pass reads the image, the
+// class map and its vectors and no window planes.
+template <typename T, bool Rec = false>
 struct CfOp {
-    static constexpr int kProfBase = kProfCfApply;
+    static constexpr int kProfBase = Rec ? kProfCfRecApply : kProfCfApply;
     CfDev d;
     const T* I;  // planar [3][n]
     // Per-window data at the centre pixel [9][n]: mean (3), inverse covariance (6).
@@ -101,7 +170,7 @@ struct CfOp {
     }
 
     template <int OX, int OY>
-    using Prefetch = CfPrefetch<T, OX, OY>;
+    using Prefetch = CfPrefetch<T, OX, OY, Rec>;
 
     // Loads the coefficients of this thread's windows into registers; issued
     // before the region loads so both latencies overlap.
@@ -118,6 +187,8 @@ struct CfOp {
             const bool ok = t < WX * WY && cx >= 1 && cx <= d.w - 2 && cy >= 1 && cy <= d.h - 2;
             pf.geo[k] = ok;
             const long long gc = ok ? (long long)cy * d.w + cx : 0;
+            (void)gc;
+            (void)n;
 #pragma unroll
             for (int q = 0; q < NQ; ++q) pf.cf[k][q] = ok ? W[q * n + gc] : T(0);
         }
@@ -126,6 +197,7 @@ struct CfOp {
     // Window records for all (OX+2) x (OY+2) windows touching the tile.
     template <int OX, int OY>
     __device__ void prepare(Shared<OX, OY>& sh, const Prefetch<OX, OY>& pf) const {
+#pragma clang fp contract(off)
         constexpr int WX = OX + 2, WY = OY + 2;
         constexpr int NW = Prefetch<OX, OY>::NW, NQ = Prefetch<OX, OY>::NQ;
         constexpr int M0 = NQ - 6;  // index of m00
@@ -146,8 +218,10 @@ struct CfOp {
             Vec4<T> p[9];
 #pragma unroll
             for (int j = 0; j < 9; ++j) p[j] = sh.px[wy + j / 3][wx + j % 3];
-            T mu[3];
-            if constexpr (NQ == 9) {
+            T mu[3], mw[6];
+            if constexpr (Rec) {
+                cf_window_f64<T>(p, d.eps, mu, mw);
+            } else if constexpr (NQ == 9) {
                 mu[0] = pf.cf[k][0];
                 mu[1] = pf.cf[k][1];
                 mu[2] = pf.cf[k][2];
@@ -169,23 +243,27 @@ struct CfOp {
 #pragma unroll
             for (int j = 0; j < 9; ++j) {
                 S += p[j].d;
-                T0 += (p[j].a - mu[0]) * p[j].d;
-                T1 += (p[j].b - mu[1]) * p[j].d;
-                T2 += (p[j].c - mu[2]) * p[j].d;
+                T0 = fmadd(p[j].a - mu[0], p[j].d, T0);
+                T1 = fmadd(p[j].b - mu[1], p[j].d, T1);
+                T2 = fmadd(p[j].c - mu[2], p[j].d, T2);
             }
             // Invalid windows get all-zero records, so the output loop needs
             // no branch.
             const T okf = valid ? T(1) : T(0);
-            const T m00 = pf.cf[k][M0], m01 = pf.cf[k][M0 + 1], m02 = pf.cf[k][M0 + 2];
-            const T m11 = pf.cf[k][M0 + 3], m12 = pf.cf[k][M0 + 4], m22 = pf.cf[k][M0 + 5];
+            if constexpr (!Rec) {
+#pragma unroll
+                for (int q = 0; q < 6; ++q) mw[q] = pf.cf[k][M0 + q];
+            }
+            const T m00 = mw[0], m01 = mw[1], m02 = mw[2];
+            const T m11 = mw[3], m12 = mw[4], m22 = mw[5];
             Vec4<T> r0, r1;
             r0.a = okf * S * inv9;
             r0.b = mu[0];
             r0.c = mu[1];
             r0.d = mu[2];
-            r1.a = okf * (m00 * T0 + m01 * T1 + m02 * T2) * inv9;
-            r1.b = okf * (m01 * T0 + m11 * T1 + m12 * T2) * inv9;
-            r1.c = okf * (m02 * T0 + m12 * T1 + m22 * T2) * inv9;
+            r1.a = okf * fmadd(m02, T2, fmadd(m01, T1, m00 * T0)) * inv9;
+            r1.b = okf * fmadd(m12, T2, fmadd(m11, T1, m01 * T0)) * inv9;
+            r1.c = okf * fmadd(m22, T2, fmadd(m12, T1, m02 * T0)) * inv9;
             r1.d = okf;
             sh.win0[wy][wx] = r0;
             sh.win1[wy][wx] = r1;
@@ -197,6 +275,7 @@ struct CfOp {
     __device__ __forceinline__ T apply(const Shared<OX, OY>& sh, const Prefetch<OX, OY>&,
                                        const T (&)[OY + 4][OX + 4], int, int lx, int ly, T xi,
                                        long long) const {
+#pragma clang fp contract(off)
         const int m = mode();
         const uint8_t ci = sh.cls[ly + 2][lx + 2];
         const Vec4<T> p = sh.px[ly + 2][lx + 2];
@@ -207,14 +286,15 @@ struct CfOp {
             for (int dx = 0; dx < 3; ++dx) {
                 const Vec4<T> r0 = sh.win0[ly + dy][lx + dx];
                 const Vec4<T> r1 = sh.win1[ly + dy][lx + dx];
-                acc -= r0.a + (p.a - r0.b) * r1.a + (p.b - r0.c) * r1.b + (p.c - r0.d) * r1.c;
+                acc -= fmadd(p.c - r0.d, r1.c,
+                             fmadd(p.b - r0.c, r1.b, fmadd(p.a - r0.b, r1.a, r0.a)));
                 cnt += r1.d;
             }
         }
         // Invalid windows have zero records; each valid one adds x_i.
-        acc += cnt * xi;
+        acc = fmadd(cnt, xi, acc);
         if (m == HM_OP_MASKED && ci != 0) return xi;
-        if (m == HM_OP_LAMBDA && ci != 0) acc += T(d.lambda) * xi;
+        if (m == HM_OP_LAMBDA && ci != 0) acc = fmadd(T(d.lambda), xi, acc);
         return acc;
     }
 };

@@ -24,6 +24,7 @@ struct hm_cf {
     double* idiag64 = nullptr;  // 1 / diag (float32 copy rounded from float64)
     float* idiag32 = nullptr;
     int ready = 0;  // kCf* parts computed so far (cf_prepare)
+    int recompute = 0;  // float32 passes recompute the windows from the image
 };
 
 // Stored symmetric 25-point stencil (slot-major [13][n], see engine.h) with

@@ -1258,6 +1258,15 @@ int hm_cf_apply(hm_cf* op, int prec, const void* x, void* y, void* stream) {
                            s);
 }
 
+int hm_cf_set_windows(hm_cf* op, int recompute) {
+    if (!op || (recompute != 0 && recompute != 1)) {
+        set_error("hm_cf_set_windows: invalid argument");
+        return HM_ERR_ARG;
+    }
+    op->recompute = recompute;
+    return HM_OK;
+}
+
 int hm_cf_diagonal(hm_cf* op, double* d, void* stream) {
     if (!op || !d) {
         set_error("hm_cf_diagonal: null argument");

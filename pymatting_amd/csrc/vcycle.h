@@ -29,7 +29,7 @@ int launch_tile(const Op& op, const In& in, const Out& out, int h, int w, int h2
     const int base = level == 0 ? (int)Op::kProfBase
                                 : (level == 1 ? (int)kProfLevel1Apply : (int)kProfCoarseApply);
     ProfScope prof(base + Out::kProf, (long long)h * w, sizeof(T) == 4, s);
-    if constexpr (std::is_same<Op, CfOp<T>>::value) {
+    if constexpr (std::is_same<Op, CfOp<T>>::value || std::is_same<Op, CfOp<T, true>>::value) {
         tile_kernel<T, Op, In, Out, 0><<<map.ntiles, kBlock, 0, s>>>(op, in, out, h, w, guard, map);
     } else {
         if (level == 0)
@@ -51,6 +51,7 @@ int launch_tile(const Op& op, const In& in, const Out& out, int h, int w, int h2
 template <typename T>
 struct LevelOp {
     bool is_cf = false;
+    bool cf_rec = false;  // float32 matrix-free passes recompute the windows
     int level = 0;  // place in the hierarchy (profiling classes and kernel names only)
     CfOp<T> cf{};
     DiaOp<T> dia{};
@@ -73,6 +74,7 @@ LevelOp<T> cf_level(const hm_cf* op, int force_mode = -1) {
     L.cf.I = pick<T>(op->I64, op->I32);
     L.cf.W = pick<T>(op->wc, op->wc32);
     L.cf.force_mode = force_mode;
+    L.cf_rec = sizeof(T) == 4 && op->recompute;
     L.iD = pick<T>(op->idiag64, op->idiag32);
     L.h = op->h;
     L.w = op->w;
@@ -93,8 +95,15 @@ LevelOp<T> dia_level(const hm_dia* A) {
 template <typename T, class In, class Out>
 int run_level(const LevelOp<T>& L, const In& in, const Out& out, const CgState* guard,
               hipStream_t s) {
-    if (L.is_cf)
+    if (L.is_cf) {
+        if constexpr (sizeof(T) == 4) {
+            if (L.cf_rec) {
+                const CfOp<T, true> rec{L.cf.d, L.cf.I, nullptr, L.cf.force_mode};
+                return launch_tile<T>(rec, in, out, L.h, L.w, L.h / 2, L.w / 2, guard, s);
+            }
+        }
         return launch_tile<T>(L.cf, in, out, L.h, L.w, L.h / 2, L.w / 2, guard, s);
+    }
     // Coarse grids below kSmallGrid points are latency-bound: one point per
     // thread with its coefficients loaded before the tile barrier. Larger
     // grids keep the bandwidth-oriented shape of the finest level.
@@ -323,7 +332,10 @@ int post_stage(const LevelOp<T>& op, const Base& base, const T* b, const T* e, i
 constexpr int kTailMaxLevels = 24;
 constexpr int kTailBlock = 1024;
 constexpr int kTailLdsBytes = 152 * 1024;
-constexpr int kTailMaxTop = 1024;  // largest top grid of a tail (points)
+#ifndef HM_TAIL_MAX_TOP
+#define HM_TAIL_MAX_TOP 1024
+#endif
+constexpr int kTailMaxTop = HM_TAIL_MAX_TOP;  // largest top grid of a tail (points)
 
 template <typename T>
 struct TailLevel {

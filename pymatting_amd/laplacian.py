@@ -74,6 +74,13 @@ class CfOperator:
                                           _ffi.stream()))
         return out
 
+    def set_windows(self, recompute):
+        """recompute=True: float32 passes recompute every window's mean and
+        inverse covariance from the image tile they stage (float64
+        arithmetic) instead of reading the stored float32 values."""
+        _ffi.check(_ffi.lib().hm_cf_set_windows(self.handle, int(bool(recompute))))
+        return self
+
     def dot(self, x):
         xd = as_device(x, torch.float64).reshape(-1)
         if xd.numel() != self.n:
