@@ -355,6 +355,25 @@ def bench_c3(args, forms_run=("f64", "mixed", "f64_rtol_1e-7", "mixed_rtol_1e-7"
     for form, f64 in (("mixed", "f64"), ("mixed_rtol_1e-7", "f64_rtol_1e-7")):
         res[form]["max_abs_dalpha_vs_f64"] = max(
             float((a - r).abs().max()) for a, r in zip(batched_alphas[form], batched_alphas[f64]))
+    # Launch per step against one persistent kernel: a kernel boundary and a
+    # grid-wide barrier (cooperative grid.sync and an atomic counter barrier)
+    # for grids of 8, 64 and 256 workgroups.
+    from pymatting_amd import _ffi
+    L = _ffi.lib()
+    sync = {}
+    for G in (8, 64, 256):
+        lu, cu, au, ok = ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        _ffi.check(L.hm_time_sync(G, 2000, ctypes.byref(lu), ctypes.byref(cu), ctypes.byref(au),
+                                  ctypes.byref(ok), _ffi.stream()))
+        sync[str(G)] = {"kernel_boundary_us": round(lu.value, 3),
+                        "coop_grid_sync_us": round(cu.value, 3),
+                        "counter_barrier_us": round(au.value, 3), "barrier_ok": bool(ok.value)}
+    graph_vs_direct = None
+    tm = res["mixed"]["autotuned"]["timings_s"]
+    if tm and tm.get("graph_iters_s"):
+        g = tm["graph_iters_s"]
+        graph_vs_direct = {"direct_ms": round(g[0] * 1e3, 3) if 0 in g else None,
+                           "graph_ms": round(g[8] * 1e3, 3) if 8 in g else None}
     line = {
         "metric": "batched alpha Mpix/s, 64 images 256..1024 px (mixed form)",
         "value": res["mixed"]["batched_Mpix_per_s"], "unit": "Mpix/s", "n_gpus": 1,
@@ -368,6 +387,11 @@ def bench_c3(args, forms_run=("f64", "mixed", "f64_rtol_1e-7", "mixed_rtol_1e-7"
                    "images": len(images), "total_pixels": npix},
         "f64": res["f64"], "mixed": res["mixed"], "mixed_refine": res.get("mixed_refine"),
         "rtol_1e-7": {"f64": res["f64_rtol_1e-7"], "mixed": res["mixed_rtol_1e-7"]},
+        # Whole batched value-form solves (autotune timings, best of one):
+        # each iteration's fixed parts replayed from HIP graphs, or launched
+        # directly.
+        "value_form_graph_vs_direct": graph_vs_direct,
+        "kernel_boundary_vs_grid_barrier": sync,
     }
     return line
 

@@ -337,12 +337,27 @@ int hm_batch_solve(hm_batch* b, const double* const* rhs, double* const* x, doub
 /* The mixed form with reliable updates (see hm_cg_solve_reliable) for a batch
  * created with precision HM_F32: one float32 CG per image with its solution in
  * float64; residual replacement per image after a drop by delta; every
- * image's convergence is confirmed on its float64 residual. Launches are
- * direct (no graph). */
+ * image's convergence is confirmed on its float64 residual. Iterations run in
+ * units of 8 between state read-backs. graph = 0 launches every kernel
+ * directly; graph = 1 records each unit iteration's two fixed parts (SpMV,
+ * finalize and update; the V-cycle) once into HIP graphs and replays them,
+ * with the per-unit residual replacements launched directly between them.
+ * Both give bit-identical results. */
 int hm_batch_solve_reliable(hm_batch* b, const double* const* rhs, double* const* x,
-                            double rtol, double delta, int maxiter, hm_cg_stats* stats,
-                            void* stream);
+                            double rtol, double delta, int maxiter, int graph,
+                            hm_cg_stats* stats, void* stream);
 int hm_batch_destroy(hm_batch* b);
+
+/* ---- kernel boundary vs grid barrier (no reference counterpart) ----
+ * Measures, on `stream`, for a grid of `workgroups` workgroups of 256 threads
+ * (at most one per compute unit): the mean time per launch of `reps`
+ * back-to-back empty launches (*launch_us), and per barrier of `reps`
+ * grid-wide barriers inside one launch, cooperative groups' grid.sync()
+ * (*coop_us) and an atomic counter barrier with a bounded spin (*counter_us;
+ * *ok = 0 if a spin ran out). These are the two ways the solve could separate
+ * its steps: launches per step, or one persistent kernel. */
+int hm_time_sync(int workgroups, int reps, double* launch_us, double* coop_us,
+                 double* counter_us, int* ok, void* stream);
 
 /* ---- adjoint of the alpha solve (no reference counterpart: pymatting is
  * not differentiable; this is the image-gradient step of the adjoint method

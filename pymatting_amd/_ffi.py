@@ -106,8 +106,9 @@ _SIGS = {
                                P]),
     "hm_batch_create": (i32, [i32, P, P, i32, i32, i32, ctypes.POINTER(P)]),
     "hm_batch_solve": (i32, [P, P, P, f64, f64, i32, i32, i32, P, P]),
-    "hm_batch_solve_reliable": (i32, [P, P, P, f64, f64, i32, P, P]),
+    "hm_batch_solve_reliable": (i32, [P, P, P, f64, f64, i32, i32, P, P]),
     "hm_batch_destroy": (i32, [P]),
+    "hm_time_sync": (i32, [i32, i32, P, P, P, P, P]),
     "hm_ichol_factor": (i32, [i64, P, P, P, f64, f64, i64, f64, i32, ctypes.POINTER(i64),
                               ctypes.POINTER(P)]),
     "hm_ichol_create": (i32, [i64, P, P, P, ctypes.POINTER(P)]),

@@ -106,7 +106,8 @@ class BatchSolver:
         stats = (_ffi.CgStats * self.count)()
         if self.precision != "f64" and mixed == "reliable":
             rc = _ffi.lib().hm_batch_solve_reliable(h, bp, xp, float(rtol), float(delta),
-                                                    int(maxiter), stats, _ffi.stream())
+                                                    int(maxiter), int(self.graph_iters > 0),
+                                                    stats, _ffi.stream())
         elif self.precision == "f64" or mixed == "refine":
             rc = _ffi.lib().hm_batch_solve(h, bp, xp, float(rtol), float(inner_rtol),
                                            int(max_outer), int(maxiter), int(self.graph_iters),
@@ -128,8 +129,9 @@ class BatchSolver:
                  delta=0.01):
         """Chooses tile shapes, then iterations per graph, by timing full solves
         of this batch with the given solve options; cached per batch signature
-        and form. Reliable updates run without graphs: only the shapes are
-        tuned (graph_iters is reported as 0). Returns the choice."""
+        and form. For reliable updates the second choice is graph replay of
+        each iteration's fixed parts (graph_iters 8) or direct launches (0).
+        Returns the choice."""
         reliable = self.precision != "f64" and mixed == "reliable"
         key = (self.signature(), "reliable" if reliable else "refine")
         if key in _TUNED:
@@ -158,11 +160,10 @@ class BatchSolver:
                 results[(s0, sc)] = timed()
         self.level0_shape, self.coarse_shape = min(results, key=results.get)
         gtimes = {}
-        if not reliable:
-            for g in GRAPH_ITERS:
-                self.graph_iters = g
-                gtimes[g] = timed()
-            self.graph_iters = min(gtimes, key=gtimes.get)
+        for g in ((0, 8) if reliable else GRAPH_ITERS):
+            self.graph_iters = g
+            gtimes[g] = timed()
+        self.graph_iters = min(gtimes, key=gtimes.get)
         self.tuning = {"shapes_s": {"%d,%d" % k: v for k, v in results.items()},
                        "graph_iters_s": gtimes}
         _TUNED[key] = (self.level0_shape, self.coarse_shape, self.graph_iters)

@@ -30,6 +30,7 @@ SOURCES = {
     "sparse.hip": [],
     "fg_cf.hip": ["-ffp-contract=off"],
     "lkm.hip": [],
+    "sync_probe.hip": [],
 }
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -969,7 +969,8 @@ int batch_pcg(hm_batch* B, BatchPlan<T, S0, SC>& plan, BatchWork<T>& wk, Arena& 
 // separate state array, so the other images are not touched.
 template <int S0, int SC>
 int batch_reliable(hm_batch* B, Arena& ar, const double* const* b, double* const* x,
-                   double rtol, double delta, int maxiter, hm_cg_stats* stats, hipStream_t s) {
+                   double rtol, double delta, int maxiter, int graph, hm_cg_stats* stats,
+                   hipStream_t s) {
     constexpr int kUnit = 8;  // even: the direction buffers alternate
     const int count = B->count;
     BatchPlan<float, S0, SC> plan;
@@ -1164,6 +1165,46 @@ int batch_reliable(hm_batch* B, Arena& ar, const double* const* b, double* const
         HM_TRY(plan.iteration_steps_x64(head[k], tail[k], xv, wk.r, wk.z, wk.q, pc, pn,
                                         st.dev));
     }
+    // graph != 0: each iteration's two fixed parts (head: SpMV, finalize,
+    // update; tail: the V-cycle) are recorded once into a HIP graph each and
+    // replayed; the replacements between them stay direct launches.
+    std::vector<hipGraph_t> graphs;
+    std::vector<hipGraphExec_t> execs;
+    struct SegGuard {
+        std::vector<hipGraph_t>* g;
+        std::vector<hipGraphExec_t>* e;
+        ~SegGuard() {
+            for (hipGraphExec_t x : *e) (void)hipGraphExecDestroy(x);
+            for (hipGraph_t x : *g) (void)hipGraphDestroy(x);
+        }
+    } seg_guard{&graphs, &execs};
+    if (graph) {
+        for (int k = 0; k < 2 * kUnit; ++k) {
+            const std::vector<Step>& part = k % 2 == 0 ? head[k / 2] : tail[k / 2];
+            HM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            const int rc = run_steps(part, s);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(s, &g);
+            if (g) graphs.push_back(g);
+            HM_TRY(rc);
+            HM_HIP(e);
+            hipGraphExec_t x = nullptr;
+            HM_HIP(hipGraphInstantiate(&x, g, nullptr, nullptr, 0));
+            execs.push_back(x);
+        }
+    }
+    auto segment = [&](int k, bool is_tail, std::vector<Step>& unit) {
+        if (!graph) {
+            const std::vector<Step>& part = is_tail ? tail[k] : head[k];
+            unit.insert(unit.end(), part.begin(), part.end());
+            return;
+        }
+        const hipGraphExec_t x = execs[2 * k + (is_tail ? 1 : 0)];
+        unit.push_back([x](hipStream_t s2) {
+            HM_HIP(hipGraphLaunch(x, s2));
+            return (int)HM_OK;
+        });
+    };
     // Restart of a subset: z = M r32 with rz into rst (the others are done there).
     std::vector<Step> restart_vc;
     {
@@ -1199,14 +1240,14 @@ int batch_reliable(hm_batch* B, Arena& ar, const double* const* b, double* const
                 due[i] = rep_at[i] == k;
                 any = any || due[i];
             }
-            unit.insert(unit.end(), head[k].begin(), head[k].end());
+            segment(k, false, unit);
             if (any) {
                 const Span sp = append(due);
                 unit.push_back(gate_step(k));
                 unit.push_back(resid_step(sp));
                 unit.push_back(copy_step(sp));
             }
-            unit.insert(unit.end(), tail[k].begin(), tail[k].end());
+            segment(k, true, unit);
             if (any) unit.push_back(note_step());
         }
         HM_TRY(upload(0, 0));
@@ -1559,9 +1600,10 @@ int hm_batch_destroy(hm_batch* b) {
 }
 
 int hm_batch_solve_reliable(hm_batch* b, const double* const* rhs, double* const* x,
-                            double rtol, double delta, int maxiter, hm_cg_stats* stats,
-                            void* stream) {
-    if (!b || !rhs || !x || !stats || maxiter < 0 || !(delta > 0.0 && delta < 1.0)) {
+                            double rtol, double delta, int maxiter, int graph,
+                            hm_cg_stats* stats, void* stream) {
+    if (!b || !rhs || !x || !stats || maxiter < 0 || !(delta > 0.0 && delta < 1.0) ||
+        (graph != 0 && graph != 1)) {
         set_error("hm_batch_solve_reliable: invalid argument");
         return HM_ERR_ARG;
     }
@@ -1582,13 +1624,17 @@ int hm_batch_solve_reliable(hm_batch* b, const double* const* rhs, double* const
     Arena ar;  // destroyed after the stream has drained
     int rc;
     if (b->s0 == 0 && b->sc == 0)
-        rc = batch_reliable<0, 0>(b, ar, rhs, x, rtol, delta, maxiter, stats, b->stream);
+        rc = batch_reliable<0, 0>(b, ar, rhs, x, rtol, delta, maxiter, graph, stats,
+                                       b->stream);
     else if (b->s0 == 0)
-        rc = batch_reliable<0, 1>(b, ar, rhs, x, rtol, delta, maxiter, stats, b->stream);
+        rc = batch_reliable<0, 1>(b, ar, rhs, x, rtol, delta, maxiter, graph, stats,
+                                       b->stream);
     else if (b->sc == 0)
-        rc = batch_reliable<1, 0>(b, ar, rhs, x, rtol, delta, maxiter, stats, b->stream);
+        rc = batch_reliable<1, 0>(b, ar, rhs, x, rtol, delta, maxiter, graph, stats,
+                                       b->stream);
     else
-        rc = batch_reliable<1, 1>(b, ar, rhs, x, rtol, delta, maxiter, stats, b->stream);
+        rc = batch_reliable<1, 1>(b, ar, rhs, x, rtol, delta, maxiter, graph, stats,
+                                       b->stream);
     (void)hipStreamSynchronize(b->stream);
     if (hipEventRecord(b->ev_out, b->stream) == hipSuccess)
         (void)hipStreamWaitEvent(user, b->ev_out, 0);

@@ -74,8 +74,8 @@ def test_argument_checks_before_any_device_call():
     vcs = (P * 1)()
     assert lib.hm_batch_create(1, ops, vcs, 0, 2, 0, ctypes.byref(out)) == ARG
     assert "tile shapes" in _err(lib)
-    lib.hm_batch_solve_reliable.argtypes = [P, P, P, f64, f64, i32, P, P]
-    assert lib.hm_batch_solve_reliable(None, None, None, 1e-9, 0.01, 10, None, None) == ARG
+    lib.hm_batch_solve_reliable.argtypes = [P, P, P, f64, f64, i32, i32, P, P]
+    assert lib.hm_batch_solve_reliable(None, None, None, 1e-9, 0.01, 10, 0, None, None) == ARG
     # Foreground: the pyramid must run from 1x1 to the image size.
     lib.hm_fb_ml.argtypes = [P, P, i32, i32, i32, P, P, f64, i32, i32, i32, f64, i32, P, P, P]
     buf = ctypes.create_string_buffer(64)

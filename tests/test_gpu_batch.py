@@ -83,10 +83,11 @@ def test_batch_autotune_and_errors():
     assert choice[0] in (0, 1) and choice[1] in (0, 1) and choice[2] in (0, 2, 4, 8, 16)
     assert bs.autotune(rtol=1e-7) == choice  # cached
     bs.solve(rtol=1e-7)
-    # Reliable updates tune the shapes only (no graphs), cached per form.
+    # Reliable updates tune the shapes and graph replay (8) against direct
+    # launches (0), cached per form.
     m = pm.BatchSolver([d[0] for d in data], [d[1] for d in data], precision="mixed")
     rel = m.autotune(rtol=1e-9, mixed="reliable")
-    assert rel[0] in (0, 1) and rel[1] in (0, 1) and rel[2] == 0
+    assert rel[0] in (0, 1) and rel[1] in (0, 1) and rel[2] in (0, 8)
     assert m.autotune(rtol=1e-9, mixed="reliable") == rel
     for st in m.stats:
         assert st["converged"]
@@ -126,6 +127,38 @@ def test_batch_reliable_updates():
         assert _true_rel(s, x, s.b) < 1e-9
         assert abs(st["iterations"] - it) <= 2
         assert float((x.clamp(0, 1).reshape(c.shape) - c).abs().max()) < 1e-4
+
+
+def test_batch_reliable_graph_replay_equals_direct_launches():
+    """Reliable updates with each iteration's fixed parts replayed from HIP
+    graphs (residual replacements, state read-backs and restarts launched
+    directly between the replays) give bit-identical solutions and iteration
+    counts to direct launches, on two solves in a row of each kind, also when
+    images restart."""
+    import torch
+    import pymatting_amd as pm
+    from pymatting_amd import synthetic
+    data = _images()
+    im1, tr1, _ = synthetic.config1()
+    cases = [([d[0] for d in data], [d[1] for d in data], dict(rtol=1e-9)),
+             ([im1, data[2][0]], [tr1, data[2][1]], dict(rtol=1e-11, delta=1e-12, maxiter=2000))]
+    for images, trimaps, kw in cases:
+        bs = pm.BatchSolver(images, trimaps, precision="mixed")
+        bs.graph_iters = 0
+        x0 = [x.clone() for x in bs.solve_x(mixed="reliable", **kw)]
+        it0 = [st["iterations"] for st in bs.stats]
+        assert all(st["converged"] for st in bs.stats)
+        for rep in range(2):
+            bs.graph_iters = 8
+            xg = bs.solve_x(mixed="reliable", **kw)
+            print(kw, rep, [st["iterations"] for st in bs.stats], it0, flush=True)
+            assert [st["iterations"] for st in bs.stats] == it0
+            for a, b in zip(x0, xg):
+                assert torch.equal(a, b)
+            bs.graph_iters = 0
+            xd = bs.solve_x(mixed="reliable", **kw)
+            for a, b in zip(x0, xd):
+                assert torch.equal(a, b)
 
 
 def test_batch_reliable_restart_and_zero_rhs():
@@ -170,3 +203,23 @@ def test_batch_reliable_restart_and_zero_rhs():
         print("restarts", st, flush=True)
         assert st["converged"]
         assert _true_rel(s, x, s.b) < 1e-11
+
+
+def test_time_sync_probe():
+    """hm_time_sync: a kernel boundary and both grid barriers are timed (all
+    barriers complete), and grids larger than one workgroup per compute unit
+    are refused."""
+    import ctypes
+    from pymatting_amd import _ffi
+    L = _ffi.lib()
+    lu, cu, au, ok = ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+    for G in (8, 64):
+        _ffi.check(L.hm_time_sync(G, 200, ctypes.byref(lu), ctypes.byref(cu), ctypes.byref(au),
+                                  ctypes.byref(ok), _ffi.stream()))
+        print(G, lu.value, cu.value, au.value, ok.value, flush=True)
+        assert ok.value == 1
+        assert 0.0 < lu.value < 1e3 and 0.0 < cu.value < 1e3 and 0.0 < au.value < 1e3
+    assert L.hm_time_sync(100000, 10, ctypes.byref(lu), ctypes.byref(cu), ctypes.byref(au),
+                          ctypes.byref(ok), _ffi.stream()) == _ffi.HM_ERR_ARG
+    assert L.hm_time_sync(8, 0, ctypes.byref(lu), ctypes.byref(cu), ctypes.byref(au),
+                          ctypes.byref(ok), _ffi.stream()) == _ffi.HM_ERR_ARG
