@@ -57,19 +57,22 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of three float32 kernels at 1920x1080, from two
-# rocprofv3 --pmc passes over `python bench.py --pmc-probe` (FETCH_SIZE, then
-# WRITE_SIZE, KiB per dispatch), each scaled by what the same counter reads
-# for a 256 MiB copy (FETCH_SIZE x 1.999796, WRITE_SIZE x 1.0):
-#   fine-grid restriction: 62025.7 KiB x 1.999796 + 2025.0 KiB = 129.1 MB
-#     (algorithmic 126.5 MB);
-#   fine-grid prolongation + smoothing: 63358.6 KiB x 1.999796 + 8100.0 KiB
-#     = 138.0 MB (algorithmic 134.8 MB);
-#   stored-stencil SpMV: 57579.3 KiB x 1.999796 + 8100.0 KiB = 126.2 MB
-#     (algorithmic 124.4 MB).
-TRAFFIC_RESTRICT_F32 = 129089331
-TRAFFIC_SMOOTH_F32 = 138039603
-TRAFFIC_SPMV_F32 = 126204826
+# HBM bytes per launch of float32 kernels, from two rocprofv3 --pmc passes
+# over `python bench.py --pmc-probe` (FETCH_SIZE, then WRITE_SIZE, KiB per
+# dispatch), each scaled by what the same counter reads for a 256 MiB copy
+# (FETCH_SIZE x 1.999799, WRITE_SIZE x 1.0):
+#   1080p fine-grid restriction: 62031.1 KiB x 1.999799 + 2025.0 KiB
+#     = 129.1 MB (algorithmic 126.5 MB);
+#   1080p fine-grid prolongation + smoothing: 63410.0 KiB x 1.999799
+#     + 8100.0 KiB = 138.1 MB (algorithmic 134.8 MB);
+#   1080p stored-stencil SpMV: 57526.3 KiB x 1.999799 + 8100.0 KiB
+#     = 126.1 MB (algorithmic 124.4 MB);
+#   4K stored-stencil SpMV: 233140.9 KiB x 1.999799 + 32400.0 KiB
+#     = 510.6 MB (algorithmic 497.7 MB).
+TRAFFIC_RESTRICT_F32 = 129100557
+TRAFFIC_SMOOTH_F32 = 138145002
+TRAFFIC_SPMV_F32 = 126096506
+TRAFFIC_SPMV_F32_4K = 510602224
 
 # Algorithmic bytes per grid point and launch, by kernel class, for vector
 # element size s (8 float64, 4 float32). Matrix-free passes read the image
@@ -530,6 +533,7 @@ def bench_c4(args):
         "foreground_f32": {"ms": round(t_fg * 1e3, 3), "Mpix_per_s": round(n / t_fg / 1e6, 2),
                            "max_abs_dF_vs_f64": dF, "max_abs_dB_vs_f64": dB},
         "spmv_f32_4k": {"bytes_per_launch": int(spmv_bytes_per_point(4) * n),
+                        "traffic": TRAFFIC_SPMV_F32_4K,
                         "avg_launch_us": round(t_spmv * 1e6, 2),
                         "achieved_GBps": round(spmv_gbps, 1),
                         "frac": round(spmv_gbps / HBM_PEAK_GBS, 4)},
