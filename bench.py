@@ -788,6 +788,7 @@ def main():
     MIXED_REFINE = dict(precision="mixed", level0="stencil", rtol=1e-7, inner_rtol=1e-3)
     MIXED_MF = dict(MIXED, level0="matrix_free")
     MIXED_MFR = dict(MIXED, level0="matrix_free_recompute")
+    MIXED_G = dict(MIXED, graph=True)
 
     last = {}
 
@@ -809,7 +810,8 @@ def main():
         return alpha, st, t
 
     for _ in range(args.warmup):
-        for cfg in (MIXED, FP64, MIXED_STRICT, FP64_STRICT, MIXED_REFINE, MIXED_MF, MIXED_MFR):
+        for cfg in (MIXED, FP64, MIXED_STRICT, FP64_STRICT, MIXED_REFINE, MIXED_MF, MIXED_MFR,
+                    MIXED_G):
             step(cfg)
     torch.cuda.synchronize()
 
@@ -844,6 +846,7 @@ def main():
     alpha_ds, st_ds, t_ds = timed(FP64_STRICT, args.steps)
     alpha_f, st_f, t_mf = timed(MIXED_MF, args.steps)
     alpha_fr, st_fr, t_mfr = timed(MIXED_MFR, args.steps)
+    alpha_g, st_g, t_g = timed(MIXED_G, args.steps)
     alpha_x, st_x, t_rf = timed(MIXED_REFINE, args.steps)
 
     # The value form from host buffers (numpy in, numpy out), as the reference's
@@ -991,6 +994,12 @@ def main():
                                         "max_abs_dalpha_vs_fp64": v_fr[0],
                                         "max_abs_dalpha_vs_converged": e_fr[0]},
         "fine_passes_by_level0_form_f32": level0_forms,
+        # The value form with its iterations without a replacement replayed
+        # from HIP graphs instead of launched directly.
+        "mixed_graph_replay": {"value": round(n / t_g / 1e6, 3), "unit": "Mpix/s",
+                               "ms_per_step": round(t_g * 1e3, 3),
+                               "iterations": st_g["iterations"],
+                               "identical_to_direct": bool(torch.equal(alpha_g, alpha_m))},
         "mixed_host_buffers": {"value": round(n / t_host / 1e6, 3), "unit": "Mpix/s",
                                "ms_per_step": round(t_host * 1e3, 3),
                                "max_abs_dalpha_vs_device": float(np.abs(

@@ -272,9 +272,13 @@ int hm_cg_solve_mixed(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, doub
  * the last replacement; convergence (||b - A x|| < rtol ||b||) is confirmed
  * on a true residual. The state is read back every min(check_every, 8)
  * iterations (replacements are decided there). Same operator arguments as
- * hm_cg_solve_mixed. */
+ * hm_cg_solve_mixed. graph = 1 records an iteration without a replacement
+ * (SpMV, update, V-cycle) once per alternation of the direction buffers into
+ * a HIP graph and replays it, on a stream of the calling thread's own
+ * ordered after and before `stream`; results are bit-identical to graph = 0
+ * (direct launches). */
 int hm_cg_solve_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x,
-                         double rtol, double delta, int maxiter, int check_every,
+                         double rtol, double delta, int maxiter, int check_every, int graph,
                          hm_cg_stats* stats, void* stream);
 
 /* ---- ichol: pymatting/preconditioner/ichol.py:6-301 ----

@@ -94,7 +94,8 @@ _SIGS = {
     "hm_vcycle_apply": (i32, [P, P, P, P]),
     "hm_cg_solve": (i32, [ctypes.POINTER(CgArgs), ctypes.POINTER(CgStats), P]),
     "hm_cg_solve_mixed": (i32, [P, P, P, P, P, f64, f64, i32, i32, ctypes.POINTER(CgStats), P]),
-    "hm_cg_solve_reliable": (i32, [P, P, P, P, P, f64, f64, i32, i32, ctypes.POINTER(CgStats),
+    "hm_cg_solve_reliable": (i32, [P, P, P, P, P, f64, f64, i32, i32, i32,
+                                   ctypes.POINTER(CgStats),
                                    P]),
     "hm_diag_apply": (i32, [P, P, P, i64, P]),
     "hm_alpha_finalize": (i32, [P, P, P, P, i64, P]),

@@ -168,7 +168,7 @@ class LambdaSolver:
         self.stats = {}
 
     def solve_system(self, b, rtol=1e-7, atol=0.0, maxiter=10000, inner_rtol=1e-4,
-                     max_outer=20, check_every=8, mixed="refine", delta=0.01):
+                     max_outer=20, check_every=8, mixed="refine", delta=0.01, graph=False):
         """x = A^-1 b for a device float64 right-hand side b (length h*w), with
         this solver's operator, preconditioner and precision.
 
@@ -176,7 +176,8 @@ class LambdaSolver:
         "refine" restarts a float32 PCG on each float64 residual (inner_rtol,
         max_outer); "reliable" runs one float32 PCG whose solution accumulates
         in float64 and whose residual is replaced by the float64 one each time
-        it has dropped by ``delta``."""
+        it has dropped by ``delta`` (``graph``: its iterations without a
+        replacement are replayed from HIP graphs, with identical results)."""
         import ctypes
         n = self.h * self.w
         b = b.reshape(-1)
@@ -195,8 +196,8 @@ class LambdaSolver:
             if mixed == "reliable":
                 rc = _ffi.lib().hm_cg_solve_reliable(
                     cf, dia, self.M.handle, _ffi.ptr(b), _ffi.ptr(x), float(rtol),
-                    float(delta), int(maxiter), int(check_every), ctypes.byref(stats),
-                    _ffi.stream())
+                    float(delta), int(maxiter), int(check_every), int(bool(graph)),
+                    ctypes.byref(stats), _ffi.stream())
             elif mixed == "refine":
                 rc = _ffi.lib().hm_cg_solve_mixed(
                     cf, dia, self.M.handle, _ffi.ptr(b), _ffi.ptr(x), float(rtol),
@@ -210,9 +211,9 @@ class LambdaSolver:
         return x
 
     def solve(self, rtol=1e-7, atol=0.0, maxiter=10000, inner_rtol=1e-4, max_outer=20,
-              check_every=8, mixed="refine", delta=0.01):
+              check_every=8, mixed="refine", delta=0.01, graph=False):
         self.x = self.solve_system(self.b, rtol, atol, maxiter, inner_rtol, max_outer,
-                                   check_every, mixed, delta)
+                                   check_every, mixed, delta, graph)
         return _finalize(self.x, self.tdev, None, (self.h, self.w))
 
 

@@ -143,6 +143,9 @@ enum ProfKind {
     kProfCount
 };
 
+// Whether per-launch profiling (hm_prof_set) is on.
+bool prof_on();
+
 struct ProfScope {
     ProfScope(int kind, long long points, int prec, hipStream_t s);
     ~ProfScope();

@@ -734,7 +734,8 @@ __global__ __launch_bounds__(kFinBlock) void k_replace_r(const double* __restric
 }
 
 int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x, double rtol,
-                 double delta, int maxiter, int check_every, hm_cg_stats* stats, hipStream_t s) {
+                 double delta, int maxiter, int check_every, bool graph, hm_cg_stats* stats,
+                 hipStream_t s) {
     OpHandle A;
     A.cf = cf;
     A.dia = dia;
@@ -808,12 +809,66 @@ int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x
     int it_prev = 0;   // iterations at the last read-back
     int rep_idx = -1;  // replacement index in the next batch (none before a rate is known)
     int enqueued = 0;
+    // The launches of an iteration without a replacement: the fused SpMV,
+    // the update and the V-cycle (pc_cur -> pc_next).
+    auto plain_iteration = [&](float* pc_cur, float* pc_next) -> int {
+        HM_TRY(op_apply_pupdate<float>(A, z.as<float>(), pc_cur, pc_next, q.as<float>(), pq,
+                                       st.dev, s));
+        if (x_beside_tail) {
+            ProfScope prof_u(kProfCgUpdateR, n, 1, s);
+            k_cg_update_pq<double, float, false><<<kUpdBlocks, kFinBlock, 0, s>>>(
+                nullptr, pc_next, r.as<float>(), q.as<float>(), n, st.dev, red, pq_blocks,
+                red_upd);
+        } else {
+            ProfScope prof_u(kProfCgUpdateX64, n, 1, s);
+            k_cg_update_pq<double, float><<<kUpdBlocks, kFinBlock, 0, s>>>(
+                x, pc_next, r.as<float>(), q.as<float>(), n, st.dev, red, pq_blocks, red_upd);
+        }
+        HM_LAUNCH_CHECK();
+        const TailSide side{x, pc_next, n, st.dev, 1};
+        return pc_apply<float>(M, r.as<float>(), z.as<float>(), st.dev, red, kDotUpdateRz,
+                               st.dev, n, s, x_beside_tail ? &side : nullptr);
+    };
+    // graph: each parity of the direction buffers' alternation gets its plain
+    // iteration recorded once into a HIP graph (on the first iteration that
+    // needs it) and replayed; replacement iterations stay direct launches.
+    // Per-launch profiling records events, so it keeps direct launches.
+    const bool use_graph = graph && !prof_on();
+    hipGraph_t graphs[2] = {nullptr, nullptr};
+    hipGraphExec_t execs[2] = {nullptr, nullptr};
+    struct GraphGuard {
+        hipGraph_t* g;
+        hipGraphExec_t* e;
+        ~GraphGuard() {
+            for (int k = 0; k < 2; ++k) {
+                if (e[k]) (void)hipGraphExecDestroy(e[k]);
+                if (g[k]) (void)hipGraphDestroy(g[k]);
+            }
+        }
+    } graph_guard{graphs, execs};
+    float* const p_first = p.as<float>();
     while (true) {
         const int k = std::min(K, maxiter - enqueued);
         if (k <= 0) break;
         const int rep_at = rep_idx < k ? rep_idx : -1;
         for (int it = 0; it < k; ++it) {
             const bool replace = it == rep_at;
+            if (use_graph && !replace) {
+                const int par = p_cur == p_first ? 0 : 1;
+                if (!execs[par]) {
+                    HM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                    const int rc = plain_iteration(p_cur, p_next);
+                    hipGraph_t g = nullptr;
+                    const hipError_t e = hipStreamEndCapture(s, &g);
+                    graphs[par] = g;
+                    HM_TRY(rc);
+                    HM_HIP(e);
+                    HM_HIP(hipGraphInstantiate(&execs[par], g, nullptr, nullptr, 0));
+                }
+                HM_HIP(hipGraphLaunch(execs[par], s));
+                std::swap(p_cur, p_next);
+                continue;
+            }
             HM_TRY(op_apply_pupdate<float>(A, z.as<float>(), p_cur, p_next, q.as<float>(), pq,
                                            st.dev, s));
             std::swap(p_cur, p_next);
@@ -1550,15 +1605,33 @@ int hm_cg_solve(const hm_cg_args* a, hm_cg_stats* stats, void* stream) {
 }
 
 int hm_cg_solve_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x,
-                         double rtol, double delta, int maxiter, int check_every,
+                         double rtol, double delta, int maxiter, int check_every, int graph,
                          hm_cg_stats* stats, void* stream) {
     if ((cf == nullptr) == (dia == nullptr) || !vc32 || !b || !x || !stats ||
-        vc32->prec != HM_F32 || !(delta > 0.0 && delta < 1.0)) {
-        set_error("hm_cg_solve_reliable: needs one operator, a float32 V-cycle and 0 < delta < 1");
+        vc32->prec != HM_F32 || !(delta > 0.0 && delta < 1.0) || (graph != 0 && graph != 1)) {
+        set_error("hm_cg_solve_reliable: needs one operator, a float32 V-cycle, 0 < delta < 1 "
+                  "and graph 0 or 1");
         return HM_ERR_ARG;
     }
-    return pcg_reliable(cf, dia, vc32, b, x, rtol, delta, maxiter, check_every, stats,
-                        static_cast<hipStream_t>(stream));
+    hipStream_t user = static_cast<hipStream_t>(stream);
+    if (!graph) return pcg_reliable(cf, dia, vc32, b, x, rtol, delta, maxiter, check_every,
+                                    false, stats, user);
+    // Graphs cannot be captured on the legacy default stream: the solve runs
+    // on a stream of this thread's own, ordered after and before the caller's.
+    static thread_local hipStream_t own = nullptr;
+    static thread_local hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    if (!own) {
+        HM_HIP(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
+        HM_HIP(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
+        HM_HIP(hipEventCreateWithFlags(&ev_out, hipEventDisableTiming));
+    }
+    HM_HIP(hipEventRecord(ev_in, user));
+    HM_HIP(hipStreamWaitEvent(own, ev_in, 0));
+    const int rc = pcg_reliable(cf, dia, vc32, b, x, rtol, delta, maxiter, check_every, true,
+                                stats, own);
+    (void)hipStreamSynchronize(own);
+    if (hipEventRecord(ev_out, own) == hipSuccess) (void)hipStreamWaitEvent(user, ev_out, 0);
+    return rc;
 }
 
 int hm_cg_solve_mixed(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x,
@@ -1666,6 +1739,8 @@ std::vector<ProfRecord> g_prof;
 std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_pool;
 size_t g_prof_used = 0;
 }  // namespace
+
+bool prof_on() { return g_prof_on; }
 
 ProfScope::ProfScope(int kind, long long points, int prec, hipStream_t s) : slot(-1), stream(s) {
     if (!g_prof_on) return;

@@ -55,8 +55,8 @@ def test_argument_checks_before_any_device_call():
     ARG = -1
     assert lib.hm_version() == 1
     # Reliable-update and refinement solves need one operator and a V-cycle.
-    lib.hm_cg_solve_reliable.argtypes = [P, P, P, P, P, f64, f64, i32, i32, P, P]
-    assert lib.hm_cg_solve_reliable(None, None, None, None, None, 1e-9, 0.01, 10, 8, None,
+    lib.hm_cg_solve_reliable.argtypes = [P, P, P, P, P, f64, f64, i32, i32, i32, P, P]
+    assert lib.hm_cg_solve_reliable(None, None, None, None, None, 1e-9, 0.01, 10, 8, 0, None,
                                     None) == ARG
     assert "0 < delta < 1" in _err(lib)
     lib.hm_cg_solve_mixed.argtypes = [P, P, P, P, P, f64, f64, i32, i32, P, P]
