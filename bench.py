@@ -109,6 +109,14 @@ def bytes_per_point(kind, s):
         "cfrec_restrict": s + s + 3 * s + 1 + s / 4,
         "cfrec_smooth": s + s + 3 * s + 1 + s / 4 + s,
         "cfrec_residual": s + s + 3 * s + 1 + s,
+        # Several coarse levels in one launch, per point of the launch's
+        # finest level a (a level-a+j point stands for 4^-j of them): the
+        # descent pair reads level a's stencil, b and 1/D and level a+1's
+        # stencil and 1/D and writes b of levels a+1 and a+2; the ascent chain
+        # of four levels reads each level's stencil, b and 1/D and the
+        # correction of level a+4 and writes z of level a.
+        "pair_down": 13 * s + 2 * s + (13 * s + s + s) / 4 + s / 16,
+        "pair_up": (13 * s + 2 * s) * (1 + 1 / 4 + 1 / 16 + 1 / 64) + s + s / 256,
     }[kind]
 
 
@@ -116,7 +124,7 @@ KINDS = ["cf_apply", "cf_restrict", "cf_smooth", "cf_residual", "dia_apply", "di
          "dia_smooth", "dia_residual", "cg_update", "cg_pupdate", "tail", "fg_sweep",
          "coarse_apply", "coarse_restrict", "coarse_smooth", "coarse_residual", "cg_update_x64",
          "cg_update_r", "l1_apply", "l1_restrict", "l1_smooth", "l1_residual", "cfrec_apply",
-         "cfrec_restrict", "cfrec_smooth", "cfrec_residual"]
+         "cfrec_restrict", "cfrec_smooth", "cfrec_residual", "pair_down", "pair_up"]
 
 
 # Algorithmic bytes per grid point of the stored-stencil SpMV y = A x:
@@ -928,7 +936,7 @@ def main():
                                                        "GBps_with_events", "frac_of_hbm_peak")}
                      for k in ("dia_apply_f32", "dia_restrict_f32", "dia_smooth_f32",
                                "l1_restrict_f32", "l1_smooth_f32", "coarse_restrict_f32",
-                               "coarse_smooth_f32") if k in breakdown}
+                               "coarse_smooth_f32", "pair_up_f32") if k in breakdown}
 
     oracle_vs = None
     if args.no_cpu:

@@ -211,6 +211,13 @@ int hm_vcycle_create_ex(hm_cf* cf, hm_dia* dia, int h, int w, int num_pre_iter,
 int hm_vcycle_create_dense(const double* A_dense, int n, int prec, void* stream, hm_vc** out);
 int hm_vcycle_destroy(hm_vc* vc);
 int hm_vcycle_levels(hm_vc* vc); /* number of grids including the direct level */
+/* No counterpart in the reference (a launch-structure switch for cycles with
+ * one pre- and one post-smoothing sweep). pairs is a bit mask over the
+ * coarse grids (below 2^18 points, from the third grid on): 1 (the default)
+ * runs consecutive grids of the ascent in one launch, 2 runs pairs of grids
+ * of the descent in one launch; 0 runs every grid in a launch of its own.
+ * All give bit-identical cycles. */
+int hm_vcycle_set_pairs(hm_vc* vc, int pairs);
 /* z = V-cycle(r) (vectors in the hierarchy's precision). */
 int hm_vcycle_apply(hm_vc* vc, const void* r, void* z, void* stream);
 

@@ -140,6 +140,8 @@ enum ProfKind {
     kProfCgUpdateR,    // CG update of r alone (x updated beside the V-cycle tail)
     kProfLevel1Apply, kProfLevel1Restrict, kProfLevel1Smooth, kProfLevel1Residual,
     kProfCfRecApply, kProfCfRecRestrict, kProfCfRecSmooth, kProfCfRecResidual,
+    kProfPairDown,  // two coarse levels of the descent in one launch
+    kProfPairUp,    // two coarse levels of the ascent in one launch
     kProfCount
 };
 

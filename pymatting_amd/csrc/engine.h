@@ -135,6 +135,69 @@ __device__ __forceinline__ double fmadd(double a, double b, double c) {
     return __builtin_fma(a, b, c);
 }
 
+// ------------------------------------------------ V-cycle pointwise arithmetic
+//
+// The V-cycle's per-point formulas (vcycle.py:32-43 and 77-100), each written
+// out operation by operation with no implicit contraction. Every kernel that
+// evaluates them -- the one-level tile kernels and the two-level kernels that
+// recompute an intermediate level on a halo -- therefore rounds alike, and
+// the two launch structures give bit-identical V-cycles.
+
+// First damped-Jacobi iterate from zero: omega * b / D, with iD = 1 / D.
+template <typename T>
+__device__ __forceinline__ T vc_jac0(T omega, T b, T iD) {
+#pragma clang fp contract(off)
+    return omega * b * iD;
+}
+
+// v + (P e) at a fine point of parities (ox, oy): P e takes 1/2 of the
+// coincident coarse value per axis at even coordinates and 1/4 of each of the
+// two neighbours at odd ones. e00 is the coarse value at (x >> 1, y >> 1),
+// e01 the one to its right, e10 below, e11 diagonal (zero outside the coarse
+// grid; taps that a parity does not use are ignored).
+template <typename T>
+__device__ __forceinline__ T vc_prolong(T v, bool ox, bool oy, T e00, T e01, T e10, T e11) {
+#pragma clang fp contract(off)
+    const T wx0 = ox ? T(0.25) : T(0.5), wy0 = oy ? T(0.25) : T(0.5);
+    T row0 = wx0 * e00;
+    if (ox) row0 = fmadd(T(0.25), e01, row0);
+    T pe = wy0 * row0;
+    if (oy) {
+        T row1 = wx0 * e10;
+        if (ox) row1 = fmadd(T(0.25), e11, row1);
+        pe = fmadd(T(0.25), row1, pe);
+    }
+    return v + pe;
+}
+
+// One damped-Jacobi sweep at a point: y + omega (b - A y) / D.
+template <typename T>
+__device__ __forceinline__ T vc_smooth(T y, T b, T Ay, T iD, T omega) {
+#pragma clang fp contract(off)
+    return fmadd(omega * (b - Ay), iD, y);
+}
+
+// Full-weighting restriction of a 3 x 3 block of fine residuals; r0, r1, r2
+// point at the centre column of its three rows.
+template <typename T>
+__device__ __forceinline__ T vc_restrict9(const T* r0, const T* r1, const T* r2) {
+#pragma clang fp contract(off)
+    const T a = fmadd(T(2), r0[0], r0[-1]) + r0[1];
+    const T b = fmadd(T(2), r1[0], r1[-1]) + r1[1];
+    const T c = fmadd(T(2), r2[0], r2[-1]) + r2[1];
+    return (fmadd(T(2), b, a) + c) * T(0.0625);
+}
+
+// 25-tap stencil sum sum_s c[s] x[s] as a chain of fused multiply-adds in
+// slot order; x(s) returns the input under slot s.
+template <typename T, class C, class X>
+__device__ __forceinline__ T vc_stencil(const C& c, const X& x) {
+    T acc = 0;
+#pragma unroll
+    for (int s = 0; s < 25; ++s) acc = fmadd(c(s), x(s), acc);
+    return acc;
+}
+
 // Rec = false: the window data are read from W. Rec = true (float32 passes):
 // every pass recomputes each window's mean and inverse covariance from the
 // image it stages anyway (cf_window_f64), so a pass reads the image, the
@@ -377,18 +440,14 @@ struct DiaOp {
     __device__ __forceinline__ T apply(const Shared<OX, OY>&, const Prefetch<OX, OY>& pf,
                                        const T (&sx)[OY + 4][OX + 4], int k, int lx, int ly, T,
                                        long long g) const {
-        T acc = 0;
+        const auto xs = [&](int s) { return sx[ly + s / 5][lx + s % 5]; };
         if constexpr (PF) {
-#pragma unroll
-            for (int s = 0; s < 25; ++s) acc += pf.c[k][s] * sx[ly + s / 5][lx + s % 5];
+            return vc_stencil<T>([&](int s) { return pf.c[k][s]; }, xs);
         } else {
             const int n = h * w;
             const int i = (int)g;
-#pragma unroll
-            for (int s = 0; s < 25; ++s)
-                acc += sym_coef32(coef, n, w, i, s) * sx[ly + s / 5][lx + s % 5];
+            return vc_stencil<T>([&](int s) { return sym_coef32(coef, n, w, i, s); }, xs);
         }
-        return acc;
     }
 };
 
@@ -423,7 +482,9 @@ struct InJac0 {
     const T* b;
     const T* iD;  // 1 / D
     T omega;
-    __device__ __forceinline__ T load(int, int, long long g) const { return omega * b[g] * iD[g]; }
+    __device__ __forceinline__ T load(int, int, long long g) const {
+        return vc_jac0(omega, b[g], iD[g]);
+    }
 };
 
 // Inputs that stage data in LDS before the tile's loads (InProlong) provide
@@ -463,48 +524,26 @@ struct InProlong {
         // Same taps and weights as load(); zeros stand for the dropped taps.
         const int lx0 = (gx >> 1) - ((ox0 - 2) >> 1), ly0 = (gy >> 1) - ((oy0 - 2) >> 1);
         const bool ox = gx & 1, oy = gy & 1;
-        const T wx0 = ox ? T(0.25) : T(0.5), wy0 = oy ? T(0.25) : T(0.5);
-        T pe = 0;
-        {
-            T row = 0;
-            row += wx0 * sh.e[ly0][lx0];
-            if (ox) row += T(0.25) * sh.e[ly0][lx0 + 1];
-            pe += wy0 * row;
-        }
-        if (oy) {
-            T row = 0;
-            row += wx0 * sh.e[ly0 + 1][lx0];
-            if (ox) row += T(0.25) * sh.e[ly0 + 1][lx0 + 1];
-            pe += T(0.25) * row;
-        }
-        return v + pe;
+        return vc_prolong(v, ox, oy, sh.e[ly0][lx0], ox ? sh.e[ly0][lx0 + 1] : T(0),
+                          oy ? sh.e[ly0 + 1][lx0] : T(0),
+                          (ox && oy) ? sh.e[ly0 + 1][lx0 + 1] : T(0));
     }
     __device__ __forceinline__ T load(int gx, int gy, long long g) const {
         const T v = base.load(gx, gy, g);
         // Fine (gx, gy) takes 1/2 of the coincident coarse point per axis when
         // even, 1/4 of each of the two neighbours when odd (x0 = gx >> 1
         // floors, so the odd case's taps are x0 and x0 + 1); taps outside the
-        // coarse grid are dropped.
+        // coarse grid are dropped (read as zero).
         const int x0 = gx >> 1, y0 = gy >> 1;
         const bool ox = gx & 1, oy = gy & 1;
-        const T wx0 = ox ? T(0.25) : T(0.5), wy0 = oy ? T(0.25) : T(0.5);
         const bool vx0 = x0 >= 0 && x0 < w2, vx1 = ox && x0 + 1 < w2;
         const bool vy0 = y0 >= 0 && y0 < h2, vy1 = oy && y0 + 1 < h2;
         const long long r0 = (long long)y0 * w2, r1 = r0 + w2;
-        T pe = 0;
-        if (vy0) {
-            T row = 0;
-            if (vx0) row += wx0 * e[r0 + x0];
-            if (vx1) row += T(0.25) * e[r0 + x0 + 1];
-            pe += wy0 * row;
-        }
-        if (vy1) {
-            T row = 0;
-            if (vx0) row += wx0 * e[r1 + x0];
-            if (vx1) row += T(0.25) * e[r1 + x0 + 1];
-            pe += T(0.25) * row;
-        }
-        return v + pe;
+        const T e00 = (vy0 && vx0) ? e[r0 + x0] : T(0);
+        const T e01 = (vy0 && vx1) ? e[r0 + x0 + 1] : T(0);
+        const T e10 = (vy1 && vx0) ? e[r1 + x0] : T(0);
+        const T e11 = (vy1 && vx1) ? e[r1 + x0 + 1] : T(0);
+        return vc_prolong(v, ox, oy, e00, e01, e10, e11);
     }
 };
 
@@ -639,7 +678,7 @@ struct OutSmooth {
     __device__ __forceinline__ void point(long long g, int, int, T Ax, T xi, double& acc,
                                           const T* pv) const {
         const T bg = pv[0];
-        const T zg = xi + omega * (bg - Ax) * pv[1];
+        const T zg = vc_smooth(xi, bg, Ax, pv[1], omega);
         z[g] = zg;
         acc += (double)bg * (double)zg;
     }
@@ -677,9 +716,7 @@ struct OutRestrict {
             const T* r0 = sres + (ly - 1) * OX + lx;
             const T* r1 = sres + ly * OX + lx;
             const T* r2 = sres + (ly + 1) * OX + lx;
-            T s = (r0[-1] + T(2) * r0[0] + r0[1]) + T(2) * (r1[-1] + T(2) * r1[0] + r1[1]) +
-                  (r2[-1] + T(2) * r2[0] + r2[1]);
-            bc[(long long)Y * w2 + X] = s * T(0.0625);
+            bc[(long long)Y * w2 + X] = vc_restrict9(r0, r1, r2);
         }
     }
 };

@@ -88,6 +88,11 @@ struct hm_vc {
     // hierarchy's precision; built once for the tail starting at tail_start.
     void* tail_pack = nullptr;
     size_t tail_start = 0;
+    // Launch structure of the coarse levels (pair.h), a bit mask: 1 runs
+    // consecutive levels of the ascent in one launch, 2 pairs of levels of
+    // the descent; 0 runs every level in launches of its own. All give the
+    // same cycle bit for bit.
+    int pairs = 1;
 };
 
 namespace hm {

@@ -18,6 +18,7 @@
 #include "engine.h"
 #include "internal.h"
 #include "vcycle.h"
+#include "pair.h"
 
 namespace {
 thread_local std::string g_last_error;
@@ -91,8 +92,15 @@ int vcycle_apply_t(hm_vc* vc, const T* r, T* z, CgState* st, const Reducer* red,
     }
     std::vector<T*> result(L, nullptr), pre_x(L, nullptr);
     const size_t tail = vc_tail_start(vc);
+    // Coarse levels that run several to a launch (pair.h).
+    const VcPlan plan = vc_plan(vc, tail);
     // Descend: pre-smooth, residual, restrict.
     for (size_t l = 0; l + 1 < L && l < tail; ++l) {
+        if (plan.down_pair[l]) {
+            HM_TRY(launch_pair_down<T>(vc, l, guard, s));
+            ++l;  // level l + 1 restricted too
+            continue;
+        }
         const VcLevel& lev = vc->levels[l];
         const VcLevel& cl = vc->levels[l + 1];
         const LevelOp<T> op = vc_level_op<T>(vc, l);
@@ -152,6 +160,14 @@ int vcycle_apply_t(hm_vc* vc, const T* r, T* z, CgState* st, const Reducer* red,
     result[tail] = vptr<T>(vc->levels[tail].x);
     // Ascend: prolongate the correction and post-smooth (vcycle.py:95-100).
     for (size_t l = tail; l-- > 0;) {
+        if (plan.up_start[l] >= 0) {  // levels l, l - 1, ..., a in one launch
+            const size_t a = (size_t)plan.up_start[l];
+            T* za = vptr<T>(vc->levels[a].x);
+            HM_TRY(launch_up_chain<T>(vc, a, l - a + 1, result[l + 1], za, guard, s));
+            result[a] = za;
+            l = a;
+            continue;
+        }
         const VcLevel& lev = vc->levels[l];
         const VcLevel& cl = vc->levels[l + 1];
         const LevelOp<T> op = vc_level_op<T>(vc, l);
@@ -1504,6 +1520,19 @@ int hm_vcycle_destroy(hm_vc* vc) {
 }
 
 int hm_vcycle_levels(hm_vc* vc) { return vc ? (int)vc->levels.size() : 0; }
+
+int hm_vcycle_set_pairs(hm_vc* vc, int pairs) {
+    if (!vc) {
+        hm::set_error("null V-cycle");
+        return HM_ERR_ARG;
+    }
+    if (pairs < 0 || pairs > (hm::kVcChainUp | hm::kVcPairDown)) {
+        hm::set_error("pairs must be 0..3, got %d", pairs);
+        return HM_ERR_ARG;
+    }
+    vc->pairs = pairs;
+    return HM_OK;
+}
 
 int hm_vcycle_apply(hm_vc* vc, const void* r, void* z, void* stream) {
     if (!vc || !r || !z) {

@@ -107,7 +107,10 @@ int run_level(const LevelOp<T>& L, const In& in, const Out& out, const CgState* 
     // Coarse grids below kSmallGrid points are latency-bound: one point per
     // thread with its coefficients loaded before the tile barrier. Larger
     // grids keep the bandwidth-oriented shape of the finest level.
-    constexpr long long kSmallGrid = 1 << 18;
+#ifndef HM_SMALL_GRID_LOG2
+#define HM_SMALL_GRID_LOG2 18
+#endif
+    constexpr long long kSmallGrid = 1LL << HM_SMALL_GRID_LOG2;
     if (L.level > 0 && (long long)L.h * L.w < kSmallGrid) {
         const DiaOp<T, true> op{L.dia.h, L.dia.w, L.dia.coef};
         return launch_tile<T>(op, in, out.small(), L.h, L.w, L.h / 2, L.w / 2, guard, s,
@@ -267,7 +270,7 @@ __global__ void k_jac0(const T* __restrict__ b, const T* __restrict__ iD, T omeg
                        T* __restrict__ x, long long n, const CgState* guard) {
     if (solve_done(guard)) return;
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = omega * b[i] * iD[i];
+    if (i < n) x[i] = vc_jac0(omega, b[i], iD[i]);
 }
 
 // ------------------------------------------------------------------ V-cycle

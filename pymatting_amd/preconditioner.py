@@ -129,6 +129,15 @@ class VCycle:
     def __del__(self):
         _ffi.release(self, "hm_vcycle_destroy")
 
+    def set_pairs(self, mode):
+        """Launch structure of the coarse grids (cycles with one pre- and one
+        post-smoothing sweep): a bit mask, 1 (the default) runs consecutive
+        coarse grids of the ascent in one launch, 2 pairs of grids of the
+        descent; 0 (or False) one grid per launch. True means 1. The cycle's
+        result is the same bit for bit."""
+        _ffi.check(_ffi.lib().hm_vcycle_set_pairs(self.handle, int(mode)))
+        return self
+
     def apply(self, r, out=None):
         dt = torch.float64 if self.precision == "f64" else torch.float32
         out = out if out is not None else empty(self.n, dt)
