@@ -211,13 +211,15 @@ int hm_vcycle_create_ex(hm_cf* cf, hm_dia* dia, int h, int w, int num_pre_iter,
 int hm_vcycle_create_dense(const double* A_dense, int n, int prec, void* stream, hm_vc** out);
 int hm_vcycle_destroy(hm_vc* vc);
 int hm_vcycle_levels(hm_vc* vc); /* number of grids including the direct level */
-/* No counterpart in the reference (a launch-structure switch for cycles with
- * one pre- and one post-smoothing sweep). pairs is a bit mask over the
- * coarse grids (below 2^18 points, from the third grid on): 1 (the default)
- * runs consecutive grids of the ascent in one launch, 2 runs pairs of grids
- * of the descent in one launch; 0 runs every grid in a launch of its own.
- * All give bit-identical cycles. */
-int hm_vcycle_set_pairs(hm_vc* vc, int pairs);
+/* No counterpart in the reference: the launch structure of a cycle, a bit
+ * mask. 1: consecutive coarse grids of the ascent (below 2^18 points, from
+ * the third grid on, cycles with one pre- and one post-smoothing sweep) in
+ * one launch; 2: pairs of those grids of the descent in one launch; 4: the
+ * coarsest grids that one workgroup would run in LDS (the tail) applied as
+ * their precomputed dense matrix. The default is 5. Bits 1 and 2 leave the
+ * cycle bit-identical; bit 4 changes it by rounding only (the tail's sums in
+ * another order). 0 runs every grid in a launch of its own. */
+int hm_vcycle_set_launches(hm_vc* vc, int pairs);
 /* z = V-cycle(r) (vectors in the hierarchy's precision). */
 int hm_vcycle_apply(hm_vc* vc, const void* r, void* z, void* stream);
 

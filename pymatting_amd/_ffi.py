@@ -91,7 +91,7 @@ _SIGS = {
     "hm_vcycle_create_dense": (i32, [P, i32, i32, P, ctypes.POINTER(P)]),
     "hm_vcycle_destroy": (i32, [P]),
     "hm_vcycle_levels": (i32, [P]),
-    "hm_vcycle_set_pairs": (i32, [P, i32]),
+    "hm_vcycle_set_launches": (i32, [P, i32]),
     "hm_vcycle_apply": (i32, [P, P, P, P]),
     "hm_cg_solve": (i32, [ctypes.POINTER(CgArgs), ctypes.POINTER(CgStats), P]),
     "hm_cg_solve_mixed": (i32, [P, P, P, P, P, f64, f64, i32, i32, ctypes.POINTER(CgStats), P]),

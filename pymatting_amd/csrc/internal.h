@@ -88,11 +88,16 @@ struct hm_vc {
     // hierarchy's precision; built once for the tail starting at tail_start.
     void* tail_pack = nullptr;
     size_t tail_start = 0;
-    // Launch structure of the coarse levels (pair.h), a bit mask: 1 runs
-    // consecutive levels of the ascent in one launch, 2 pairs of levels of
-    // the descent; 0 runs every level in launches of its own. All give the
-    // same cycle bit for bit.
-    int pairs = 1;
+    // The tail as a dense row-major matrix over its top grid (tail_mat_n
+    // points), applied instead of the tail's launch when present.
+    void* tail_mat = nullptr;
+    int tail_mat_n = 0;
+    // Launch structure of the coarse levels, a bit mask: 1 runs consecutive
+    // levels of the ascent in one launch, 2 pairs of levels of the descent
+    // (pair.h), 4 applies the tail as its dense matrix (vcycle.h); 0 runs
+    // every level in launches of its own. 1 and 2 leave the cycle
+    // bit-identical, 4 changes its rounding.
+    int pairs = 5;
 };
 
 namespace hm {

@@ -529,7 +529,7 @@ constexpr int kChainOX = HM_CHAIN_OX, kChainOY = HM_CHAIN_OY, kChainBlock = HM_C
 #endif
 static_assert(HM_CHAIN_LEN >= 2 && HM_CHAIN_LEN <= kChainMax, "chain length");
 
-// Launch structure bits of hm_vc::pairs.
+// Launch structure bits of hm_vc::pairs (kVcTailMatrix: vcycle.h).
 constexpr int kVcChainUp = 1;    // ascent chains
 constexpr int kVcPairDown = 2;   // descent pairs
 

@@ -808,7 +808,10 @@ int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x
     // With an LDS tail in the V-cycle, x += alpha p runs in the tail's launch
     // on the CUs the tail leaves idle, and the update kernel only updates r;
     // replacement iterations need x first and update both.
-    const bool x_beside_tail = vc_has_tail(vc32);
+#ifndef HM_TAIL_SIDE
+#define HM_TAIL_SIDE 1
+#endif
+    const bool x_beside_tail = HM_TAIL_SIDE && vc_has_tail(vc32);
     // The state is read back every K iterations. A replacement is placed at
     // the iteration where the residual norm is predicted (from the rate of
     // the last batch) to have dropped by delta since the last replacement;
@@ -989,6 +992,7 @@ void vc_free(hm_vc* vc) {
     dev_free(vc->dense_inv64);
     dev_free(vc->dense_inv32);
     dev_free(vc->tail_pack);
+    dev_free(vc->tail_mat);
     free_reducer(&vc->red);
     delete vc;
 }
@@ -1165,6 +1169,12 @@ int vc_create(hm_cf* cf, hm_dia* dia, int h, int w, int pre, int post, double om
     if (rc == HM_OK && vc_tail_start(vc) < vc->levels.size() - 1)
         rc = prec == HM_F64 ? build_tail_pack<double>(vc, vc_tail_start(vc), s)
                             : build_tail_pack<float>(vc, vc_tail_start(vc), s);
+#ifndef HM_TAIL_MATRIX
+#define HM_TAIL_MATRIX 1
+#endif
+    if (HM_TAIL_MATRIX && rc == HM_OK && vc_tail_start(vc) < vc->levels.size() - 1)
+        rc = prec == HM_F64 ? build_tail_matrix<double>(vc, vc_tail_start(vc), s)
+                            : build_tail_matrix<float>(vc, vc_tail_start(vc), s);
     if (rc != HM_OK) { vc_free(vc); return rc; }
     HM_HIP(hipStreamSynchronize(s));
     *out = vc;
@@ -1521,13 +1531,13 @@ int hm_vcycle_destroy(hm_vc* vc) {
 
 int hm_vcycle_levels(hm_vc* vc) { return vc ? (int)vc->levels.size() : 0; }
 
-int hm_vcycle_set_pairs(hm_vc* vc, int pairs) {
+int hm_vcycle_set_launches(hm_vc* vc, int pairs) {
     if (!vc) {
         hm::set_error("null V-cycle");
         return HM_ERR_ARG;
     }
-    if (pairs < 0 || pairs > (hm::kVcChainUp | hm::kVcPairDown)) {
-        hm::set_error("pairs must be 0..3, got %d", pairs);
+    if (pairs < 0 || pairs > (hm::kVcChainUp | hm::kVcPairDown | hm::kVcTailMatrix)) {
+        hm::set_error("launch structure must be 0..7, got %d", pairs);
         return HM_ERR_ARG;
     }
     vc->pairs = pairs;

@@ -422,8 +422,13 @@ __device__ __forceinline__ T tail_prolong(const TailView<T>& C, const T* e, int 
     return T(0.0625) * ((r0[0] + r0[1]) + (r0[C.pitch] + r0[C.pitch + 1]));
 }
 
+// basis >= 0: the right-hand side is the unit vector of the top grid's point
+// `basis` instead of the top grid's b, and the correction is written to
+// column `basis` of the row-major top-grid matrix basis_out (see
+// build_tail_matrix).
 template <typename T>
-__device__ __forceinline__ void vc_tail_body(const TailParams<T>& P) {
+__device__ __forceinline__ void vc_tail_body(const TailParams<T>& P, int basis = -1,
+                                             T* basis_out = nullptr) {
     extern __shared__ __align__(16) unsigned char tail_lds_raw[];
     T* lds = reinterpret_cast<T*>(tail_lds_raw);
     const int tid = threadIdx.x;
@@ -463,7 +468,9 @@ __device__ __forceinline__ void vc_tail_body(const TailParams<T>& P) {
                 v[u] = T(0);
                 if (g < rhs_n) {
                     const int y = g / V0.pitch - 2, x = g - (g / V0.pitch) * V0.pitch - 2;
-                    if (y >= 0 && y < V0.h && x >= 0 && x < V0.w) v[u] = P.lv[0].b[y * V0.w + x];
+                    if (y >= 0 && y < V0.h && x >= 0 && x < V0.w)
+                        v[u] = basis < 0 ? P.lv[0].b[y * V0.w + x]
+                                         : (y * V0.w + x == basis ? T(1) : T(0));
                 } else if (g < total) {
                     v[u] = P.pack[g - rhs_n];
                 }
@@ -609,7 +616,10 @@ __device__ __forceinline__ void vc_tail_body(const TailParams<T>& P) {
         const TailView<T> V = tail_view(lds, P.lv[0]);
         for (int i = tid; i < V.n; i += kTailBlock) {
             const int y = i / V.w;
-            P.lv[0].x[i] = e[V.at(y, i - y * V.w)];
+            if (basis < 0)
+                P.lv[0].x[i] = e[V.at(y, i - y * V.w)];
+            else
+                basis_out[(size_t)i * V.n + basis] = e[V.at(y, i - y * V.w)];
         }
     }
 }
@@ -628,15 +638,16 @@ struct TailSide {
 };
 constexpr unsigned kTailSideBlocks = 255;
 
-// x += alpha p over workgroup b of nb (the arithmetic of k_cg_update_pq).
-template <typename TX, typename T>
-__device__ __forceinline__ void tail_side_update(TX* __restrict__ x, const T* __restrict__ p,
-                                                 long long n, double alpha, unsigned b,
-                                                 unsigned nb) {
+// x += alpha p over workgroup b of nb workgroups of BS threads (the
+// arithmetic of k_cg_update_pq).
+template <typename TX, typename T, int BS>
+__device__ __forceinline__ void tail_side_update_n(TX* __restrict__ x, const T* __restrict__ p,
+                                                   long long n, double alpha, unsigned b,
+                                                   unsigned nb) {
     const T alphaT = (T)alpha;
     constexpr int U = 4;
-    const long long stride = (long long)nb * kTailBlock;
-    for (long long i0 = (long long)b * kTailBlock + threadIdx.x; i0 < n; i0 += U * stride) {
+    const long long stride = (long long)nb * BS;
+    for (long long i0 = (long long)b * BS + threadIdx.x; i0 < n; i0 += U * stride) {
         TX xv[U];
         T pv[U];
 #pragma unroll
@@ -658,6 +669,13 @@ __device__ __forceinline__ void tail_side_update(TX* __restrict__ x, const T* __
     }
 }
 
+template <typename TX, typename T>
+__device__ __forceinline__ void tail_side_update(TX* __restrict__ x, const T* __restrict__ p,
+                                                 long long n, double alpha, unsigned b,
+                                                 unsigned nb) {
+    tail_side_update_n<TX, T, kTailBlock>(x, p, n, alpha, b, nb);
+}
+
 template <typename T, bool kSide>
 __global__ __launch_bounds__(kTailBlock) void k_vc_tail(TailParams<T> P, const CgState* guard,
                                                         TailSide side) {
@@ -676,6 +694,72 @@ __global__ __launch_bounds__(kTailBlock) void k_vc_tail(TailParams<T> P, const C
         }
     }
     vc_tail_body<T>(P);
+}
+
+// The tail as a matrix. The tail is a fixed linear map from its top grid's
+// right-hand side to its top grid's correction; k_vc_tail_basis applies it to
+// every unit vector of the top grid (one workgroup each, the tail's own code)
+// and so writes the map as a dense row-major matrix, once per hierarchy.
+// Each V-cycle then applies the matrix (k_tail_matvec, one wavefront per row)
+// instead of running the tail's chain of dependent phases in one workgroup.
+// The result is the tail's up to rounding (the sums run in another order).
+template <typename T>
+__global__ __launch_bounds__(kTailBlock) void k_vc_tail_basis(TailParams<T> P, T* out) {
+    vc_tail_body<T>(P, (int)blockIdx.x, out);
+}
+
+constexpr int kMatvecBlock = 256;
+constexpr int kMatvecRowsPerBlock = kMatvecBlock / kWave;
+
+template <typename T>
+__device__ __forceinline__ T wave_sum1(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// x = M b for the n x n row-major M (n <= kTailMaxTop): workgroups below
+// row_blocks take four rows each, one per wavefront; the others stream the CG
+// solution update beside them (TailSide, as in k_vc_tail).
+template <typename T, bool kSide>
+__global__ __launch_bounds__(kMatvecBlock) void k_tail_matvec(const T* __restrict__ M,
+                                                              const T* __restrict__ b,
+                                                              T* __restrict__ x, int n,
+                                                              unsigned row_blocks,
+                                                              const CgState* guard,
+                                                              TailSide side) {
+    if (solve_done(guard)) return;
+    if constexpr (kSide) {
+        if (blockIdx.x >= row_blocks) {
+            const double alpha = side.st->alpha;
+            const T* p = static_cast<const T*>(side.p);
+            const unsigned nb = gridDim.x - row_blocks, bi = blockIdx.x - row_blocks;
+            if (side.x_f64)
+                tail_side_update_n<double, T, kMatvecBlock>(static_cast<double*>(side.x), p,
+                                                            side.n, alpha, bi, nb);
+            else
+                tail_side_update_n<T, T, kMatvecBlock>(static_cast<T*>(side.x), p, side.n,
+                                                       alpha, bi, nb);
+            return;
+        }
+    }
+    const int lane = threadIdx.x % kWave;
+    const int r = (int)blockIdx.x * kMatvecRowsPerBlock + (int)threadIdx.x / kWave;
+    if (r >= n) return;
+    constexpr int U = kTailMaxTop / kWave;
+    const T* row = M + (size_t)r * n;
+    T mv[U], bv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int j = lane + u * kWave;
+        mv[u] = j < n ? row[j] : T(0);
+        bv[u] = j < n ? b[j] : T(0);
+    }
+    T acc = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc = fmadd(mv[u], bv[u], acc);
+    acc = wave_sum1(acc);
+    if (lane == 0) x[r] = acc;
 }
 
 // LDS elements of a tail over levels [start, L) with the top level's stencil
@@ -800,9 +884,57 @@ int build_tail_pack(hm_vc* vc, size_t start, hipStream_t s) {
     return HM_OK;
 }
 
+// The tail of the hierarchy at `start` as a dense matrix (k_vc_tail_basis),
+// in the hierarchy's precision; needs the packed tail contents.
+template <typename T>
+int build_tail_matrix(hm_vc* vc, size_t start, hipStream_t s) {
+    TailParams<T> P;
+    const size_t lds = make_tail_params<T>(vc, start, P);
+    const long long n = vc->levels[start].n;
+    dev_free(vc->tail_mat);
+    vc->tail_mat = nullptr;
+    vc->tail_mat_n = 0;
+    if (n > kTailMaxTop) {
+        set_error("internal: tail grid of %lld points", n);
+        return HM_ERR_ARG;
+    }
+    HM_TRY(dev_alloc(&vc->tail_mat, sizeof(T) * (size_t)(n * n)));
+    HM_TRY(set_max_dynamic_lds(reinterpret_cast<const void*>(&k_vc_tail_basis<T>),
+                               kTailLdsBytes));
+    k_vc_tail_basis<T><<<(unsigned)n, kTailBlock, lds, s>>>(P, static_cast<T*>(vc->tail_mat));
+    HM_LAUNCH_CHECK();
+    vc->tail_mat_n = (int)n;
+    return HM_OK;
+}
+
+constexpr unsigned kMatvecSideBlocks = 4 * kTailSideBlocks;
+constexpr int kVcTailMatrix = 4;  // launch structure bit (hm_vc::pairs): tail as a matrix
+
 template <typename T>
 int launch_tail(hm_vc* vc, size_t start, const CgState* guard, hipStream_t s,
                 const TailSide* side = nullptr) {
+    if ((vc->pairs & kVcTailMatrix) && vc->tail_mat && vc->tail_mat_n == vc->levels[start].n &&
+        start == vc->tail_start) {
+        const VcLevel& V = vc->levels[start];
+        const int n = (int)V.n;
+        const unsigned rows = div_up(n, kMatvecRowsPerBlock);
+        ProfScope prof(kProfDense, n, sizeof(T) == 4, s);
+        if (side && side->x) {
+            if (!side->p || !side->st || side->n < 0) {
+                set_error("internal: incomplete V-cycle tail side work");
+                return HM_ERR_ARG;
+            }
+            k_tail_matvec<T, true><<<rows + kMatvecSideBlocks, kMatvecBlock, 0, s>>>(
+                static_cast<const T*>(vc->tail_mat), static_cast<const T*>(V.b),
+                static_cast<T*>(V.x), n, rows, guard, *side);
+        } else {
+            k_tail_matvec<T, false><<<rows, kMatvecBlock, 0, s>>>(
+                static_cast<const T*>(vc->tail_mat), static_cast<const T*>(V.b),
+                static_cast<T*>(V.x), n, rows, guard, TailSide{});
+        }
+        HM_LAUNCH_CHECK();
+        return HM_OK;
+    }
     TailParams<T> P;
     const size_t lds = make_tail_params<T>(vc, start, P);
     if (start != vc->tail_start || (P.pack_count > 0 && !P.pack)) {

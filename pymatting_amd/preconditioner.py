@@ -129,13 +129,14 @@ class VCycle:
     def __del__(self):
         _ffi.release(self, "hm_vcycle_destroy")
 
-    def set_pairs(self, mode):
-        """Launch structure of the coarse grids (cycles with one pre- and one
-        post-smoothing sweep): a bit mask, 1 (the default) runs consecutive
+    def set_launches(self, mode):
+        """Launch structure of the cycle, a bit mask: 1 runs consecutive
         coarse grids of the ascent in one launch, 2 pairs of grids of the
-        descent; 0 (or False) one grid per launch. True means 1. The cycle's
-        result is the same bit for bit."""
-        _ffi.check(_ffi.lib().hm_vcycle_set_pairs(self.handle, int(mode)))
+        descent, 4 applies the coarsest grids (those one workgroup runs in
+        LDS) as their precomputed dense matrix; 0 runs every grid in a launch
+        of its own. The default is 5. Bits 1 and 2 leave the result the same
+        bit for bit; bit 4 changes it by rounding."""
+        _ffi.check(_ffi.lib().hm_vcycle_set_launches(self.handle, int(mode)))
         return self
 
     def apply(self, r, out=None):

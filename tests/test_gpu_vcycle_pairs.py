@@ -1,13 +1,18 @@
-"""Several coarse V-cycle levels per launch (csrc/pair.h: ascent chains,
-descent pairs) against one level per launch: the cycle, and PCG with it,
-must agree bit for bit, in float64 and float32, on grids with odd and thin
-coarse levels. The one-level path is itself pinned against the oracle's
-vcycle (test_gpu_solver.py)."""
-import numpy as np
+"""The V-cycle's launch structures (hm_vcycle_set_launches) against one grid
+per launch. Ascent chains and descent pairs (csrc/pair.h) recompute
+intermediate grids on halos with the same arithmetic, so the cycle, and PCG
+with it, must agree bit for bit, in float64 and float32, on grids with odd
+and thin coarse levels. The tail applied as its precomputed matrix
+(csrc/vcycle.h) changes the cycle by rounding only. The one-grid-per-launch
+path is itself pinned against the oracle's vcycle (test_gpu_solver.py)."""
+import ctypes
+
 import pytest
 import torch
 
 pytestmark = pytest.mark.gpu
+
+CHAIN, PAIR_DOWN, TAIL_MATRIX = 1, 2, 4
 
 
 def _solver(h, w, seed, precision):
@@ -18,44 +23,8 @@ def _solver(h, w, seed, precision):
                            precision=precision, level0="stencil")
 
 
-@pytest.mark.parametrize("precision", ["f64", "mixed"])
-@pytest.mark.parametrize("shape", [(1080, 1920), (333, 517), (129, 700), (700, 66), (190, 190)])
-def test_pairs_equal_single_levels(shape, precision):
-    h, w = shape
-    print("pairs vs single levels", shape, precision, flush=True)
-    s = _solver(h, w, 7, precision)
-    M = s.M
-    dt = torch.float64 if precision == "f64" else torch.float32
-    g = torch.Generator().manual_seed(h * 7 + w)
-    r = torch.rand(h * w, generator=g, dtype=torch.float64).to(dt).cuda()
-    z_single = M.set_pairs(0).apply(r).clone()
-    torch.cuda.synchronize()
-    assert torch.isfinite(z_single).all()
-    for mode in (1, 2, 3):
-        assert torch.equal(M.set_pairs(mode).apply(r), z_single), mode
-    # A second application (the hierarchy's buffers hold the previous cycle).
-    r2 = torch.rand(h * w, generator=g, dtype=torch.float64).to(dt).cuda()
-    z2 = M.set_pairs(0).apply(r2).clone()
-    assert torch.equal(M.set_pairs(3).apply(r2), z2)
-    assert torch.equal(M.set_pairs(1).apply(r2), z2)
-    if shape == (1080, 1920):  # grids 480x270 .. 60x33 run together
-        down, up = _pair_launches(lambda: M.set_pairs(1).apply(r))
-        assert down == 0 and 1 <= up <= 2
-        down, up = _pair_launches(lambda: M.set_pairs(3).apply(r))
-        assert down == 2 and 1 <= up <= 2
-        assert _pair_launches(lambda: M.set_pairs(0).apply(r)) == (0, 0)
-    M.set_pairs(1)
-
-
-def test_set_pairs_rejects_unknown_modes():
-    from pymatting_amd import _ffi
-    s = _solver(64, 64, 1, "f64")
-    with pytest.raises(Exception, match="pairs must be 0..3"):
-        s.M.set_pairs(4)
-
-
-def _pair_launches(fn):
-    import ctypes
+def _launches(fn):
+    """(descent pairs, ascent chains, tail launches) that fn() made."""
     import bench
     from pymatting_amd import _ffi
     L = _ffi.lib()
@@ -64,7 +33,7 @@ def _pair_launches(fn):
     fn()
     torch.cuda.synchronize()
     out = []
-    for kind in ("pair_down", "pair_up"):
+    for kind in ("pair_down", "pair_up", "tail"):
         total = 0
         for prec in (0, 1):
             ms, cnt, pts = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64()
@@ -77,16 +46,60 @@ def _pair_launches(fn):
 
 
 @pytest.mark.parametrize("precision", ["f64", "mixed"])
-def test_pcg_with_pairs_equals_single_levels(precision):
+@pytest.mark.parametrize("shape", [(1080, 1920), (333, 517), (129, 700), (700, 66), (190, 190)])
+def test_launch_structures_agree(shape, precision):
+    h, w = shape
+    print("launch structures", shape, precision, flush=True)
+    s = _solver(h, w, 7, precision)
+    M = s.M
+    dt = torch.float64 if precision == "f64" else torch.float32
+    g = torch.Generator().manual_seed(h * 7 + w)
+    r = torch.rand(h * w, generator=g, dtype=torch.float64).to(dt).cuda()
+    r2 = torch.rand(h * w, generator=g, dtype=torch.float64).to(dt).cuda()
+    z_kernel = None
+    for tail in (0, TAIL_MATRIX):
+        z_single = M.set_launches(tail).apply(r).clone()
+        z2 = M.set_launches(tail).apply(r2).clone()
+        torch.cuda.synchronize()
+        assert torch.isfinite(z_single).all()
+        for mode in (CHAIN, PAIR_DOWN, CHAIN | PAIR_DOWN):
+            assert torch.equal(M.set_launches(mode | tail).apply(r), z_single), (mode, tail)
+            # Again: the hierarchy's buffers hold the previous cycle's values.
+            assert torch.equal(M.set_launches(mode | tail).apply(r2), z2), (mode, tail)
+        if tail == 0:
+            z_kernel = z_single
+    # The tail as a matrix: the same map up to the rounding of its sums.
+    z_matrix = M.set_launches(TAIL_MATRIX).apply(r)
+    rel = float((z_matrix - z_kernel).abs().max() / z_kernel.abs().max())
+    assert rel < (1e-12 if precision == "f64" else 1e-5), rel
+    if shape == (1080, 1920):  # grids 480x270 .. 60x33 run in one ascent launch
+        assert _launches(lambda: M.set_launches(CHAIN | TAIL_MATRIX).apply(r)) == (0, 1, 1)
+        assert _launches(lambda: M.set_launches(CHAIN | PAIR_DOWN).apply(r)) == (2, 1, 1)
+        assert _launches(lambda: M.set_launches(0).apply(r)) == (0, 0, 1)
+    M.set_launches(CHAIN | TAIL_MATRIX)
+
+
+def test_set_launches_rejects_unknown_modes():
+    s = _solver(64, 64, 1, "f64")
+    with pytest.raises(ValueError, match="launch structure must be 0..7"):
+        s.M.set_launches(8)
+
+
+@pytest.mark.parametrize("precision", ["f64", "mixed"])
+def test_pcg_with_launch_structures(precision):
     kw = dict(rtol=1e-7)
     if precision == "mixed":
         kw.update(mixed="reliable")
     s = _solver(270, 480, 3, precision)
-    s.M.set_pairs(0)
+    s.M.set_launches(0)
     a_single = s.solve(**kw).clone()
     it_single = s.stats["iterations"]
-    for mode in (1, 3):
-        s.M.set_pairs(mode)
+    for mode in (CHAIN, CHAIN | PAIR_DOWN):
+        s.M.set_launches(mode)
         a_pairs = s.solve(**kw)
         assert s.stats["iterations"] == it_single
         assert torch.equal(a_pairs, a_single)
+    s.M.set_launches(CHAIN | TAIL_MATRIX)
+    a_matrix = s.solve(**kw)
+    assert abs(s.stats["iterations"] - it_single) <= 1
+    assert float((a_matrix - a_single).abs().max()) < (1e-9 if precision == "f64" else 1e-5)
