@@ -88,10 +88,13 @@ struct hm_vc {
     // hierarchy's precision; built once for the tail starting at tail_start.
     void* tail_pack = nullptr;
     size_t tail_start = 0;
-    // The tail as a dense row-major matrix over its top grid (tail_mat_n
-    // points), applied instead of the tail's launch when present.
+    // The tail as a dense row-major tail_rows x tail_cols matrix, applied
+    // instead of the tail's launch when present: from the right-hand side of
+    // grid tail_src (the tail's top grid, or the grid above it with its
+    // restriction folded in) to the top grid's correction.
     void* tail_mat = nullptr;
-    int tail_mat_n = 0;
+    int tail_rows = 0, tail_cols = 0;
+    size_t tail_src = 0;
     // Launch structure of the coarse levels, a bit mask: 1 runs consecutive
     // levels of the ascent in one launch, 2 pairs of levels of the descent
     // (pair.h), 4 applies the tail as its dense matrix (vcycle.h); 0 runs

@@ -539,7 +539,9 @@ struct VcPlan {
                                   // starting at up_start[l] (else -1)
 };
 
-inline VcPlan vc_plan(const hm_vc* vc, size_t tail) {
+// desc_end: the descent restricts grids [0, desc_end) (the tail, or the grid
+// whose right-hand side the tail matrix reads).
+inline VcPlan vc_plan(const hm_vc* vc, size_t tail, size_t desc_end) {
     const size_t L = vc->levels.size();
     VcPlan p;
     p.down_pair.assign(L, 0);
@@ -548,7 +550,7 @@ inline VcPlan vc_plan(const hm_vc* vc, size_t tail) {
     size_t l0 = HM_CHAIN_FROM < 1 ? 1 : HM_CHAIN_FROM;
     while (l0 < L && vc->levels[l0].n >= (1LL << HM_CHAIN_MAX_LOG2)) ++l0;
     if (vc->pairs & kVcPairDown)
-        for (size_t l = l0; l + 1 < tail && l + 2 < L; l += 2) p.down_pair[l] = 1;
+        for (size_t l = l0; l + 1 < desc_end && l + 2 < L; l += 2) p.down_pair[l] = 1;
     if (vc->pairs & kVcChainUp) {
         for (size_t l = l0; l + 1 < tail && l + 1 < L;) {
             const size_t len = std::min<size_t>(HM_CHAIN_LEN, tail - l);

@@ -92,10 +92,16 @@ int vcycle_apply_t(hm_vc* vc, const T* r, T* z, CgState* st, const Reducer* red,
     }
     std::vector<T*> result(L, nullptr), pre_x(L, nullptr);
     const size_t tail = vc_tail_start(vc);
+    // With the tail applied as its matrix, the matrix may start at the grid
+    // above the tail (its restriction folded in): the descent stops there.
+    size_t tail_src = tail;
+    const bool tail_mat = tail < L - 1 && vc->coarse_sweeps == 0 &&
+                          vc_tail_matrix(vc, tail, &tail_src);
+    const size_t desc_end = tail_mat ? tail_src : tail;
     // Coarse levels that run several to a launch (pair.h).
-    const VcPlan plan = vc_plan(vc, tail);
+    const VcPlan plan = vc_plan(vc, tail, desc_end);
     // Descend: pre-smooth, residual, restrict.
-    for (size_t l = 0; l + 1 < L && l < tail; ++l) {
+    for (size_t l = 0; l + 1 < L && l < desc_end; ++l) {
         if (plan.down_pair[l]) {
             HM_TRY(launch_pair_down<T>(vc, l, guard, s));
             ++l;  // level l + 1 restricted too

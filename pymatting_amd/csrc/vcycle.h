@@ -718,14 +718,17 @@ __device__ __forceinline__ T wave_sum1(T v) {
     return v;
 }
 
-// x = M b for the n x n row-major M (n <= kTailMaxTop): workgroups below
-// row_blocks take four rows each, one per wavefront; the others stream the CG
-// solution update beside them (TailSide, as in k_vc_tail).
+// x = M b for the rows x cols row-major M (rows <= kTailMaxTop): workgroups
+// below row_blocks take four rows each, one per wavefront, in chunks of
+// 64 x kMatvecU columns; the others stream the CG solution update beside
+// them (TailSide, as in k_vc_tail).
+constexpr int kMatvecU = 8;
+
 template <typename T, bool kSide>
 __global__ __launch_bounds__(kMatvecBlock) void k_tail_matvec(const T* __restrict__ M,
                                                               const T* __restrict__ b,
-                                                              T* __restrict__ x, int n,
-                                                              unsigned row_blocks,
+                                                              T* __restrict__ x, int rows,
+                                                              int cols, unsigned row_blocks,
                                                               const CgState* guard,
                                                               TailSide side) {
     if (solve_done(guard)) return;
@@ -745,19 +748,20 @@ __global__ __launch_bounds__(kMatvecBlock) void k_tail_matvec(const T* __restric
     }
     const int lane = threadIdx.x % kWave;
     const int r = (int)blockIdx.x * kMatvecRowsPerBlock + (int)threadIdx.x / kWave;
-    if (r >= n) return;
-    constexpr int U = kTailMaxTop / kWave;
-    const T* row = M + (size_t)r * n;
-    T mv[U], bv[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const int j = lane + u * kWave;
-        mv[u] = j < n ? row[j] : T(0);
-        bv[u] = j < n ? b[j] : T(0);
-    }
+    if (r >= rows) return;
+    const T* row = M + (size_t)r * cols;
     T acc = 0;
+    for (int j0 = 0; j0 < cols; j0 += kWave * kMatvecU) {
+        T mv[kMatvecU], bv[kMatvecU];
 #pragma unroll
-    for (int u = 0; u < U; ++u) acc = fmadd(mv[u], bv[u], acc);
+        for (int u = 0; u < kMatvecU; ++u) {
+            const int j = j0 + lane + u * kWave;
+            mv[u] = j < cols ? row[j] : T(0);
+            bv[u] = j < cols ? b[j] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kMatvecU; ++u) acc = fmadd(mv[u], bv[u], acc);
+    }
     acc = wave_sum1(acc);
     if (lane == 0) x[r] = acc;
 }
@@ -884,8 +888,80 @@ int build_tail_pack(hm_vc* vc, size_t start, hipStream_t s) {
     return HM_OK;
 }
 
+// Folds the restriction of the grid f above the tail (one damped-Jacobi
+// sweep from zero, residual, full weighting; vcycle.py:62-80) into the tail
+// matrix: F = Tm G, with Tm the nt x nt tail matrix and G the nt x nf map
+// b_f -> b_t, so that x_t = F b_f replaces the restriction's launch and the
+// tail's. Column j of G is the restriction of e_j - A (omega / D_j) e_j,
+// whose residual lives on the 5 x 5 points around j, so it reaches at most
+// the 4 x 4 coarse points from ((jx - 2) >> 1, (jy - 2) >> 1) on.
+// k_tail_fold_g: one thread per (column j, coarse slot c) writes Gc[j][c].
+template <typename T>
+__global__ __launch_bounds__(256) void k_tail_fold_g(int ht, int wt,
+                                                     const T* __restrict__ coef,
+                                                     const T* __restrict__ iD, int hf, int wf,
+                                                     T omega, T* __restrict__ Gc) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nf = hf * wf;
+    if (t >= 16LL * nf) return;
+    const int j = (int)(t / 16), c = (int)(t % 16);
+    const int jy = j / wf, jx = j - jy * wf;
+    const int kx = ((jx - 2) >> 1) + c % 4, ky = ((jy - 2) >> 1) + c / 4;
+    T v = T(0);
+    if (kx >= 0 && kx < wt && ky >= 0 && ky < ht && 2 * kx + 1 >= jx - 2 && 2 * kx - 1 <= jx + 2 &&
+        2 * ky + 1 >= jy - 2 && 2 * ky - 1 <= jy + 2) {
+        const T z0 = vc_jac0(omega, T(1), iD[j]);
+        T r[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const int iy = 2 * ky - 1 + a, ix = 2 * kx - 1 + q;
+                const int dy = iy - jy, dx = ix - jx;
+                T rv = T(0);
+                if (dy >= -2 && dy <= 2 && dx >= -2 && dx <= 2 && iy >= 0 && iy < hf && ix >= 0 &&
+                    ix < wf) {
+                    const int i = iy * wf + ix;
+                    const int s = (2 - dy) * 5 + (2 - dx);  // slot of row i that couples to j
+                    rv = (i == j ? T(1) : T(0)) - sym_coef32(coef, nf, wf, i, s) * z0;
+                }
+                r[a][q] = rv;
+            }
+        v = vc_restrict9(&r[0][1], &r[1][1], &r[2][1]);
+    }
+    Gc[t] = v;
+}
+
+// k_tail_fold_rows: one workgroup per row i of F, F[i][j] = sum_c
+// Tm[i][K(j, c)] Gc[j][c], with Tm's row i staged in LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tail_fold_rows(const T* __restrict__ Tm, int ht, int wt,
+                                                        const T* __restrict__ Gc, int hf, int wf,
+                                                        T* __restrict__ F) {
+    __shared__ T row[kTailMaxTop];
+    const int i = blockIdx.x;
+    const int nt = ht * wt, nf = hf * wf;
+    for (int k = threadIdx.x; k < nt; k += blockDim.x) row[k] = Tm[(size_t)i * nt + k];
+    __syncthreads();
+    for (int j = threadIdx.x; j < nf; j += blockDim.x) {
+        const int jy = j / wf, jx = j - jy * wf;
+        const int kx0 = (jx - 2) >> 1, ky0 = (jy - 2) >> 1;
+        T acc = 0;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            const int kx = kx0 + c % 4, ky = ky0 + c / 4;
+            if (kx >= 0 && kx < wt && ky >= 0 && ky < ht)
+                acc = fmadd(row[ky * wt + kx], Gc[(size_t)j * 16 + c], acc);
+        }
+        F[(size_t)i * nf + j] = acc;
+    }
+}
+
 // The tail of the hierarchy at `start` as a dense matrix (k_vc_tail_basis),
-// in the hierarchy's precision; needs the packed tail contents.
+// in the hierarchy's precision; needs the packed tail contents. With a cycle
+// of one pre-smoothing sweep and a grid f = start - 1 >= 1 above the tail,
+// f's restriction is folded in (k_tail_fold_*) and the matrix maps b_f to the
+// tail's correction.
 template <typename T>
 int build_tail_matrix(hm_vc* vc, size_t start, hipStream_t s) {
     TailParams<T> P;
@@ -893,44 +969,94 @@ int build_tail_matrix(hm_vc* vc, size_t start, hipStream_t s) {
     const long long n = vc->levels[start].n;
     dev_free(vc->tail_mat);
     vc->tail_mat = nullptr;
-    vc->tail_mat_n = 0;
+    vc->tail_rows = vc->tail_cols = 0;
+    vc->tail_src = 0;
     if (n > kTailMaxTop) {
         set_error("internal: tail grid of %lld points", n);
         return HM_ERR_ARG;
     }
-    HM_TRY(dev_alloc(&vc->tail_mat, sizeof(T) * (size_t)(n * n)));
+    void* tm = nullptr;
+    HM_TRY(dev_alloc(&tm, sizeof(T) * (size_t)(n * n)));
     HM_TRY(set_max_dynamic_lds(reinterpret_cast<const void*>(&k_vc_tail_basis<T>),
                                kTailLdsBytes));
-    k_vc_tail_basis<T><<<(unsigned)n, kTailBlock, lds, s>>>(P, static_cast<T*>(vc->tail_mat));
+    k_vc_tail_basis<T><<<(unsigned)n, kTailBlock, lds, s>>>(P, static_cast<T*>(tm));
     HM_LAUNCH_CHECK();
-    vc->tail_mat_n = (int)n;
+#ifndef HM_TAIL_FOLD
+#define HM_TAIL_FOLD 1
+#endif
+    if (HM_TAIL_FOLD && vc->pre == 1 && start >= 2) {
+        const VcLevel& Lf = vc->levels[start - 1];
+        const VcLevel& Lt = vc->levels[start];
+        void* F = nullptr;
+        void* Gc = nullptr;
+        int rc = dev_alloc(&F, sizeof(T) * (size_t)(Lt.n * Lf.n));
+        if (rc == HM_OK) rc = dev_alloc(&Gc, sizeof(T) * 16 * (size_t)Lf.n);
+        if (rc != HM_OK) {
+            dev_free(tm);
+            dev_free(F);
+            return rc;
+        }
+        k_tail_fold_g<T><<<div_up(16 * Lf.n, 256), 256, 0, s>>>(
+            Lt.h, Lt.w, pick<T>(Lf.A->c64, Lf.A->c32), pick<T>(Lf.A->idiag64, Lf.A->idiag32),
+            Lf.h, Lf.w, (T)vc->omega, static_cast<T*>(Gc));
+        k_tail_fold_rows<T><<<(unsigned)Lt.n, 256, 0, s>>>(
+            static_cast<const T*>(tm), Lt.h, Lt.w, static_cast<const T*>(Gc), Lf.h, Lf.w,
+            static_cast<T*>(F));
+        const hipError_t e = hipGetLastError();
+        // Back to the pool: a later allocation reuses them in stream order.
+        dev_free(tm);
+        dev_free(Gc);
+        if (e != hipSuccess) {
+            dev_free(F);
+            set_error("tail fold failed: %s", hipGetErrorString(e));
+            return HM_ERR_HIP;
+        }
+        vc->tail_mat = F;
+        vc->tail_rows = (int)Lt.n;
+        vc->tail_cols = (int)Lf.n;
+        vc->tail_src = start - 1;
+        return HM_OK;
+    }
+    vc->tail_mat = tm;
+    vc->tail_rows = vc->tail_cols = (int)n;
+    vc->tail_src = start;
     return HM_OK;
 }
 
 constexpr unsigned kMatvecSideBlocks = 4 * kTailSideBlocks;
 constexpr int kVcTailMatrix = 4;  // launch structure bit (hm_vc::pairs): tail as a matrix
 
+// Whether a cycle starting its tail at `tail` applies the tail matrix, and
+// the grid whose right-hand side it reads (the descent stops above it).
+inline bool vc_tail_matrix(const hm_vc* vc, size_t tail, size_t* src) {
+    const bool on = (vc->pairs & kVcTailMatrix) && vc->tail_mat && tail == vc->tail_start &&
+                    vc->tail_rows == (int)vc->levels[tail].n;
+    if (on && src) *src = vc->tail_src;
+    return on;
+}
+
 template <typename T>
 int launch_tail(hm_vc* vc, size_t start, const CgState* guard, hipStream_t s,
                 const TailSide* side = nullptr) {
-    if ((vc->pairs & kVcTailMatrix) && vc->tail_mat && vc->tail_mat_n == vc->levels[start].n &&
-        start == vc->tail_start) {
+    size_t src = start;
+    if (vc_tail_matrix(vc, start, &src)) {
         const VcLevel& V = vc->levels[start];
-        const int n = (int)V.n;
-        const unsigned rows = div_up(n, kMatvecRowsPerBlock);
-        ProfScope prof(kProfDense, n, sizeof(T) == 4, s);
+        const T* b = static_cast<const T*>(vc->levels[src].b);
+        const int rows = vc->tail_rows, cols = vc->tail_cols;
+        const unsigned rb = div_up(rows, kMatvecRowsPerBlock);
+        ProfScope prof(kProfDense, rows, sizeof(T) == 4, s);
         if (side && side->x) {
             if (!side->p || !side->st || side->n < 0) {
                 set_error("internal: incomplete V-cycle tail side work");
                 return HM_ERR_ARG;
             }
-            k_tail_matvec<T, true><<<rows + kMatvecSideBlocks, kMatvecBlock, 0, s>>>(
-                static_cast<const T*>(vc->tail_mat), static_cast<const T*>(V.b),
-                static_cast<T*>(V.x), n, rows, guard, *side);
+            k_tail_matvec<T, true><<<rb + kMatvecSideBlocks, kMatvecBlock, 0, s>>>(
+                static_cast<const T*>(vc->tail_mat), b, static_cast<T*>(V.x), rows, cols, rb,
+                guard, *side);
         } else {
-            k_tail_matvec<T, false><<<rows, kMatvecBlock, 0, s>>>(
-                static_cast<const T*>(vc->tail_mat), static_cast<const T*>(V.b),
-                static_cast<T*>(V.x), n, rows, guard, TailSide{});
+            k_tail_matvec<T, false><<<rb, kMatvecBlock, 0, s>>>(
+                static_cast<const T*>(vc->tail_mat), b, static_cast<T*>(V.x), rows, cols, rb,
+                guard, TailSide{});
         }
         HM_LAUNCH_CHECK();
         return HM_OK;
