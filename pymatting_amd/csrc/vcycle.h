@@ -639,33 +639,49 @@ struct TailSide {
 constexpr unsigned kTailSideBlocks = 255;
 
 // x += alpha p over workgroup b of nb workgroups of BS threads (the
-// arithmetic of k_cg_update_pq).
+// arithmetic of k_cg_update_pq), four consecutive elements per access.
 template <typename TX, typename T, int BS>
 __device__ __forceinline__ void tail_side_update_n(TX* __restrict__ x, const T* __restrict__ p,
                                                    long long n, double alpha, unsigned b,
                                                    unsigned nb) {
     const T alphaT = (T)alpha;
-    constexpr int U = 4;
+    const auto upd = [&](TX xv, T pv) -> TX {
+        if constexpr (std::is_same<TX, T>::value)
+            return xv + alphaT * pv;
+        else
+            return xv + alpha * (TX)pv;
+    };
+    constexpr int U = 2;
+    const long long nq = n / 4;
     const long long stride = (long long)nb * BS;
-    for (long long i0 = (long long)b * BS + threadIdx.x; i0 < n; i0 += U * stride) {
-        TX xv[U];
-        T pv[U];
+    Vec4<TX>* __restrict__ x4 = reinterpret_cast<Vec4<TX>*>(x);
+    const Vec4<T>* __restrict__ p4 = reinterpret_cast<const Vec4<T>*>(p);
+    for (long long q0 = (long long)b * BS + threadIdx.x; q0 < nq; q0 += U * stride) {
+        Vec4<TX> xv[U];
+        Vec4<T> pv[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const long long i = i0 + u * stride;
-            const bool in = i < n;
-            xv[u] = in ? x[i] : TX(0);
-            pv[u] = in ? p[i] : T(0);
+            const long long q = q0 + u * stride;
+            if (q < nq) {
+                xv[u] = x4[q];
+                pv[u] = p4[q];
+            }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const long long i = i0 + u * stride;
-            if (i >= n) continue;
-            if constexpr (std::is_same<TX, T>::value)
-                x[i] = xv[u] + alphaT * pv[u];
-            else
-                x[i] = xv[u] + alpha * (TX)pv[u];
+            const long long q = q0 + u * stride;
+            if (q >= nq) continue;
+            Vec4<TX> o;
+            o.a = upd(xv[u].a, pv[u].a);
+            o.b = upd(xv[u].b, pv[u].b);
+            o.c = upd(xv[u].c, pv[u].c);
+            o.d = upd(xv[u].d, pv[u].d);
+            x4[q] = o;
         }
+    }
+    if (b == 0 && threadIdx.x < (unsigned)(n & 3)) {  // the last n % 4 elements
+        const long long i = (n & ~3LL) + threadIdx.x;
+        x[i] = upd(x[i], p[i]);
     }
 }
 
@@ -1023,7 +1039,6 @@ int build_tail_matrix(hm_vc* vc, size_t start, hipStream_t s) {
     return HM_OK;
 }
 
-constexpr unsigned kMatvecSideBlocks = 4 * kTailSideBlocks;
 constexpr int kVcTailMatrix = 4;  // launch structure bit (hm_vc::pairs): tail as a matrix
 
 // Whether a cycle starting its tail at `tail` applies the tail matrix, and
@@ -1050,7 +1065,8 @@ int launch_tail(hm_vc* vc, size_t start, const CgState* guard, hipStream_t s,
                 set_error("internal: incomplete V-cycle tail side work");
                 return HM_ERR_ARG;
             }
-            k_tail_matvec<T, true><<<rb + kMatvecSideBlocks, kMatvecBlock, 0, s>>>(
+            const unsigned sb = div_up(side->n, (long long)kMatvecBlock * 8);
+            k_tail_matvec<T, true><<<rb + sb, kMatvecBlock, 0, s>>>(
                 static_cast<const T*>(vc->tail_mat), b, static_cast<T*>(V.x), rows, cols, rb,
                 guard, *side);
         } else {
