@@ -639,7 +639,8 @@ struct TailSide {
 constexpr unsigned kTailSideBlocks = 255;
 
 // x += alpha p over workgroup b of nb workgroups of BS threads (the
-// arithmetic of k_cg_update_pq), four consecutive elements per access.
+// arithmetic of k_cg_update_pq), four consecutive elements per access when
+// both vectors are aligned for it.
 template <typename TX, typename T, int BS>
 __device__ __forceinline__ void tail_side_update_n(TX* __restrict__ x, const T* __restrict__ p,
                                                    long long n, double alpha, unsigned b,
@@ -652,7 +653,9 @@ __device__ __forceinline__ void tail_side_update_n(TX* __restrict__ x, const T* 
             return xv + alpha * (TX)pv;
     };
     constexpr int U = 2;
-    const long long nq = n / 4;
+    const bool vec = ((reinterpret_cast<uintptr_t>(x) % alignof(Vec4<TX>)) |
+                      (reinterpret_cast<uintptr_t>(p) % alignof(Vec4<T>))) == 0;
+    const long long nq = vec ? n / 4 : 0;
     const long long stride = (long long)nb * BS;
     Vec4<TX>* __restrict__ x4 = reinterpret_cast<Vec4<TX>*>(x);
     const Vec4<T>* __restrict__ p4 = reinterpret_cast<const Vec4<T>*>(p);
@@ -679,10 +682,9 @@ __device__ __forceinline__ void tail_side_update_n(TX* __restrict__ x, const T* 
             x4[q] = o;
         }
     }
-    if (b == 0 && threadIdx.x < (unsigned)(n & 3)) {  // the last n % 4 elements
-        const long long i = (n & ~3LL) + threadIdx.x;
+    // The elements after the last whole group (all of them when not aligned).
+    for (long long i = 4 * nq + (long long)b * BS + threadIdx.x; i < n; i += stride)
         x[i] = upd(x[i], p[i]);
-    }
 }
 
 template <typename TX, typename T>

@@ -103,3 +103,26 @@ def test_pcg_with_launch_structures(precision):
     a_matrix = s.solve(**kw)
     assert abs(s.stats["iterations"] - it_single) <= 1
     assert float((a_matrix - a_single).abs().max()) < (1e-9 if precision == "f64" else 1e-5)
+
+
+def test_unaligned_solution_vector():
+    """The CG updates stream four consecutive elements per access when the
+    vectors allow it and fall back to single elements otherwise: a solution
+    vector at an odd float64 offset gives the same solve up to the order of
+    the residual-norm sums."""
+    from pymatting_amd import _ffi
+    s = _solver(270, 480, 5, "mixed")
+    n = s.h * s.w
+    L = _ffi.lib()
+    out = []
+    for offset in (0, 1):
+        buf = torch.zeros(n + 4, dtype=torch.float64, device="cuda")
+        x = buf[offset:offset + n]
+        stats = _ffi.CgStats()
+        _ffi.check(L.hm_cg_solve_reliable(None, s.op.handle, s.M.handle, _ffi.ptr(s.b),
+                                          _ffi.ptr(x), 1e-7, 0.01, 10000, 8, 0,
+                                          ctypes.byref(stats), _ffi.stream()))
+        out.append((x.clone(), stats.iterations))
+    (x0, it0), (x1, it1) = out
+    assert abs(it0 - it1) <= 1
+    assert float((x0 - x1).abs().max()) < 1e-6 * float(x0.abs().max())
