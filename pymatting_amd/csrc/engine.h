@@ -205,6 +205,7 @@ __device__ __forceinline__ T vc_stencil(const C& c, const X& x) {
 template <typename T, bool Rec = false>
 struct CfOp {
     static constexpr int kProfBase = Rec ? kProfCfRecApply : kProfCfApply;
+    static constexpr bool kPointPrefetch = false;
     CfDev d;
     const T* I;  // planar [3][n]
     // Per-window data at the centre pixel [9][n]: mean (3), inverse covariance (6).
@@ -406,6 +407,7 @@ struct DiaPrefetch<T, OX, OY, false> {};
 template <typename T, bool PF = false>
 struct DiaOp {
     static constexpr int kProfBase = kProfDiaApply;
+    static constexpr bool kPointPrefetch = PF;
     int h, w;
     const T* coef;  // [13][n] symmetric storage
 
@@ -726,6 +728,40 @@ struct IsRestrict { static constexpr bool value = false; };
 template <typename T, int S>
 struct IsRestrict<OutRestrict<T, S>> { static constexpr bool value = true; };
 
+// Large-grid restriction tiles hand out their (2CX+1) x (2CY+1) residual
+// points in an order aligned with the grid's 64-point rows: first the 2CX x
+// 2CY block the tile owns (local (1, 1) on, two rows per wavefront pair),
+// then the row above it, then the column left of both. A row-major walk of
+// the region starts one point before a 64-point boundary, and each
+// wavefront's coefficient loads then span three cache lines instead of two.
+template <class Out>
+struct AlignedRestrict { static constexpr bool value = false; };
+template <typename T>
+struct AlignedRestrict<OutRestrict<T, 0>> { static constexpr bool value = true; };
+
+// Local coordinates (in the output region) of the tile's t-th point.
+template <class Out>
+__device__ __forceinline__ void out_local(int t, int& lx, int& ly) {
+    if constexpr (AlignedRestrict<Out>::value) {
+        constexpr int BX = 2 * Out::CX, BY = 2 * Out::CY;
+        static_assert(Out::OX == BX + 1 && Out::OY == BY + 1, "restriction region");
+        if (t < BX * BY) {
+            const int q = t / BX;
+            ly = 1 + q;
+            lx = 1 + (t - q * BX);
+        } else if (t < BX * BY + BX) {
+            ly = 0;
+            lx = 1 + (t - BX * BY);
+        } else {
+            ly = t - BX * BY - BX;
+            lx = 0;
+        }
+    } else {
+        ly = t / Out::OX;
+        lx = t - ly * Out::OX;
+    }
+}
+
 // -------------------------------------------------------------------- kernel
 
 // Workgroups are dispatched round-robin over the XCDs (workgroup b runs on
@@ -769,6 +805,9 @@ __device__ __forceinline__ void tile_body(const Op& op, const In& in, const Out&
     __shared__ typename Op::template Shared<OX, OY> sh;
     __shared__ T sres[IsRestrict<Out>::value ? OX * OY : 1];
     const int ox0 = out.ox0(bx), oy0 = out.oy0(by);
+    // Operators that prefetch per output point walk the region row-major.
+    static_assert(!AlignedRestrict<Out>::value || !Op::kPointPrefetch,
+                  "aligned restriction tiles need operators without per-point prefetch");
     typename Op::template Prefetch<OX, OY> pf;
     op.template prefetch<OX, OY>(pf, ox0, oy0);
     // Per-point output operands (b, D), loaded before the barrier.
@@ -777,7 +816,8 @@ __device__ __forceinline__ void tile_body(const Op& op, const In& in, const Out&
 #pragma unroll
         for (int k = 0; k < NO; ++k) {
             const int t = threadIdx.x + k * kBlock;
-            const int ly = t / OX, lx = t - ly * OX;
+            int lx, ly;
+            out_local<Out>(t, lx, ly);
             const int gx = ox0 + lx, gy = oy0 + ly;
             const bool in = t < OX * OY && gx >= 0 && gx < w && gy >= 0 && gy < h;
             const long long g = in ? (long long)gy * w + gx : 0;
@@ -824,7 +864,8 @@ __device__ __forceinline__ void tile_body(const Op& op, const In& in, const Out&
     for (int k = 0; k < NO; ++k) {
         const int t = threadIdx.x + k * kBlock;
         if (t >= OX * OY) continue;
-        const int ly = t / OX, lx = t - ly * OX;
+        int lx, ly;
+        out_local<Out>(t, lx, ly);
         const int gx = ox0 + lx, gy = oy0 + ly;
         if (gx < 0 || gx >= w || gy < 0 || gy >= h) {
             o.outside(lx, ly);
