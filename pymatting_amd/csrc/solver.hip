@@ -503,6 +503,19 @@ __global__ __launch_bounds__(kFinBlock) void k_cg_update_pq(TX* __restrict__ x,
     block_partials<kFinBlock, 1>(v, red);
 }
 
+// Length of the next batch of iterations enqueued between two read-backs of
+// the state: up to where the measured rate (||r||^2 ratio rho2 per
+// iteration) predicts ||r||^2 < conv, between 1 and 2K iterations, or K
+// while no rate is known. A batch that ends at the converging iteration
+// leaves no frozen iterations behind it, and long batches keep read-backs
+// from interrupting the stream of launches.
+inline int next_batch(int K, double rr, double rho2, double conv) {
+    if (!(rho2 > 0.0 && rho2 < 1.0)) return K;
+    if (!(rr > conv)) return 1;
+    const double j = std::ceil(std::log(conv / rr) / std::log(rho2));
+    return (int)std::max(1.0, std::min(j, 2.0 * K));
+}
+
 // Preconditioned CG (cg.py:4-117) on device vectors of type T with float64
 // scalars. Returns HM_ERR_NOCONV if maxiter iterations did not converge;
 // stats are filled in either case. With host callbacks (callable A or M, or a
@@ -599,8 +612,11 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
     pq.defer = fused;
     const unsigned pq_blocks =
         fused ? tile_map(OutApply<T>{}, A.h(), A.w(), A.h() / 2, A.w() / 2).ntiles : 0;
+    int kb = check_every;  // length of the next batch (next_batch once a rate is known)
+    double rr_prev = norm_r * norm_r, rho2 = 0.0;
+    int it_prev = 0;
     while (enqueued < maxiter) {
-        const int k = (maxiter - enqueued < check_every) ? (maxiter - enqueued) : check_every;
+        const int k = std::min(maxiter - enqueued, kb);
         for (int it = 0; it < k; ++it) {
             if (opt.reorth) {
                 k_copy<T><<<div_up(n, 256), 256, 0, s>>>(r.as<T>(), rold.as<T>(), n, st.dev);
@@ -647,6 +663,16 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
         enqueued += k;
         HM_TRY(st.pull(s));
         if (st.host->done || st.host->nonfinite) break;
+        if (!interactive) {
+            const double rr = st.host->rr;
+            const int its = st.host->iters;
+            if (its > it_prev && rr < rr_prev)
+                rho2 = std::pow(rr / rr_prev, 1.0 / (its - it_prev));
+            rr_prev = rr;
+            it_prev = its;
+            const double tol = std::max(opt.atol, opt.rtol * norm_b);
+            kb = next_batch(check_every, rr, rho2, tol * tol);
+        }
     }
     stats->iterations = st.host->iters;
     stats->norm_r = std::sqrt(st.host->rr);
@@ -823,7 +849,10 @@ int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x
     // the last batch) to have dropped by delta since the last replacement;
     // deciding this only at read-backs far apart lets the float32 recursion
     // drift from the true residual and stall the convergence.
-    constexpr int kReliableBatch = 8;
+#ifndef HM_RELIABLE_BATCH
+#define HM_RELIABLE_BATCH 8
+#endif
+    constexpr int kReliableBatch = HM_RELIABLE_BATCH;
     const int K = check_every < 1 ? kReliableBatch : std::min(check_every, kReliableBatch);
     // <p, A p> partials of the SpMV, summed by the update kernel.
     DotSink pq{st.dev, red, kDotPQ};
@@ -872,8 +901,9 @@ int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x
         }
     } graph_guard{graphs, execs};
     float* const p_first = p.as<float>();
+    int kb = K;  // length of the next batch
     while (true) {
-        const int k = std::min(K, maxiter - enqueued);
+        const int k = std::min(kb, maxiter - enqueued);
         if (k <= 0) break;
         const int rep_at = rep_idx < k ? rep_idx : -1;
         for (int it = 0; it < k; ++it) {
@@ -964,6 +994,7 @@ int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x
         if (its > it_prev && rr < rr_prev) rho2 = std::pow(rr / rr_prev, 1.0 / (its - it_prev));
         rr_prev = rr;
         it_prev = its;
+        kb = next_batch(K, rr, rho2, rtol * rtol * norm_b * norm_b);
         // Iterations from now until ||r||^2 < delta^2 rr_replaced.
         const double target = delta * delta * rr_replaced;
         rep_idx = -1;
@@ -971,7 +1002,7 @@ int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x
             rep_idx = 0;
         } else if (rho2 > 0.0 && rho2 < 1.0) {
             const double j = std::ceil(std::log(target / rr) / std::log(rho2));
-            if (j <= K) rep_idx = std::max(1, (int)j) - 1;
+            if (j <= kb) rep_idx = std::max(1, (int)j) - 1;
         }
     }
     stats->iterations = st.host->iters;
