@@ -60,19 +60,19 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of float32 kernels, from two rocprofv3 --pmc passes
 # over `python bench.py --pmc-probe` (FETCH_SIZE, then WRITE_SIZE, KiB per
 # dispatch), each scaled by what the same counter reads for a 256 MiB copy
-# (FETCH_SIZE x 1.999799, WRITE_SIZE x 1.0):
-#   1080p fine-grid restriction: 62031.1 KiB x 1.999799 + 2025.0 KiB
-#     = 129.1 MB (algorithmic 126.5 MB);
-#   1080p fine-grid prolongation + smoothing: 63410.0 KiB x 1.999799
-#     + 8100.0 KiB = 138.1 MB (algorithmic 134.8 MB);
-#   1080p stored-stencil SpMV: 57526.3 KiB x 1.999799 + 8100.0 KiB
+# (FETCH_SIZE x 1.999788, WRITE_SIZE x 1.0):
+#   1080p fine-grid restriction: 63541.5 KiB x 1.999788 + 2025.0 KiB
+#     = 132.2 MB (algorithmic 126.5 MB);
+#   1080p fine-grid prolongation + smoothing: 63341.6 KiB x 1.999788
+#     + 8100.0 KiB = 138.0 MB (algorithmic 134.8 MB);
+#   1080p stored-stencil SpMV: 57550.0 KiB x 1.999788 + 8100.0 KiB
 #     = 126.1 MB (algorithmic 124.4 MB);
-#   4K stored-stencil SpMV: 233140.9 KiB x 1.999799 + 32400.0 KiB
-#     = 510.6 MB (algorithmic 497.7 MB).
-TRAFFIC_RESTRICT_F32 = 129100557
-TRAFFIC_SMOOTH_F32 = 138145002
-TRAFFIC_SPMV_F32 = 126096506
-TRAFFIC_SPMV_F32_4K = 510602224
+#   4K stored-stencil SpMV: 233182.9 KiB x 1.999788 + 32400.0 KiB
+#     = 510.7 MB (algorithmic 497.7 MB).
+TRAFFIC_RESTRICT_F32 = 132192787
+TRAFFIC_SMOOTH_F32 = 138004242
+TRAFFIC_SPMV_F32 = 126144221
+TRAFFIC_SPMV_F32_4K = 510685591
 
 # Algorithmic bytes per grid point and launch, by kernel class, for vector
 # element size s (8 float64, 4 float32). Matrix-free passes read the image
