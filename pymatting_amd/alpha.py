@@ -147,7 +147,9 @@ class LambdaSolver:
                  level0="matrix_free", max_levels=0, coarse_sweeps=0):
         self.h, self.w = int(trimap.shape[0]), int(trimap.shape[1])
         self.precision = precision
-        self.classes, self.tdev = device_classes(trimap)
+        # The trimap's errors are raised once the operator's set-up is
+        # enqueued, which then overlaps the read-back of its class counts.
+        self.classes, self.tdev, check_trimap_counts = device_classes(trimap, deferred=True)
         self.cf = CfOperator(image, epsilon, 1, "lambda", self.classes, self.tdev, lambda_value)
         self.b = self.cf.rhs()
         # level0 "matrix_free": the stencil formed from the image and the
@@ -156,6 +158,7 @@ class LambdaSolver:
         # from the image tile (float64 arithmetic); "stencil": the 25-point
         # operator assembled once (exactly) and streamed (the stored ELL/DIA
         # form).
+        check_trimap_counts()
         if level0 not in ("matrix_free", "matrix_free_recompute", "stencil"):
             raise ValueError("level0 must be 'matrix_free', 'matrix_free_recompute' or "
                              "'stencil'")

@@ -52,6 +52,7 @@ struct CgState {
     double atol;
     double rtol;
     double aux;     // scratch scalar for one-off reductions
+    double rr0;     // ||r||^2 of the initial residual (kDotInitRes)
     int done;       // 1 once ||r|| < atol or ||r|| < rtol * ||b||
     int iters;      // completed iterations
     int nonfinite;  // set when a reduction produced NaN/Inf

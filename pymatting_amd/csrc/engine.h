@@ -572,7 +572,11 @@ enum DotKind {
     // <r, z> whose finalisation also completes the preceding CG update (its
     // ||r||^2 partials in slot 2 of the same reducer): one finalize launch
     // per iteration instead of two.
-    kDotUpdateRz = 6
+    kDotUpdateRz = 6,
+    // Start of a solve without a host round trip (cg.py:66-76): ||b|| of a
+    // solve from x = 0 (r = b; b = 0 is solved by x = 0 and freezes the
+    // solve), ||b|| alone, and the initial residual test.
+    kDotNormB = 7, kDotNormBOnly = 8, kDotInitRes = 9
 };
 
 // Applies a finished grid-wide sum to the solve state (cg.py:90-110).
@@ -601,6 +605,20 @@ __device__ __forceinline__ void dot_finalize(CgState* st, int kind, double s, do
             double beta = 1.0 / st->rz;  // cg.py:108-110
             st->rz = s;
             st->beta = beta * s;
+            break;
+        }
+        case kDotNormB:
+            st->norm_b = sqrt(s);
+            st->rr = s;
+            if (!(s > 0.0)) st->done = 1;
+            break;
+        case kDotNormBOnly:
+            st->norm_b = sqrt(s);
+            break;
+        case kDotInitRes: {  // cg.py:73-76
+            const double norm_r = sqrt(s);
+            st->rr = st->rr0 = s;
+            if (norm_r < st->atol || norm_r < st->rtol * st->norm_b) st->done = 1;
             break;
         }
         default:

@@ -420,9 +420,13 @@ __global__ __launch_bounds__(256) void k_gauss_jordan_lds(int n, const double* _
     }
 }
 
-static int gauss_jordan(int n, double* M, double* inv64, float* inv32, hipStream_t s) {
-    int* status = nullptr;
-    HM_TRY(dev_alloc(reinterpret_cast<void**>(&status), sizeof(int)));
+// status_out (a device int, nullable): the kernel's singularity flag is left
+// there for the caller to read at its own next synchronisation; without it
+// the flag is read back here.
+static int gauss_jordan(int n, double* M, double* inv64, float* inv32, hipStream_t s,
+                        int* status_out) {
+    int* status = status_out;
+    if (!status) HM_TRY(dev_alloc(reinterpret_cast<void**>(&status), sizeof(int)));
     HM_HIP(hipMemsetAsync(status, 0, sizeof(int), s));
     if (n <= kGjLds) {
         HM_TRY(set_max_dynamic_lds(reinterpret_cast<const void*>(&k_gauss_jordan_lds),
@@ -432,6 +436,7 @@ static int gauss_jordan(int n, double* M, double* inv64, float* inv32, hipStream
         k_gauss_jordan<<<1, 1024, 0, s>>>(n, M, inv64, inv32, status);
     }
     HM_LAUNCH_CHECK();
+    if (status_out) return HM_OK;
     int host_status = 0;
     HM_HIP(hipMemcpyAsync(&host_status, status, sizeof(int), hipMemcpyDeviceToHost, s));
     HM_HIP(hipStreamSynchronize(s));
@@ -443,7 +448,7 @@ static int gauss_jordan(int n, double* M, double* inv64, float* inv32, hipStream
     return HM_OK;
 }
 
-int dense_inverse(const hm_dia* A, double* inv64, float* inv32, hipStream_t s) {
+int dense_inverse(const hm_dia* A, double* inv64, float* inv32, hipStream_t s, int* status) {
     const int n = (int)A->n;
     if (n > kMaxDirect) {
         set_error("direct solve of a %d-point coarsest grid exceeds the %d-point limit", n,
@@ -454,7 +459,7 @@ int dense_inverse(const hm_dia* A, double* inv64, float* inv32, hipStream_t s) {
     HM_TRY(dev_alloc(reinterpret_cast<void**>(&M), sizeof(double) * (size_t)n * 2 * n));
     k_aug_from_dia<<<div_up((long long)n * 2 * n, 256), 256, 0, s>>>(A->c64, A->h, A->w, M);
     HM_LAUNCH_CHECK();
-    int rc = gauss_jordan(n, M, inv64, inv32, s);
+    int rc = gauss_jordan(n, M, inv64, inv32, s, status);
     dev_free(M);
     return rc;
 }
@@ -468,7 +473,7 @@ int dense_inverse_from_dense(const double* A, int n, double* inv64, float* inv32
     HM_TRY(dev_alloc(reinterpret_cast<void**>(&M), sizeof(double) * (size_t)n * 2 * n));
     k_aug_from_dense<<<div_up((long long)n * 2 * n, 256), 256, 0, s>>>(A, n, M);
     HM_LAUNCH_CHECK();
-    int rc = gauss_jordan(n, M, inv64, inv32, s);
+    int rc = gauss_jordan(n, M, inv64, inv32, s, nullptr);
     dev_free(M);
     return rc;
 }

@@ -135,7 +135,10 @@ int dia_finish(hm_dia* A, hipStream_t s);  // diagonal + float32 copies
 int dia_pack(const double* full, hm_dia* A, hipStream_t s);     // [25][n] -> [13][n]
 int dia_unpack(const hm_dia* A, double* full, hipStream_t s);   // [13][n] -> [25][n]
 constexpr int kMaxDirect = 2048;  // largest coarsest grid solved by the dense inverse
-int dense_inverse(const hm_dia* A, double* inv64, float* inv32, hipStream_t s);
+// status (a device int, nullable): left set when the matrix is singular, for
+// the caller to read at its next synchronisation; null reads it back here.
+int dense_inverse(const hm_dia* A, double* inv64, float* inv32, hipStream_t s,
+                  int* status = nullptr);
 int dense_inverse_from_dense(const double* A, int n, double* inv64, float* inv32, hipStream_t s);
 int csr_apply(const hm_csr* A, const double* x, double* y, CgState* st, Reducer* red, int kind,
               hipStream_t s);

@@ -551,10 +551,18 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
     const bool interactive =
         A.cb || M.kind == HM_PC_HOST || M.kind == HM_PC_ICHOL || opt.callback;
 
-    // norm_b = ||b|| (cg.py:66)
-    HM_TRY(launch_dot<T>(b, b, nullptr, n, st.dev, red, kDotAux, s));
-    HM_TRY(st.pull(s));
-    const double norm_b = std::sqrt(st.host->aux);
+    // norm_b = ||b|| (cg.py:66). With host callbacks it is read here; the
+    // other solves finalise it and the initial residual test on the device
+    // (kDotNormBOnly, kDotInitRes: a passing test freezes the solve) and read
+    // both with the first batch's state, with no round trip before it.
+    double norm_b = 0.0, norm_r = 0.0;
+    if (interactive) {
+        HM_TRY(launch_dot<T>(b, b, nullptr, n, st.dev, red, kDotAux, s));
+        HM_TRY(st.pull(s));
+        norm_b = std::sqrt(st.host->aux);
+    } else {
+        HM_TRY(launch_dot<T>(b, b, nullptr, n, st.dev, red, kDotNormBOnly, s));
+    }
     // x = x0 or 0; r = b - A x (cg.py:64-71)
     if (x0) {
         k_copy<T><<<div_up(n, 256), 256, 0, s>>>(x0, x, n, nullptr);
@@ -570,20 +578,36 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
             HM_LAUNCH_CHECK();
         }
     }
-    HM_TRY(launch_dot<T>(r.as<T>(), r.as<T>(), nullptr, n, st.dev, red, kDotAux, s));
-    HM_TRY(st.pull(s));
-    const double norm_r = std::sqrt(st.host->aux);
-    stats->norm_b = norm_b;
-    stats->norm_r = norm_r;
     stats->iterations = 0;
     stats->converged = 0;
-    if (norm_r < opt.atol || norm_r < opt.rtol * norm_b) {  // cg.py:75-76
-        stats->converged = 1;
-        return HM_OK;
+    if (interactive) {
+        HM_TRY(launch_dot<T>(r.as<T>(), r.as<T>(), nullptr, n, st.dev, red, kDotAux, s));
+        HM_TRY(st.pull(s));
+        norm_r = std::sqrt(st.host->aux);
+        stats->norm_b = norm_b;
+        stats->norm_r = norm_r;
+        if (norm_r < opt.atol || norm_r < opt.rtol * norm_b) {  // cg.py:75-76
+            stats->converged = 1;
+            return HM_OK;
+        }
+        st.host->norm_b = norm_b;
+        st.host->rr = norm_r * norm_r;
+        HM_TRY(st.push(s));
+    } else {
+        HM_TRY(launch_dot<T>(r.as<T>(), r.as<T>(), nullptr, n, st.dev, red, kDotInitRes, s));
+        if (maxiter == 0) {  // no batch: read the initial test here
+            HM_TRY(st.pull(s));
+            stats->norm_b = st.host->norm_b;
+            stats->norm_r = std::sqrt(st.host->rr);
+            if (st.host->done) {
+                stats->converged = 1;
+                return HM_OK;
+            }
+            set_error("Conjugate gradient descent did not converge within %d iterations",
+                      maxiter);
+            return HM_ERR_NOCONV;
+        }
     }
-    st.host->norm_b = norm_b;
-    st.host->rr = norm_r * norm_r;
-    HM_TRY(st.push(s));
     // z = M r; rz = <r, z> (cg.py:78-80). p = z comes from the first fused
     // direction update with beta = 0 and a zero previous direction.
     HM_TRY(pc_apply<T>(M, r.as<T>(), z.as<T>(), st.dev, red, kDotRzFirst, nullptr, n, s));
@@ -613,7 +637,7 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
     const unsigned pq_blocks =
         fused ? tile_map(OutApply<T>{}, A.h(), A.w(), A.h() / 2, A.w() / 2).ntiles : 0;
     int kb = check_every;  // length of the next batch (next_batch once a rate is known)
-    double rr_prev = norm_r * norm_r, rho2 = 0.0;
+    double rr_prev = norm_r * norm_r, rho2 = 0.0;  // (non-interactive: from the first read)
     int it_prev = 0;
     while (enqueued < maxiter) {
         const int k = std::min(maxiter - enqueued, kb);
@@ -662,6 +686,10 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
         }
         enqueued += k;
         HM_TRY(st.pull(s));
+        if (!interactive && enqueued == k) {  // the first read: ||b||, initial ||r||^2
+            norm_b = st.host->norm_b;
+            rr_prev = st.host->rr0;
+        }
         if (st.host->done || st.host->nonfinite) break;
         if (!interactive) {
             const double rr = st.host->rr;
@@ -676,6 +704,7 @@ int pcg_t(const OpHandle& A, const PcHandle& M, const T* b, const T* x0, T* x,
     }
     stats->iterations = st.host->iters;
     stats->norm_r = std::sqrt(st.host->rr);
+    stats->norm_b = st.host->norm_b;
     stats->converged = st.host->done;
     if (!st.host->done) {
         set_error("Conjugate gradient descent did not converge within %d iterations", maxiter);
@@ -809,24 +838,15 @@ int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x
     HM_TRY(st.push(s));
     stats->iterations = 0;
     stats->converged = 0;
-    // norm_b; x = 0; r64 = b; r = (float)b.
-    HM_TRY(launch_dot<double>(b, b, nullptr, n, st.dev, red, kDotAux, s));
-    HM_TRY(st.pull(s));
-    const double norm_b = std::sqrt(st.host->aux);
-    stats->norm_b = norm_b;
-    stats->norm_r = norm_b;
-    if (!(norm_b > 0.0)) {  // x = 0 solves b = 0
-        k_fill<double><<<div_up(n, 256), 256, 0, s>>>(x, 0.0, n);
-        HM_LAUNCH_CHECK();
-        stats->converged = 1;
-        return HM_OK;
-    }
+    // x = 0, r = (float)b. ||b|| is finalised on the device (kDotNormB, which
+    // also freezes the solve for b = 0, solved by x = 0) and reaches the host
+    // with the first batch's state: no round trip before the first batch.
     std::memset(st.host, 0, sizeof(CgState));
     st.host->atol = 0.0;
     st.host->rtol = rtol;
-    st.host->norm_b = norm_b;
-    st.host->rr = norm_b * norm_b;
     HM_TRY(st.push(s));
+    HM_TRY(launch_dot<double>(b, b, nullptr, n, st.dev, red, kDotNormB, s));
+    double norm_b = 0.0;  // set from the first read-back
     k_fill<double><<<div_up(n, 256), 256, 0, s>>>(x, 0.0, n);
     HM_LAUNCH_CHECK();
     k_replace_r<<<kUpdBlocks, kFinBlock, 0, s>>>(b, r.as<float>(), n, nullptr, red_upd);
@@ -858,8 +878,8 @@ int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x
     DotSink pq{st.dev, red, kDotPQ};
     pq.defer = true;
     const unsigned pq_blocks = tile_map(OutApply<float>{}, A.h(), A.w(), A.h() / 2, A.w() / 2).ntiles;
-    double rr_replaced = norm_b * norm_b;  // ||r||^2 at the last replacement
-    double rr_prev = rr_replaced, rho2 = 0.0;  // rate: ||r||^2 ratio per iteration
+    double rr_replaced = 0.0;  // ||r||^2 at the last replacement (from the first read-back)
+    double rr_prev = 0.0, rho2 = 0.0;  // rate: ||r||^2 ratio per iteration
     int it_prev = 0;   // iterations at the last read-back
     int rep_idx = -1;  // replacement index in the next batch (none before a rate is known)
     int enqueued = 0;
@@ -901,6 +921,14 @@ int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x
         }
     } graph_guard{graphs, execs};
     float* const p_first = p.as<float>();
+    if (maxiter <= 0) {  // no batch: read ||b|| here (b = 0 is solved by x = 0)
+        HM_TRY(st.pull(s));
+        stats->norm_b = stats->norm_r = st.host->norm_b;
+        if (!(st.host->norm_b > 0.0)) {
+            stats->converged = 1;
+            return HM_OK;
+        }
+    }
     int kb = K;  // length of the next batch
     while (true) {
         const int k = std::min(kb, maxiter - enqueued);
@@ -956,8 +984,19 @@ int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x
             }
         }
         const int rep_iter = rep_at >= 0 ? enqueued + rep_at + 1 : -1;
+        const bool first_read = enqueued == 0;
         enqueued += k;
         HM_TRY(st.pull(s));
+        if (first_read) {
+            norm_b = st.host->norm_b;
+            stats->norm_b = norm_b;
+            stats->norm_r = norm_b;
+            if (!(norm_b > 0.0)) {  // x = 0 solves b = 0 (the solve froze at once)
+                stats->converged = 1;
+                return HM_OK;
+            }
+            rr_replaced = rr_prev = norm_b * norm_b;
+        }
         if (st.host->nonfinite) {
             set_error("non-finite value in the mixed solve");
             return HM_ERR_VALUE;
@@ -1201,7 +1240,13 @@ int vc_create(hm_cf* cf, hm_dia* dia, int h, int w, int pre, int post, double om
     rc = dev_alloc(reinterpret_cast<void**>(&vc->dense_inv64), sizeof(double) * last->n * last->n);
     if (rc == HM_OK)
         rc = dev_alloc(reinterpret_cast<void**>(&vc->dense_inv32), sizeof(float) * last->n * last->n);
-    if (rc == HM_OK && last->n > 0) rc = dense_inverse(last, vc->dense_inv64, vc->dense_inv32, s);
+    // The inverse's singularity flag is read with the set-up's closing
+    // synchronisation rather than with one of its own.
+    int* gj_status = nullptr;
+    if (rc == HM_OK && last->n > 0)
+        rc = dev_alloc(reinterpret_cast<void**>(&gj_status), sizeof(int));
+    if (rc == HM_OK && last->n > 0)
+        rc = dense_inverse(last, vc->dense_inv64, vc->dense_inv32, s, gj_status);
     if (rc == HM_OK) rc = make_reducer(&vc->red, reducer_blocks(n0, h, w), 2);
     if (rc == HM_OK && vc_tail_start(vc) < vc->levels.size() - 1)
         rc = prec == HM_F64 ? build_tail_pack<double>(vc, vc_tail_start(vc), s)
@@ -1212,8 +1257,26 @@ int vc_create(hm_cf* cf, hm_dia* dia, int h, int w, int pre, int post, double om
     if (HM_TAIL_MATRIX && rc == HM_OK && vc_tail_start(vc) < vc->levels.size() - 1)
         rc = prec == HM_F64 ? build_tail_matrix<double>(vc, vc_tail_start(vc), s)
                             : build_tail_matrix<float>(vc, vc_tail_start(vc), s);
+    int singular = 0;
+    if (rc == HM_OK && gj_status) {
+        const hipError_t e =
+            hipMemcpyAsync(&singular, gj_status, sizeof(int), hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) {
+            set_error("hipMemcpyAsync: %s", hipGetErrorString(e));
+            rc = HM_ERR_HIP;
+        }
+    }
+    const hipError_t es_ = hipStreamSynchronize(s);
+    dev_free(gj_status);
+    if (rc == HM_OK && es_ != hipSuccess) {
+        set_error("hipStreamSynchronize: %s", hipGetErrorString(es_));
+        rc = HM_ERR_HIP;
+    }
+    if (rc == HM_OK && singular) {
+        set_error("coarsest-grid matrix is singular");
+        rc = HM_ERR_VALUE;
+    }
     if (rc != HM_OK) { vc_free(vc); return rc; }
-    HM_HIP(hipStreamSynchronize(s));
     *out = vc;
     return HM_OK;
 }

@@ -192,7 +192,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dot(const T* __restrict__ a, const T* __restrict__ b,
                                                 const T* __restrict__ c, long long n, CgState* st,
                                                 Reducer red, int kind) {
-    if (kind != kDotAux && solve_done(st)) return;
+    if (kind != kDotAux && kind < kDotNormB && solve_done(st)) return;
     double acc[2] = {0.0, 0.0};
     const long long base = (long long)blockIdx.x * kBlock * kVecPerThread + threadIdx.x;
 #pragma unroll
@@ -214,7 +214,7 @@ int launch_dot(const T* a, const T* b, const T* c, long long n, CgState* st, con
     k_dot<T><<<g, kBlock, 0, s>>>(a, b, c, n, st, red, kind);
     HM_LAUNCH_CHECK();
     return launch_finalize(c ? 2 : 1, red, g, st, c ? kDotReorth : kind,
-                           kind == kDotAux ? nullptr : st, s);
+                           kind == kDotAux || kind >= kDotNormB ? nullptr : st, s);
 }
 
 template <typename T>

@@ -82,11 +82,21 @@ def sanity_check_image(image):
                       "np.float32 or np.float64 instead?" % name, stacklevel=3)
 
 
-def device_classes(trimap, fg_threshold=0.9, bg_threshold=0.1):
+def _check_counts(n_fg, n_bg, fg_threshold, bg_threshold):
+    if n_bg == 0:
+        raise ValueError("Trimap did not contain background values (values <= %f)" % bg_threshold)
+    if n_fg == 0:
+        raise ValueError("Trimap did not contain foreground values (values >= %f)" % fg_threshold)
+
+
+def device_classes(trimap, fg_threshold=0.9, bg_threshold=0.1, deferred=False):
     """Device class map (0 unknown, 1 fg, 2 bg) and the device trimap.
 
     Raises ValueError like trimap_split when the trimap has no foreground or
-    no background pixels.
+    no background pixels. With deferred=True it returns (classes, trimap,
+    check) instead: the counts travel to pinned host memory behind the
+    stream's work and check() raises those errors once they have arrived, so
+    that device work enqueued before check() overlaps the wait.
     """
     t = as_device(trimap, torch.float64).reshape(-1)
     n = t.numel()
@@ -95,9 +105,16 @@ def device_classes(trimap, fg_threshold=0.9, bg_threshold=0.1):
     L = _ffi.lib()
     _ffi.check(L.hm_trimap_classes(_ffi.ptr(t), n, fg_threshold, bg_threshold, _ffi.ptr(cls),
                                    _ffi.ptr(counts), _ffi.stream()))
+    if deferred:
+        host = torch.empty(2, dtype=torch.int64, pin_memory=True)
+        host.copy_(counts, non_blocking=True)
+        arrived = torch.cuda.Event()
+        arrived.record()
+
+        def check():
+            arrived.synchronize()
+            _check_counts(int(host[0]), int(host[1]), fg_threshold, bg_threshold)
+        return cls, t, check
     n_fg, n_bg = (int(v) for v in counts.cpu())
-    if n_bg == 0:
-        raise ValueError("Trimap did not contain background values (values <= %f)" % bg_threshold)
-    if n_fg == 0:
-        raise ValueError("Trimap did not contain foreground values (values >= %f)" % fg_threshold)
+    _check_counts(n_fg, n_bg, fg_threshold, bg_threshold)
     return cls, t
