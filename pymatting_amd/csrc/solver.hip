@@ -1037,11 +1037,21 @@ int pcg_reliable(hm_cf* cf, hm_dia* dia, hm_vc* vc32, const double* b, double* x
         // Iterations from now until ||r||^2 < delta^2 rr_replaced.
         const double target = delta * delta * rr_replaced;
         rep_idx = -1;
+        int j_rep = -1;
         if (rr < target) {
-            rep_idx = 0;
+            j_rep = 1;
         } else if (rho2 > 0.0 && rho2 < 1.0) {
-            const double j = std::ceil(std::log(target / rr) / std::log(rho2));
-            if (j <= kb) rep_idx = std::max(1, (int)j) - 1;
+            j_rep = (int)std::min<double>(
+                1e9, std::max(1.0, std::ceil(std::log(target / rr) / std::log(rho2))));
+        }
+        if (j_rep > 0 && j_rep <= kb) {
+            rep_idx = j_rep - 1;
+            // A batch holds one replacement: it ends before the next one is
+            // due, delta^2 further down at the measured rate.
+            int gap = K;
+            if (rho2 > 0.0 && rho2 < 1.0)
+                gap = (int)std::max(1.0, std::ceil(std::log(delta * delta) / std::log(rho2)));
+            kb = std::min(kb, j_rep + gap - 1);
         }
     }
     stats->iterations = st.host->iters;
