@@ -452,6 +452,27 @@ __global__ __launch_bounds__(kFinBlock) void k_cg_update_pq(TX* __restrict__ x,
                                                             CgState* st, Reducer red_pq,
                                                             unsigned pq_blocks, Reducer red) {
     if (solve_done(st)) return;
+    constexpr int U = 4;
+    const long long stride = (long long)gridDim.x * kFinBlock;
+    const long long first = (long long)blockIdx.x * kFinBlock + threadIdx.x;
+    TX xv[U];
+    T pv[U], rv[U], qv[U];
+    const auto load = [&](long long i0) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long i = i0 + u * stride;
+            const bool in = i < n;
+            if constexpr (kUpdX) {
+                xv[u] = in ? x[i] : TX(0);
+                pv[u] = in ? p[i] : T(0);
+            }
+            rv[u] = in ? r[i] : T(0);
+            qv[u] = in ? q[i] : T(0);
+        }
+    };
+    // The first chunk's loads go out before the partial sums of alpha, so
+    // both latencies overlap.
+    load(first);
     __shared__ double s_alpha;
     {
         const double* src[1] = {red_pq.partials};
@@ -466,24 +487,9 @@ __global__ __launch_bounds__(kFinBlock) void k_cg_update_pq(TX* __restrict__ x,
     }
     const double alpha = s_alpha;
     const T alphaT = (T)alpha;
-    constexpr int U = 4;
-    const long long stride = (long long)gridDim.x * kFinBlock;
     double acc = 0.0;
-    for (long long i0 = (long long)blockIdx.x * kFinBlock + threadIdx.x; i0 < n;
-         i0 += U * stride) {
-        TX xv[U];
-        T pv[U], rv[U], qv[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const long long i = i0 + u * stride;
-            const bool in = i < n;
-            if constexpr (kUpdX) {
-                xv[u] = in ? x[i] : TX(0);
-                pv[u] = in ? p[i] : T(0);
-            }
-            rv[u] = in ? r[i] : T(0);
-            qv[u] = in ? q[i] : T(0);
-        }
+    for (long long i0 = first; i0 < n; i0 += U * stride) {
+        if (i0 != first) load(i0);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const long long i = i0 + u * stride;
