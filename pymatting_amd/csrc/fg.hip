@@ -124,10 +124,30 @@ __device__ __forceinline__ void fg_solve_pixel(const S (&da)[4], const S* b0c, c
     }
 }
 
+// Lane i gets lane i-1's value (shfl_up1) or lane i+1's (shfl_down1), the
+// first / last lane keeps its own, as __shfl_up / __shfl_down by 1 over the
+// wave; done with the wavefront-shift DPP moves (one VALU instruction per
+// 32-bit word) instead of an LDS permute, since the sweep's step latency
+// runs through them.
+template <int Ctrl>
+__device__ __forceinline__ float dpp_wave_move(float v) {
+    const int b = __builtin_bit_cast(int, v);
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(b, b, Ctrl, 0xf, 0xf, false));
+}
+template <int Ctrl>
+__device__ __forceinline__ double dpp_wave_move(double v) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = (int)b, hi = (int)(b >> 32);
+    const int rlo = __builtin_amdgcn_update_dpp(lo, lo, Ctrl, 0xf, 0xf, false);
+    const int rhi = __builtin_amdgcn_update_dpp(hi, hi, Ctrl, 0xf, 0xf, false);
+    return __builtin_bit_cast(double,
+                              (long long)(((unsigned long long)(unsigned)rhi << 32) | (unsigned)rlo));
+}
+constexpr int kDppWaveShr1 = 0x138, kDppWaveShl1 = 0x130;
 template <typename S>
-__device__ __forceinline__ S shfl_up1(S v) { return __shfl_up(v, 1, kWave); }
+__device__ __forceinline__ S shfl_up1(S v) { return dpp_wave_move<kDppWaveShr1>(v); }
 template <typename S>
-__device__ __forceinline__ S shfl_down1(S v) { return __shfl_down(v, 1, kWave); }
+__device__ __forceinline__ S shfl_down1(S v) { return dpp_wave_move<kDppWaveShl1>(v); }
 
 // Tagged halo words: the value's bit pattern with the sweep tag (1..3) in its
 // two top bits.
@@ -214,7 +234,10 @@ constexpr int kFgPad = kWave + 1;  // lane stride (conflict-free staging stores)
 
 template <typename S>
 struct FgChunk {
-    static constexpr int C = sizeof(S) == 4 ? 10 : 5;  // steps per chunk
+#ifndef HM_FG_CHUNK
+#define HM_FG_CHUNK 10
+#endif
+    static constexpr int C = sizeof(S) == 4 ? HM_FG_CHUNK : 5;  // steps per chunk
 };
 
 template <typename S>
@@ -245,18 +268,16 @@ __device__ __forceinline__ void fg_writeback(const FgStage<S>& st, int bf, int t
     }
 }
 
-// Stages chunk [t0, t0 + C) into buffer `bf`; run by the loader threads
-// (lt = threadIdx.x - kWave). Returns false if the wait for strip s-1 timed out.
+// Stages the rows above and below the strip for chunk [t0, t0 + C) into
+// buffer `bf`; run by the loader threads (lt = threadIdx.x - kFgSweepers).
+// Returns false if the wait for strip s-1 timed out.
 template <typename S>
-__device__ __forceinline__ bool fg_stage(FgStage<S>& st, int bf, int t0, int s, int lt,
-                                         const S* __restrict__ cs, const double* __restrict__ cd,
-                                         const S* F, const S* B, int h, int w, int last_lane,
-                                         const typename FgWord<S>::U* halo, unsigned tag,
-                                         long long& waited) {
+__device__ __forceinline__ bool fg_stage_rows(FgStage<S>& st, int bf, int t0, int s, int lt,
+                                              const S* F, const S* B, int h, int w,
+                                              int last_lane, const typename FgWord<S>::U* halo,
+                                              unsigned tag, long long& waited) {
     using W = typename FgWord<S>::U;
     constexpr int C = FgChunk<S>::C;
-    constexpr int U = (kWave * C + kFgLoaders - 1) / kFgLoaders;
-    const long long n = (long long)h * w;
     bool ok = true;
     // Row above the strip (new values, from strip s-1's halo words, waiting
     // for the current tag) and row below it (old values). Wave 1 only.
@@ -300,10 +321,30 @@ __device__ __forceinline__ bool fg_stage(FgStage<S>& st, int bf, int t0, int s, 
             }
         }
     }
-    // Per-lane inputs, U (lane, step) pairs per thread with all loads issued
-    // before the stores.
+    return ok;
+}
+
+// A loader thread's per-lane inputs of one chunk (U (lane, step) pairs),
+// held in registers between their loads and their LDS stores.
+template <typename S>
+struct FgLane {
+    static constexpr int C = FgChunk<S>::C;
+    static constexpr int U = (kWave * C + kFgLoaders - 1) / kFgLoaders;
     S sval[U][kFgS];
     double dval[U][kFgD];
+};
+
+// Loads chunk [t0, t0 + C)'s per-lane inputs (coefficients and the old F/B
+// one column to the right) into registers; issued a chunk before they are
+// stored, so their latency overlaps a whole chunk of sweeping.
+template <typename S>
+__device__ __forceinline__ void fg_load_lanes(FgLane<S>& L, int t0, int s, int lt,
+                                              const S* __restrict__ cs,
+                                              const double* __restrict__ cd, const S* F,
+                                              const S* B, int h, int w) {
+    constexpr int C = FgChunk<S>::C;
+    constexpr int U = FgLane<S>::U;
+    const long long n = (long long)h * w;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int q = lt + u * kFgLoaders;
@@ -318,26 +359,32 @@ __device__ __forceinline__ bool fg_stage(FgStage<S>& st, int bf, int t0, int s, 
         const long long i = row + xc;
         const long long pn = (row + (x + 1 >= 0 && x + 1 < w ? x + 1 : xn)) * 3;
 #pragma unroll
-        for (int f = 0; f < 10; ++f) sval[u][f] = cs[f * n + i];
+        for (int f = 0; f < 10; ++f) L.sval[u][f] = cs[f * n + i];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            sval[u][10 + c] = F[pn + c];
-            sval[u][13 + c] = B[pn + c];
+            L.sval[u][10 + c] = F[pn + c];
+            L.sval[u][13 + c] = B[pn + c];
         }
 #pragma unroll
-        for (int f = 0; f < kFgD; ++f) dval[u][f] = cd[f * n + i];
+        for (int f = 0; f < kFgD; ++f) L.dval[u][f] = cd[f * n + i];
     }
+}
+
+// Stores loaded per-lane inputs into buffer bf.
+template <typename S>
+__device__ __forceinline__ void fg_store_lanes(FgStage<S>& st, int bf, int lt,
+                                               const FgLane<S>& L) {
+    constexpr int C = FgChunk<S>::C;
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
+    for (int u = 0; u < FgLane<S>::U; ++u) {
         const int q = lt + u * kFgLoaders;
         if (q >= kWave * C) continue;
         const int y = q / C, j = q - y * C;
 #pragma unroll
-        for (int f = 0; f < kFgS; ++f) st.sv[bf][j][f][y] = sval[u][f];
+        for (int f = 0; f < kFgS; ++f) st.sv[bf][j][f][y] = L.sval[u][f];
 #pragma unroll
-        for (int f = 0; f < kFgD; ++f) st.dv[bf][j][f][y] = dval[u][f];
+        for (int f = 0; f < kFgD; ++f) st.dv[bf][j][f][y] = L.dval[u][f];
     }
-    return ok;
 }
 
 template <typename S>
@@ -372,8 +419,15 @@ __global__ __launch_bounds__(kFgThreads) void k_fg_sweep(const S* __restrict__ c
         cur_old[1] = B[row * 3 + ch];
         prev_res[0] = cur_old[0];
         prev_res[1] = cur_old[1];
-    } else {
-        ok = fg_stage<S>(st, 0, 0, s, lt, cs, cd, F, B, h, w, last_lane, halo, tag, waited);
+    }
+    // Loaders: a chunk's per-lane inputs are loaded into registers one chunk
+    // before they are stored into LDS.
+    FgLane<S> lanes;
+    if (!computer) {
+        fg_load_lanes<S>(lanes, 0, s, lt, cs, cd, F, B, h, w);
+        ok = fg_stage_rows<S>(st, 0, 0, s, lt, F, B, h, w, last_lane, halo, tag, waited);
+        fg_store_lanes<S>(st, 0, lt, lanes);
+        if (nchunks > 1) fg_load_lanes<S>(lanes, C, s, lt, cs, cd, F, B, h, w);
     }
     __syncthreads();
     for (int k = 0; k < nchunks; ++k) {
@@ -381,15 +435,25 @@ __global__ __launch_bounds__(kFgThreads) void k_fg_sweep(const S* __restrict__ c
         const int t0 = k * C;
         if (computer) {
             const int t1 = min(t0 + C, steps);
-            for (int t = t0; t < t1; ++t) {
-                const int j = t - t0;
+            // The chunk's steps unrolled, the last chunk's surplus steps
+            // predicated off (no branch), so each step's LDS reads, which do
+            // not depend on earlier steps, can be issued ahead of the step
+            // chain.
+#pragma unroll
+            for (int j = 0; j < C; ++j) {
+                const int t = t0 + j;
+                const bool step_ok = t < t1;  // uniform across the wave
                 const int x = t - lane;
-                const bool active = row_ok && x >= 0 && x < w;
+                const bool active = step_ok && row_ok && x >= 0 && x < w;
                 const int xc = x < 0 ? 0 : (x >= w ? w - 1 : x);
                 const S* sv = &st.sv[bf][j][0][lane];
                 // This channel's fields: F at q = ch, B at q = 3 + ch.
                 const int qf = ch, qb = 3 + ch;
                 S next_old[2] = {sv[(10 + qf) * kFgPad], sv[(10 + qb) * kFgPad]};
+                // The staged rows above and below the strip (read by every
+                // lane, used by the first and last).
+                const S up_s[2] = {st.up[bf][j][qf], st.up[bf][j][qb]};
+                const S dn_s[2] = {st.dn[bf][j][qf], st.dn[bf][j][qb]};
                 S left[2], right[2], up[2], down[2];
                 S from_above[2], from_below[2];
 #pragma unroll
@@ -401,26 +465,8 @@ __global__ __launch_bounds__(kFgThreads) void k_fg_sweep(const S* __restrict__ c
                 for (int v = 0; v < 2; ++v) {
                     left[v] = x > 0 ? prev_res[v] : cur_old[v];
                     right[v] = x + 1 < w ? next_old[v] : cur_old[v];
-                }
-                if (r == 0) {
-                    up[0] = cur_old[0];
-                    up[1] = cur_old[1];
-                } else if (lane > 0) {
-                    up[0] = from_above[0];
-                    up[1] = from_above[1];
-                } else {
-                    up[0] = st.up[bf][j][qf];
-                    up[1] = st.up[bf][j][qb];
-                }
-                if (r >= h - 1) {
-                    down[0] = cur_old[0];
-                    down[1] = cur_old[1];
-                } else if (lane < last_lane) {
-                    down[0] = from_below[0];
-                    down[1] = from_below[1];
-                } else {
-                    down[0] = st.dn[bf][j][qf];
-                    down[1] = st.dn[bf][j][qb];
+                    up[v] = r == 0 ? cur_old[v] : (lane > 0 ? from_above[v] : up_s[v]);
+                    down[v] = r >= h - 1 ? cur_old[v] : (lane < last_lane ? from_below[v] : dn_s[v]);
                 }
                 // The 2x2 solve (estimate_foreground_ml.py:158-175).
                 S da[4];
@@ -432,8 +478,10 @@ __global__ __launch_bounds__(kFgThreads) void k_fg_sweep(const S* __restrict__ c
                 S res[2];
                 fg_solve_channel<S>(da, sv[(4 + ch) * kFgPad], sv[(7 + ch) * kFgPad], b00, b01,
                                     b11, left, right, up, down, res);
-                st.out[bf][j][qf][lane] = res[0];
-                st.out[bf][j][qb][lane] = res[1];
+                if (step_ok) {
+                    st.out[bf][j][qf][lane] = res[0];
+                    st.out[bf][j][qb][lane] = res[1];
+                }
                 if (active) {
                     if (lane == last_lane && r + 1 < h) {
                         // The strip's last row, tagged, for strip s+1.
@@ -446,15 +494,19 @@ __global__ __launch_bounds__(kFgThreads) void k_fg_sweep(const S* __restrict__ c
                 }
 #pragma unroll
                 for (int v = 0; v < 2; ++v) {
-                    prev_res[v] = res[v];
-                    cur_old[v] = next_old[v];
+                    prev_res[v] = step_ok ? res[v] : prev_res[v];
+                    cur_old[v] = step_ok ? next_old[v] : cur_old[v];
                 }
             }
         } else {
             if (k > 0) fg_writeback<S>(st, bf ^ 1, t0 - C, s, lt, F, B, h, w);
-            if (k + 1 < nchunks)
-                ok = fg_stage<S>(st, bf ^ 1, t0 + C, s, lt, cs, cd, F, B, h, w, last_lane, halo,
-                                 tag, waited) && ok;
+            if (k + 1 < nchunks) {
+                ok = fg_stage_rows<S>(st, bf ^ 1, t0 + C, s, lt, F, B, h, w, last_lane, halo,
+                                      tag, waited) && ok;
+                fg_store_lanes<S>(st, bf ^ 1, lt, lanes);
+                if (k + 2 < nchunks)
+                    fg_load_lanes<S>(lanes, t0 + 2 * C, s, lt, cs, cd, F, B, h, w);
+            }
         }
         __syncthreads();
     }
